@@ -1,0 +1,579 @@
+// C ABI of libblink_mi355x.so: argument checking, context state (twiddle
+// tables, scratch buffers, captured graphs) and the orchestration of the
+// kernels in blink_kernels.hip.
+#include "../../include/blink_mi355x.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "blink_kernels.hpp"
+
+using namespace blink;
+
+namespace {
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct GraphEntry {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+}  // namespace
+
+struct blink_ctx {
+    int device = 0;
+    hipDeviceProp_t props{};
+    std::string error;
+    std::map<int, float*> twiddles;               // N -> device table of N float2
+    std::map<std::string, DeviceBuffer> scratch;  // named grow-only buffers
+    std::map<std::string, GraphEntry> graphs;     // descriptor bytes -> graph
+    hipStream_t capture_stream = nullptr;
+    bool use_graphs = true;
+
+    int fail(int code, const char* fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        error = buf;
+        return code;
+    }
+};
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ctx->fail(BLINK_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_ARG(cond, ...)                                                \
+    do {                                                                    \
+        if (!(cond)) return ctx->fail(BLINK_ERR_INVALID_ARGUMENT, __VA_ARGS__); \
+    } while (0)
+
+namespace {
+
+int set_device(blink_ctx* ctx) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    return BLINK_OK;
+}
+
+void drop_graphs(blink_ctx* ctx) {
+    for (auto& kv : ctx->graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    ctx->graphs.clear();
+}
+
+// Grow-only named scratch buffer.  Growing one invalidates every captured
+// graph (they hold the old address), so those are dropped first.
+int scratch(blink_ctx* ctx, const char* name, size_t bytes, void** out) {
+    DeviceBuffer& b = ctx->scratch[name];
+    if (b.bytes < bytes) {
+        if (b.ptr) {
+            HIP_TRY(hipDeviceSynchronize());
+            drop_graphs(ctx);
+            HIP_TRY(hipFree(b.ptr));
+        }
+        b.ptr = nullptr;
+        b.bytes = 0;
+        HIP_TRY(hipMalloc(&b.ptr, bytes));
+        b.bytes = bytes;
+    }
+    *out = b.ptr;
+    return BLINK_OK;
+}
+
+// tw[m] = exp(+2 pi i m / N), computed in double, rounded to FP32.
+int twiddles(blink_ctx* ctx, int n, const float** out) {
+    auto it = ctx->twiddles.find(n);
+    if (it != ctx->twiddles.end()) {
+        *out = it->second;
+        return BLINK_OK;
+    }
+    std::vector<float> h(2 * (size_t)n);
+    for (int m = 0; m < n; m++) {
+        const double a = 2.0 * M_PI * (double)m / (double)n;
+        h[2 * m] = (float)std::cos(a);
+        h[2 * m + 1] = (float)std::sin(a);
+    }
+    float* d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(float) * h.size()));
+    HIP_TRY(hipMemcpy(d, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    ctx->twiddles[n] = d;
+    *out = d;
+    return BLINK_OK;
+}
+
+int check_fft_size(blink_ctx* ctx, int n, FftPlan& plan) {
+    if (!make_fft_plan(n, plan))
+        return ctx->fail(BLINK_ERR_UNSUPPORTED,
+                         "grid side %d is not supported (need 2 <= N <= %d with prime factors 2, 3, 5)",
+                         n, kMaxPixels);
+    return BLINK_OK;
+}
+
+// Rows then columns, optionally with the shift+scale epilogue on the column
+// store.  in -> out (in == out allowed).
+int fft2d(blink_ctx* ctx, const float* in, float* out, int n, int batch, int sign, bool shift,
+          const double* sums, hipStream_t stream) {
+    FftPlan plan;
+    int rc = check_fft_size(ctx, n, plan);
+    if (rc) return rc;
+    const float* tw;
+    if ((rc = twiddles(ctx, n, &tw))) return rc;
+    HIP_TRY(launch_fft_rows(in, out, plan, (int64_t)batch * n, sign, tw, stream));
+    // The shifted store is fused into the in-place column pass when a
+    // workgroup can hold a column block and its shift partner; otherwise a
+    // separate shift+norm pass follows.
+    const bool fused = shift && cols_paired_fits(n);
+    ColArgs a{};
+    a.in = (const float2*)out;
+    a.out = (float2*)out;
+    a.in_img_stride = a.out_img_stride = (int64_t)n * n;
+    a.out_img_offset = 0;
+    a.paired = fused ? 1 : 0;
+    a.shift = fused ? 1 : 0;
+    a.sums = fused ? sums : nullptr;
+    a.n_ch = 1;
+    HIP_TRY(launch_fft_cols(a, plan, batch, sign, tw, stream));
+    if (shift && !fused) HIP_TRY(launch_shift_norm(out, n, n, batch, sums, batch, stream));
+    return BLINK_OK;
+}
+
+int check_vis_shape(blink_ctx* ctx, int n_ant, int n_ch, int n_t) {
+    CHECK_ARG(n_ant >= 1 && n_ch >= 0 && n_t >= 0, "invalid shape n_ant=%d n_ch=%d n_t=%d", n_ant,
+              n_ch, n_t);
+    const int64_t n_bl = (int64_t)n_ant * (n_ant + 1) / 2;
+    CHECK_ARG(n_bl < kMaxBaselines, "n_ant=%d gives %lld baselines, above the supported %d", n_ant,
+              (long long)n_bl, kMaxBaselines);
+    return BLINK_OK;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+
+extern "C" {
+
+BLINK_API const char* blink_version(void) { return "blink-mi355x 0.1 (gfx950)"; }
+
+BLINK_API int blink_fft_supported(int n_pixels) {
+    FftPlan p;
+    return make_fft_plan(n_pixels, p) ? 1 : 0;
+}
+
+BLINK_API int blink_create(int device, blink_ctx** out) {
+    if (!out) return BLINK_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0) return BLINK_ERR_HIP;
+    if (device < 0 || device >= n_dev) return BLINK_ERR_INVALID_ARGUMENT;
+    blink_ctx* ctx = new blink_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipGetDeviceProperties(&ctx->props, device) != hipSuccess) {
+        delete ctx;
+        return BLINK_ERR_HIP;
+    }
+    // Dynamic LDS per workgroup: gfx950 has 160 KiB per CU; keep 16 KiB for
+    // the kernels' static LDS and headroom.
+    int64_t lds = (int64_t)ctx->props.sharedMemPerBlock;
+    if (lds <= 0) lds = 64 * 1024;
+    g_lds_limit = std::min<int64_t>(lds, 160 * 1024) - 4 * 1024;
+    g_lds_budget = g_lds_limit - 12 * 1024;
+    if (set_kernel_attributes() != hipSuccess) {
+        (void)hipGetLastError();
+    }
+    if (hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return BLINK_ERR_HIP;
+    }
+    const char* ng = getenv("BLINK_NO_GRAPH");
+    ctx->use_graphs = !(ng && ng[0] == '1');
+    // Twiddle tables of the common grid sides, so that later calls do not
+    // allocate.
+    static const int common[] = {128, 180, 256, 512, 1024, 2048, 4096};
+    for (int n : common) {
+        const float* tw;
+        if (twiddles(ctx, n, &tw) != BLINK_OK) {
+            blink_destroy(ctx);
+            return BLINK_ERR_HIP;
+        }
+    }
+    *out = ctx;
+    return BLINK_OK;
+}
+
+BLINK_API void blink_destroy(blink_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    blink_release_cache(ctx);
+    for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+    if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+    delete ctx;
+}
+
+BLINK_API const char* blink_last_error(const blink_ctx* ctx) {
+    return ctx ? ctx->error.c_str() : "null context";
+}
+
+BLINK_API int blink_release_cache(blink_ctx* ctx) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    drop_graphs(ctx);
+    for (auto& kv : ctx->scratch)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    ctx->scratch.clear();
+    return BLINK_OK;
+}
+
+// ------------------------------------------------------------------ (a)
+
+static int phase_correct(blink_ctx* ctx, float* vis, int n_ant, int n_ch, int n_t,
+                         const float* w_m, const double* cable_m, const double* freq_hz,
+                         unsigned which, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    int rc = check_vis_shape(ctx, n_ant, n_ch, n_t);
+    if (rc) return rc;
+    CHECK_ARG(vis && freq_hz, "vis and freq_hz must be non-null");
+    CHECK_ARG(!(which & 1u) || w_m, "w_m must be non-null");
+    CHECK_ARG(!(which & 2u) || cable_m, "cable_m must be non-null");
+    if ((rc = set_device(ctx))) return rc;
+    HIP_TRY(launch_phase_correct(vis, n_ant, n_ch, n_t, w_m, cable_m, freq_hz, which,
+                                 (hipStream_t)stream));
+    return BLINK_OK;
+}
+
+BLINK_API int blink_apply_geometric_corrections(blink_ctx* ctx, float* vis, int n_ant, int n_ch,
+                                                int n_t, const float* w_m, const double* freq_hz,
+                                                void* stream) {
+    return phase_correct(ctx, vis, n_ant, n_ch, n_t, w_m, nullptr, freq_hz, 1u, stream);
+}
+
+BLINK_API int blink_apply_cable_corrections(blink_ctx* ctx, float* vis, int n_ant, int n_ch,
+                                            int n_t, const double* cable_m, const double* freq_hz,
+                                            void* stream) {
+    return phase_correct(ctx, vis, n_ant, n_ch, n_t, nullptr, cable_m, freq_hz, 2u, stream);
+}
+
+// ------------------------------------------------------------------ (b)
+
+BLINK_API int blink_uv_cells(blink_ctx* ctx, const float* u_m, const float* v_m,
+                             const int* bl_flags, int n_bl, const double* freq_hz, int n_ch,
+                             double delta_u, double delta_v, double min_uv, int n_pixels,
+                             int* cells, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(n_bl >= 0 && n_ch >= 0 && n_pixels >= 1, "invalid sizes");
+    CHECK_ARG(u_m && v_m && bl_flags && freq_hz && cells, "null pointer argument");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    HIP_TRY(launch_uv_cells(u_m, v_m, bl_flags, n_bl, freq_hz, n_ch, delta_u, delta_v, min_uv,
+                            n_pixels, cells, (hipStream_t)stream));
+    return BLINK_OK;
+}
+
+// Builds the per-channel row plan into context scratch.
+static int build_plan(blink_ctx* ctx, const float* u_m, const float* v_m, const int* flags,
+                      int n_bl, const double* freq_hz, int n_ch, double du, double dv,
+                      double min_uv, int n_pixels, int** row_start, uint32_t** entries,
+                      hipStream_t stream, const char* tag) {
+    void *rs, *en;
+    int rc;
+    std::string a = std::string("plan_rows") + tag, b = std::string("plan_entries") + tag;
+    if ((rc = scratch(ctx, a.c_str(), sizeof(int) * (size_t)n_ch * (n_pixels + 1), &rs))) return rc;
+    if ((rc = scratch(ctx, b.c_str(), sizeof(uint32_t) * 2 * (size_t)n_ch * std::max(n_bl, 1), &en)))
+        return rc;
+    HIP_TRY(launch_plan_rows(u_m, v_m, flags, n_bl, freq_hz, n_ch, du, dv, min_uv, n_pixels,
+                             (int*)rs, (uint32_t*)en, stream));
+    *row_start = (int*)rs;
+    *entries = (uint32_t*)en;
+    return BLINK_OK;
+}
+
+BLINK_API int blink_gridding(blink_ctx* ctx, const float* vis, int n_ant, int n_ch, int n_t,
+                             const float* u_m, const float* v_m, const int* bl_flags,
+                             const double* freq_hz, double delta_u, double delta_v, double min_uv,
+                             int n_pixels, int pol, int weighting, int algo, float* grids,
+                             float* counts, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    int rc = check_vis_shape(ctx, n_ant, n_ch, n_t);
+    if (rc) return rc;
+    CHECK_ARG(vis && u_m && v_m && bl_flags && freq_hz && grids && counts, "null pointer argument");
+    CHECK_ARG(pol >= 0 && pol <= 3, "pol must be 0..3 (XX, YY, I, V), got %d", pol);
+    CHECK_ARG(weighting == BLINK_WEIGHTING_NATURAL || weighting == BLINK_WEIGHTING_UNIFORM,
+              "weighting must be NATURAL or UNIFORM");
+    CHECK_ARG(algo == BLINK_GRID_TILED || algo == BLINK_GRID_ATOMIC, "unknown gridding algorithm");
+    CHECK_ARG(n_pixels >= 1 && n_pixels <= (algo == BLINK_GRID_TILED ? kMaxPixels : 65535),
+              "n_pixels=%d out of range", n_pixels);
+    if ((rc = set_device(ctx))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int n_bl = n_ant * (n_ant + 1) / 2;
+    const int n_img = n_ch * n_t;
+    if (algo == BLINK_GRID_ATOMIC) {
+        HIP_TRY(launch_grid_atomic(vis, n_ant, n_ch, n_t, u_m, v_m, bl_flags, freq_hz, delta_u,
+                                   delta_v, min_uv, n_pixels, pol, grids, counts, s));
+    } else {
+        int* rs;
+        uint32_t* en;
+        if ((rc = build_plan(ctx, u_m, v_m, bl_flags, n_bl, freq_hz, n_ch, delta_u, delta_v, min_uv,
+                             n_pixels, &rs, &en, s, "")))
+            return rc;
+        GatherArgs g{};
+        g.vis = (const float4*)vis;
+        g.freq_hz = freq_hz;
+        g.corrections = 0;
+        g.pol = pol;
+        g.n_bl = n_bl;
+        g.n_ch = n_ch;
+        HIP_TRY(launch_grid_tiles(g, rs, en, n_img, n_pixels, grids, counts, s));
+    }
+    if (weighting == BLINK_WEIGHTING_UNIFORM)
+        HIP_TRY(launch_uniform_weight(grids, counts, (int64_t)n_img * n_pixels * n_pixels, s));
+    return BLINK_OK;
+}
+
+// ------------------------------------------------------------------ (c)
+
+BLINK_API int blink_fft2d_c2c(blink_ctx* ctx, const float* in, float* out, int n_pixels,
+                              int batch, int direction, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(in && out, "null pointer argument");
+    CHECK_ARG(batch >= 0, "batch must be >= 0");
+    CHECK_ARG(direction == BLINK_FFT_BACKWARD || direction == BLINK_FFT_FORWARD,
+              "direction must be BLINK_FFT_BACKWARD (+1) or BLINK_FFT_FORWARD (-1)");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if (batch == 0) return BLINK_OK;
+    return fft2d(ctx, in, out, n_pixels, batch, direction, false, nullptr, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ (d)
+
+BLINK_API int blink_vector_sum(blink_ctx* ctx, const float* values, int64_t image_size,
+                               int n_images, double* sums, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(values && sums, "null pointer argument");
+    CHECK_ARG(image_size >= 0 && n_images >= 0, "invalid sizes");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    HIP_TRY(launch_vector_sum(values, image_size, n_images, sums, (hipStream_t)stream));
+    return BLINK_OK;
+}
+
+BLINK_API int blink_fft_shift_and_norm(blink_ctx* ctx, float* images, int nx, int ny,
+                                       int n_images, const double* sums, int n_sums,
+                                       void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(images, "null pointer argument");
+    CHECK_ARG(nx >= 1 && ny >= 1 && n_images >= 0, "invalid sizes nx=%d ny=%d", nx, ny);
+    CHECK_ARG(!sums || n_sums >= 1, "n_sums must be >= 1 when sums are given");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    HIP_TRY(launch_shift_norm(images, nx, ny, n_images, sums, n_sums, (hipStream_t)stream));
+    return BLINK_OK;
+}
+
+BLINK_API int blink_image(blink_ctx* ctx, float* grids, float* counts, int n_pixels, int n_images,
+                          float* images, int zero_grids, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(grids && counts && images, "grids are not initialised and cannot be imaged");
+    CHECK_ARG(n_images >= 0, "n_images must be >= 0");
+    CHECK_ARG(images != grids, "images must not alias grids");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    FftPlan plan;
+    if ((rc = check_fft_size(ctx, n_pixels, plan))) return rc;
+    if (n_images == 0) return BLINK_OK;
+    hipStream_t s = (hipStream_t)stream;
+    void* sums;
+    if ((rc = scratch(ctx, "image_sums", sizeof(double) * (size_t)n_images, &sums))) return rc;
+    const int64_t gs = (int64_t)n_pixels * n_pixels;
+    HIP_TRY(launch_vector_sum(counts, gs, n_images, (double*)sums, s));
+    if ((rc = fft2d(ctx, grids, images, n_pixels, n_images, BLINK_FFT_BACKWARD, true,
+                    (const double*)sums, s)))
+        return rc;
+    if (zero_grids) {
+        HIP_TRY(launch_zero(grids, (int64_t)sizeof(float) * 2 * gs * n_images, s));
+        HIP_TRY(launch_zero(counts, (int64_t)sizeof(float) * gs * n_images, s));
+    }
+    return BLINK_OK;
+}
+
+BLINK_API int blink_image_averaging(blink_ctx* ctx, const float* images, int64_t image_size,
+                                    int n_images, int real_only, float* avg, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(images && avg, "null pointer argument");
+    CHECK_ARG(image_size >= 0 && n_images >= 1, "invalid sizes");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    HIP_TRY(launch_average(images, image_size, n_images, real_only, avg, (hipStream_t)stream));
+    return BLINK_OK;
+}
+
+// ------------------------------------------------------------ batched
+
+// Enqueues the fused pipeline of every set on `s`.  All scratch must have
+// been reserved beforehand (see reserve_batch) so that this can be captured.
+struct SetScratch {
+    int* row_start;
+    uint32_t* entries;
+    uint32_t* isums;
+    float* inter;
+    int chunk;
+};
+
+static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
+                         std::vector<SetScratch>& out) {
+    out.resize(n_sets);
+    int rc;
+    // Row-pass buffer: a chunk of images whose row-transformed grids stay
+    // well inside the 256 MiB Infinity Cache between the two passes.
+    const int64_t inter_budget = 96ll << 20;
+    size_t inter_bytes = 0, rs_bytes = 0, en_bytes = 0, is_bytes = 0;
+    std::vector<size_t> rs_off(n_sets), en_off(n_sets), is_off(n_sets);
+    for (int i = 0; i < n_sets; i++) {
+        const blink_image_set& st = sets[i];
+        const int64_t n = st.n_pixels;
+        const int n_img = st.n_ch * st.n_t;
+        const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
+        const int64_t per = 8 * n * n;
+        int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
+        out[i].chunk = chunk;
+        inter_bytes = std::max(inter_bytes, (size_t)(per * chunk));
+        rs_off[i] = rs_bytes;
+        rs_bytes += sizeof(int) * (size_t)st.n_ch * (n + 1);
+        en_off[i] = en_bytes;
+        en_bytes += sizeof(uint32_t) * 2 * (size_t)st.n_ch * std::max(n_bl, 1);
+        is_off[i] = is_bytes;
+        is_bytes += sizeof(uint32_t) * (size_t)std::max(n_img, 1);
+    }
+    void *inter, *rs, *en, *is;
+    if ((rc = scratch(ctx, "batch_inter", std::max<size_t>(inter_bytes, 16), &inter))) return rc;
+    if ((rc = scratch(ctx, "batch_rows", std::max<size_t>(rs_bytes, 16), &rs))) return rc;
+    if ((rc = scratch(ctx, "batch_entries", std::max<size_t>(en_bytes, 16), &en))) return rc;
+    if ((rc = scratch(ctx, "batch_isums", std::max<size_t>(is_bytes, 16), &is))) return rc;
+    for (int i = 0; i < n_sets; i++) {
+        out[i].inter = (float*)inter;
+        out[i].row_start = (int*)((char*)rs + rs_off[i]);
+        out[i].entries = (uint32_t*)((char*)en + en_off[i]);
+        out[i].isums = (uint32_t*)((char*)is + is_off[i]);
+    }
+    for (int i = 0; i < n_sets; i++) {
+        const float* tw;
+        if ((rc = twiddles(ctx, sets[i].n_pixels, &tw))) return rc;
+    }
+    return BLINK_OK;
+}
+
+static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
+                         const std::vector<SetScratch>& sc, hipStream_t s) {
+    for (int i = 0; i < n_sets; i++) {
+        const blink_image_set& st = sets[i];
+        const int n = st.n_pixels;
+        const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
+        const int n_img = st.n_ch * st.n_t;
+        FftPlan plan;
+        make_fft_plan(n, plan);
+        const float* tw = ctx->twiddles[n];
+        HIP_TRY(launch_zero(sc[i].isums, sizeof(uint32_t) * (int64_t)n_img, s));
+        HIP_TRY(launch_plan_rows(st.u_m, st.v_m, st.bl_flags, n_bl, st.freq_hz, st.n_ch, st.delta_u,
+                                 st.delta_v, st.min_uv, n, sc[i].row_start, sc[i].entries, s));
+        GatherArgs g{};
+        g.vis = (const float4*)st.vis;
+        g.w_m = st.w_m;
+        g.cable_m = st.cable_m;
+        g.freq_hz = st.freq_hz;
+        g.corrections = st.corrections & 3u;
+        g.pol = st.pol;
+        g.n_bl = n_bl;
+        g.n_ch = st.n_ch;
+        const int64_t gs = (int64_t)n * n;
+        for (int img0 = 0; img0 < n_img; img0 += sc[i].chunk) {
+            const int cnt = std::min(sc[i].chunk, n_img - img0);
+            HIP_TRY(launch_grid_rows_fft(g, sc[i].row_start, sc[i].entries, plan, img0, cnt,
+                                         sc[i].inter, sc[i].isums, tw, s));
+            ColArgs a{};
+            a.in = (const float2*)sc[i].inter;
+            a.out = (float2*)(st.images + 2 * gs * img0);
+            a.in_img_stride = a.out_img_stride = gs;
+            a.out_img_offset = img0;
+            a.paired = 0;   // out of place: no shift partner needed
+            a.shift = 1;
+            a.isums = sc[i].isums;
+            a.row_start = sc[i].row_start;
+            a.n_ch = st.n_ch;
+            HIP_TRY(launch_fft_cols(a, plan, cnt, BLINK_FFT_BACKWARD, tw, s));
+        }
+        if (st.count_sums) HIP_TRY(launch_isums_to_double(sc[i].isums, st.count_sums, n_img, s));
+    }
+    return BLINK_OK;
+}
+
+BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
+                                void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(sets && n_sets >= 1, "need at least one image set");
+    for (int i = 0; i < n_sets; i++) {
+        const blink_image_set& st = sets[i];
+        int rc = check_vis_shape(ctx, st.n_ant, st.n_ch, st.n_t);
+        if (rc) return rc;
+        CHECK_ARG(st.vis && st.u_m && st.v_m && st.bl_flags && st.freq_hz && st.images,
+                  "set %d: null pointer argument", i);
+        CHECK_ARG(!(st.corrections & BLINK_CORR_GEOMETRIC) || st.w_m, "set %d: w_m is null", i);
+        CHECK_ARG(!(st.corrections & BLINK_CORR_CABLE) || st.cable_m, "set %d: cable_m is null", i);
+        CHECK_ARG(st.pol >= 0 && st.pol <= 3, "set %d: pol must be 0..3", i);
+        CHECK_ARG(st.n_pixels % 2 == 0, "set %d: the batched path needs an even grid side", i);
+        FftPlan plan;
+        if ((rc = check_fft_size(ctx, st.n_pixels, plan))) return rc;
+    }
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    std::vector<SetScratch> sc;
+    if ((rc = reserve_batch(ctx, sets, n_sets, sc))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (!ctx->use_graphs) return enqueue_batch(ctx, sets, n_sets, sc, s);
+
+    std::string key((const char*)sets, sizeof(blink_image_set) * (size_t)n_sets);
+    auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+        GraphEntry ge;
+        hipStream_t cs = ctx->capture_stream;
+        HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_batch(ctx, sets, n_sets, sc, cs);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(cs, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return ctx->fail(BLINK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        ge.graph = graph;
+        e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return ctx->fail(BLINK_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        it = ctx->graphs.emplace(key, ge).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second.exec, s));
+    return BLINK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,840 @@
+// gfx950 kernels of the visibility -> dirty-image path, with their host-side
+// launchers.  Every launcher validates the shape it is given against what the
+// kernel and its grid assume before it enqueues anything.
+#include "blink_kernels.hpp"
+
+#include <algorithm>
+
+namespace blink {
+
+int64_t g_lds_limit = 64 * 1024;
+int64_t g_lds_budget = 48 * 1024;
+
+// Launch-time guard: a kernel whose LDS request exceeds the device limit is
+// not launched.
+static bool lds_fits(size_t bytes) { return (int64_t)bytes <= g_lds_limit; }
+
+// ------------------------------------------------------------------ helpers
+
+__global__ void k_zero(float4* __restrict__ p, int64_t n_vec) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_vec; i += stride)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_zero_tail(float* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+static unsigned grid_for(int64_t work, int block, int max_blocks) {
+    int64_t b = (work + block - 1) / block;
+    if (b < 1) b = 1;
+    if (b > max_blocks) b = max_blocks;
+    return (unsigned)b;
+}
+
+hipError_t launch_zero(void* ptr, int64_t bytes, hipStream_t stream) {
+    if (bytes <= 0) return hipSuccess;
+    float* f = (float*)ptr;
+    const int64_t n_float = bytes / 4;
+    const bool aligned = ((uintptr_t)ptr % 16) == 0;
+    const int64_t n_vec = aligned ? n_float / 4 : 0;
+    if (n_vec > 0) {
+        hipLaunchKernelGGL(k_zero, dim3(grid_for(n_vec, 256, 8192)), dim3(256), 0, stream,
+                           (float4*)f, n_vec);
+    }
+    const int64_t tail = n_float - n_vec * 4;
+    if (tail > 0) {
+        hipLaunchKernelGGL(k_zero_tail, dim3(grid_for(tail, 256, 1 << 20)), dim3(256), 0, stream,
+                           f + n_vec * 4, tail);
+    }
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------- (a) phase corrections
+
+// One thread per (t, ch, bl) record: the 4 pol products (32 bytes) are
+// rotated by exp(i 2 pi turns) in place.
+__global__ void k_phase_correct(float4* __restrict__ vis, int n_bl, int n_ch, int64_t n_rec,
+                                const float* __restrict__ w_m, const double* __restrict__ cable_m,
+                                const double* __restrict__ freq_hz, unsigned which) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_rec; i += stride) {
+        const unsigned bl = (unsigned)(i % n_bl);
+        const int ch = (int)((i / n_bl) % n_ch);
+        const double nu = freq_hz[ch];
+        double turns = 0.0;
+        if (which & 1u) turns += reduce_turns(geometric_turns(w_m[bl], nu));
+        if (which & 2u) {
+            unsigned a1, a2;
+            baseline_antennas(bl, a1, a2);
+            turns += reduce_turns(cable_turns(cable_m[a2] - cable_m[a1], nu));
+        }
+        const float2 ph = unit_phasor(turns);
+        float4 a = vis[2 * i], b = vis[2 * i + 1];
+        float4 ra, rb;
+        ra.x = a.x * ph.x - a.y * ph.y;  ra.y = a.y * ph.x + a.x * ph.y;
+        ra.z = a.z * ph.x - a.w * ph.y;  ra.w = a.w * ph.x + a.z * ph.y;
+        rb.x = b.x * ph.x - b.y * ph.y;  rb.y = b.y * ph.x + b.x * ph.y;
+        rb.z = b.z * ph.x - b.w * ph.y;  rb.w = b.w * ph.x + b.z * ph.y;
+        vis[2 * i] = ra;
+        vis[2 * i + 1] = rb;
+    }
+}
+
+hipError_t launch_phase_correct(float* vis, int n_ant, int n_ch, int n_t, const float* w_m,
+                                const double* cable_m, const double* freq_hz, unsigned which,
+                                hipStream_t stream) {
+    const int n_bl = n_ant * (n_ant + 1) / 2;
+    const int64_t n_rec = (int64_t)n_bl * n_ch * n_t;
+    if (n_rec == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_phase_correct, dim3(grid_for(n_rec, 256, 65536)), dim3(256), 0, stream,
+                       (float4*)vis, n_bl, n_ch, n_rec, w_m, cable_m, freq_hz, which);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------- cell indices
+
+__global__ void k_uv_cells(const float* __restrict__ u_m, const float* __restrict__ v_m,
+                           const int* __restrict__ flags, int n_bl,
+                           const double* __restrict__ freq_hz, int n_ch, double du, double dv,
+                           double min_uv, int n_pixels, int4* __restrict__ cells) {
+    const int64_t n = (int64_t)n_bl * n_ch;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int bl = (int)(i % n_bl);
+        const int ch = (int)(i / n_bl);
+        int4 c = make_int4(-1, -1, -1, -1);
+        int up, vp;
+        if (!flags[bl] &&
+            cell_of(u_m[bl], v_m[bl], wavelength_of(freq_hz[ch]), du, dv, min_uv, n_pixels, up, vp)) {
+            c = make_int4(wrap_index(up, n_pixels), wrap_index(vp, n_pixels),
+                          wrap_index(-up, n_pixels), wrap_index(-vp, n_pixels));
+        }
+        cells[i] = c;
+    }
+}
+
+hipError_t launch_uv_cells(const float* u_m, const float* v_m, const int* flags, int n_bl,
+                           const double* freq_hz, int n_ch, double du, double dv, double min_uv,
+                           int n_pixels, int* cells, hipStream_t stream) {
+    const int64_t n = (int64_t)n_bl * n_ch;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_uv_cells, dim3(grid_for(n, 256, 65536)), dim3(256), 0, stream, u_m, v_m,
+                       flags, n_bl, freq_hz, n_ch, du, dv, min_uv, n_pixels, (int4*)cells);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------ (b) gridding, plain atomic
+
+// One thread per (t, ch, bl), FP32 global atomics (the plain path, in the
+// shape of gridding_kernel, src/gpu/gridding_gpu.cpp:58-124, with per-image
+// counts as the CPU path keeps them, src/pacer_imager.cpp:387,408).
+__global__ void k_grid_atomic(const float4* __restrict__ vis, int n_bl, int n_ch, int64_t n_rec,
+                              const float* __restrict__ u_m, const float* __restrict__ v_m,
+                              const int* __restrict__ flags, const double* __restrict__ freq_hz,
+                              double du, double dv, double min_uv, int n_pixels, int pol,
+                              float* __restrict__ grids, float* __restrict__ counts) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t gs = (int64_t)n_pixels * n_pixels;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_rec; i += stride) {
+        const int bl = (int)(i % n_bl);
+        const int64_t img = i / n_bl;
+        const int ch = (int)(img % n_ch);
+        if (flags[bl]) continue;
+        const float2 val = select_pol(vis[2 * i], vis[2 * i + 1], pol);
+        if (!vis_ok(val)) continue;
+        int up, vp;
+        if (!cell_of(u_m[bl], v_m[bl], wavelength_of(freq_hz[ch]), du, dv, min_uv, n_pixels, up, vp))
+            continue;
+        const int64_t p = img * gs + (int64_t)wrap_index(vp, n_pixels) * n_pixels + wrap_index(up, n_pixels);
+        const int64_t q = img * gs + (int64_t)wrap_index(-vp, n_pixels) * n_pixels + wrap_index(-up, n_pixels);
+        unsafeAtomicAdd(&grids[2 * p], val.x);
+        unsafeAtomicAdd(&grids[2 * p + 1], val.y);
+        unsafeAtomicAdd(&counts[p], 1.0f);
+        unsafeAtomicAdd(&grids[2 * q], val.x);
+        unsafeAtomicAdd(&grids[2 * q + 1], -val.y);
+        unsafeAtomicAdd(&counts[q], 1.0f);
+    }
+}
+
+hipError_t launch_grid_atomic(const float* vis, int n_ant, int n_ch, int n_t, const float* u_m,
+                              const float* v_m, const int* flags, const double* freq_hz, double du,
+                              double dv, double min_uv, int n_pixels, int pol, float* grids,
+                              float* counts, hipStream_t stream) {
+    const int n_bl = n_ant * (n_ant + 1) / 2;
+    const int64_t n_rec = (int64_t)n_bl * n_ch * n_t;
+    if (n_rec == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grid_atomic, dim3(grid_for(n_rec, 256, 65536)), dim3(256), 0, stream,
+                       (const float4*)vis, n_bl, n_ch, n_rec, u_m, v_m, flags, freq_hz, du, dv,
+                       min_uv, n_pixels, pol, grids, counts);
+    return hipGetLastError();
+}
+
+// --------------------------------------------- (b) gridding plan (row CSR)
+
+// Per channel: every gridded (baseline, conjugate) pair becomes two entries,
+// counting-sorted by grid row.  row_start[ch][0..N] (exclusive prefix sum of
+// the row histogram) and entries[ch][0..2*n_bl) packed as
+// u | conj << 13 | bl << 14.  One workgroup per channel.
+__global__ void __launch_bounds__(kPlanThreads)
+k_plan_rows(const float* __restrict__ u_m, const float* __restrict__ v_m,
+            const int* __restrict__ flags, int n_bl, const double* __restrict__ freq_hz,
+            double du, double dv, double min_uv, int n_pixels, int* __restrict__ row_start,
+            uint32_t* __restrict__ entries) {
+    extern __shared__ int plan_lds[];
+    int* hist = plan_lds;                 // [n_pixels]
+    int* cursor = plan_lds + n_pixels;    // [n_pixels]
+    __shared__ int warp_tot[kPlanThreads / 64];
+    const int ch = blockIdx.x;
+    const int tid = threadIdx.x;
+    const double wl = wavelength_of(freq_hz[ch]);
+    int* rs = row_start + (int64_t)ch * (n_pixels + 1);
+    uint32_t* ent = entries + (int64_t)ch * 2 * n_bl;
+
+    for (int i = tid; i < n_pixels; i += kPlanThreads) hist[i] = 0;
+    __syncthreads();
+    for (int bl = tid; bl < n_bl; bl += kPlanThreads) {
+        int up, vp;
+        if (flags[bl] || !cell_of(u_m[bl], v_m[bl], wl, du, dv, min_uv, n_pixels, up, vp)) continue;
+        atomicAdd(&hist[wrap_index(vp, n_pixels)], 1);
+        atomicAdd(&hist[wrap_index(-vp, n_pixels)], 1);
+    }
+    __syncthreads();
+    // Exclusive scan: each thread sums a contiguous chunk, the chunk totals
+    // are scanned across the block, then each chunk is rewritten.
+    const int chunk = (n_pixels + kPlanThreads - 1) / kPlanThreads;
+    const int c0 = min(tid * chunk, n_pixels), c1 = min(c0 + chunk, n_pixels);
+    int local = 0;
+    for (int i = c0; i < c1; i++) local += hist[i];
+    // inclusive scan of `local` across the block
+    const int lane = tid & 63, wid = tid >> 6;
+    int x = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) warp_tot[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        int t = (lane < kPlanThreads / 64) ? warp_tot[lane] : 0;
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(t, o, 64);
+            if (lane >= o) t += y;
+        }
+        if (lane < kPlanThreads / 64) warp_tot[lane] = t;
+    }
+    __syncthreads();
+    int run = x - local + (wid > 0 ? warp_tot[wid - 1] : 0);
+    for (int i = c0; i < c1; i++) {
+        const int h = hist[i];
+        rs[i] = run;
+        cursor[i] = run;
+        run += h;
+    }
+    if (tid == kPlanThreads - 1) rs[n_pixels] = warp_tot[kPlanThreads / 64 - 1];
+    __syncthreads();
+    for (int bl = tid; bl < n_bl; bl += kPlanThreads) {
+        int up, vp;
+        if (flags[bl] || !cell_of(u_m[bl], v_m[bl], wl, du, dv, min_uv, n_pixels, up, vp)) continue;
+        const int p = atomicAdd(&cursor[wrap_index(vp, n_pixels)], 1);
+        ent[p] = (uint32_t)wrap_index(up, n_pixels) | ((uint32_t)bl << 14);
+        const int q = atomicAdd(&cursor[wrap_index(-vp, n_pixels)], 1);
+        ent[q] = (uint32_t)wrap_index(-up, n_pixels) | (1u << 13) | ((uint32_t)bl << 14);
+    }
+}
+
+hipError_t launch_plan_rows(const float* u_m, const float* v_m, const int* flags, int n_bl,
+                            const double* freq_hz, int n_ch, double du, double dv, double min_uv,
+                            int n_pixels, int* row_start, uint32_t* entries, hipStream_t stream) {
+    if (n_ch == 0) return hipSuccess;
+    const size_t lds = sizeof(int) * 2 * (size_t)n_pixels;
+    if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k_plan_rows, dim3(n_ch), dim3(kPlanThreads), lds, stream, u_m, v_m, flags,
+                       n_bl, freq_hz, du, dv, min_uv, n_pixels, row_start, entries);
+    return hipGetLastError();
+}
+
+// Gathers the visibility of one plan entry (pol combination, optional phase
+// correction), or returns false when it is not finite / above MAX_VIS.
+__device__ __forceinline__ bool gather_entry(const GatherArgs& g, int64_t img, int ch, uint32_t e,
+                                             int& u_idx, float2& val) {
+    const unsigned bl = e >> 14;
+    u_idx = (int)(e & 0x1fffu);
+    const int64_t rec = img * g.n_bl + bl;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+    if (g.pol != 1) a = g.vis[2 * rec];
+    if (g.pol != 0) b = g.vis[2 * rec + 1];
+    float2 v = select_pol(a, b, g.pol);
+    if (g.corrections) {
+        const double nu = g.freq_hz[ch];
+        double turns = 0.0;
+        if (g.corrections & 1u) turns += reduce_turns(geometric_turns(g.w_m[bl], nu));
+        if (g.corrections & 2u) {
+            unsigned a1, a2;
+            baseline_antennas(bl, a1, a2);
+            turns += reduce_turns(cable_turns(g.cable_m[a2] - g.cable_m[a1], nu));
+        }
+        v = cmul(v, unit_phasor(turns));
+    }
+    if (!vis_ok(v)) return false;
+    if (e & (1u << 13)) v.y = -v.y;
+    val = v;
+    return true;
+}
+
+// Block-wide sum of one int per thread; the result is valid in thread 0.
+__device__ __forceinline__ int block_sum_int(int x, int* scratch) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) scratch[wid] = x;
+    __syncthreads();
+    int t = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += scratch[w];
+    return t;
+}
+
+// Locates the tile row of entry e given the tile's row starts rs[0..nr].
+__device__ __forceinline__ int row_of_entry(const int* rs, int nr, int e) {
+    int r = 0;
+    while (r + 1 < nr && rs[r + 1] <= e) r++;
+    return r;
+}
+
+// ------------------------------- (b) gridding, LDS-privatised row tiles
+
+// Workgroup = (row tile of RB rows, image).  The tile's plan entries are
+// accumulated with LDS atomics (complex grid and counts), then the tile is
+// added to the global grids/counts (each cell is owned by one workgroup, so
+// the flush needs no atomics).
+__global__ void __launch_bounds__(kTileThreads)
+k_grid_tiles(GatherArgs g, const int* __restrict__ row_start, const uint32_t* __restrict__ entries,
+             int n_pixels, int rb, float2* __restrict__ grids, float* __restrict__ counts) {
+    extern __shared__ float4 tiles_lds4[];
+    float2* tile = (float2*)tiles_lds4;                       // [rb][N]
+    float* cnt = (float*)(tile + (size_t)rb * n_pixels);      // [rb][N]
+    __shared__ int rs[kMaxTileRows + 1];
+    const int n_rb = (n_pixels + rb - 1) / rb;
+    const int tile_id = blockIdx.x % n_rb;
+    const int64_t img = blockIdx.x / n_rb;
+    const int ch = (int)(img % g.n_ch);
+    const int r0 = tile_id * rb;
+    const int nr = min(rb, n_pixels - r0);
+    const int* rsg = row_start + (int64_t)ch * (n_pixels + 1);
+    const uint32_t* ent = entries + (int64_t)ch * 2 * g.n_bl;
+    if (threadIdx.x <= nr) rs[threadIdx.x] = rsg[r0 + threadIdx.x];
+    __syncthreads();
+    if (rs[nr] == rs[0]) return;  // nothing lands in this tile
+    const int cells = nr * n_pixels;
+    for (int i = threadIdx.x; i < cells; i += blockDim.x) {
+        tile[i] = make_float2(0.f, 0.f);
+        cnt[i] = 0.f;
+    }
+    __syncthreads();
+    for (int e = rs[0] + threadIdx.x; e < rs[nr]; e += blockDim.x) {
+        int u_idx;
+        float2 v;
+        if (!gather_entry(g, img, ch, ent[e], u_idx, v)) continue;
+        const int r = row_of_entry(rs, nr, e);
+        const int c = r * n_pixels + u_idx;
+        atomicAdd(&tile[c].x, v.x);
+        atomicAdd(&tile[c].y, v.y);
+        atomicAdd(&cnt[c], 1.0f);
+    }
+    __syncthreads();
+    const int64_t base = img * (int64_t)n_pixels * n_pixels + (int64_t)r0 * n_pixels;
+    for (int i = threadIdx.x; i < cells; i += blockDim.x) {
+        const int r = i / n_pixels;
+        if (rs[r + 1] == rs[r]) continue;  // empty row: nothing to add
+        const float2 t = tile[i];
+        float2 o = grids[base + i];
+        o.x += t.x;
+        o.y += t.y;
+        grids[base + i] = o;
+        counts[base + i] += cnt[i];
+    }
+}
+
+hipError_t launch_grid_tiles(const GatherArgs& g, const int* row_start, const uint32_t* entries,
+                             int n_images, int n_pixels, float* grids, float* counts,
+                             hipStream_t stream) {
+    if (n_images == 0) return hipSuccess;
+    // 12 bytes per cell (complex + count) within the LDS budget.
+    int rb = (int)(g_lds_budget / (12 * (int64_t)n_pixels));
+    rb = std::max(1, std::min(rb, kMaxTileRows));
+    const int n_rb = (n_pixels + rb - 1) / rb;
+    const size_t lds = (size_t)rb * n_pixels * 12;
+    if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k_grid_tiles, dim3((unsigned)((int64_t)n_rb * n_images)), dim3(kTileThreads),
+                       lds, stream, g, row_start, entries, n_pixels, rb, (float2*)grids, counts);
+    return hipGetLastError();
+}
+
+// grid /= count per cell, the "U" weighting of src/pacer_imager.cpp:421-428.
+__global__ void k_uniform_weight(float2* __restrict__ grids, const float* __restrict__ counts,
+                                 int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float2 g = grids[i];
+        const float c = counts[i];
+        g.x = g.x / c;
+        g.y = g.y / c;
+        grids[i] = g;
+    }
+}
+
+hipError_t launch_uniform_weight(float* grids, const float* counts, int64_t n_cells,
+                                 hipStream_t stream) {
+    if (n_cells == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_uniform_weight, dim3(grid_for(n_cells, 256, 65536)), dim3(256), 0, stream,
+                       (float2*)grids, counts, n_cells);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------- (c) FFT
+
+// Row pass: each team loads one row from global memory, transforms it in LDS
+// and writes it back (in == out allowed: a row is fully read before written).
+template <int S, bool kBlockTeam>
+__global__ void __launch_bounds__(kFftThreads)
+k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out, FftPlan plan, int team,
+           int64_t n_rows, const float2* __restrict__ tw) {
+    extern __shared__ float4 rows_lds4[];
+    float2* lds = (float2*)rows_lds4;
+    const int n = plan.n;
+    const int teams = kBlockTeam ? 1 : (int)blockDim.x / team;
+    const int t_id = kBlockTeam ? 0 : (int)threadIdx.x / team;
+    const int lane = kBlockTeam ? (int)threadIdx.x : (int)threadIdx.x % team;
+    float2* buf = lds + (size_t)t_id * 2 * n;
+    float2* tmp = buf + n;
+    const int64_t row = (int64_t)blockIdx.x * teams + t_id;
+    const bool active = row < n_rows;
+    if (active) {
+        const float2* src = in + row * n;
+        for (int i = lane; i < n; i += team) buf[i] = src[i];
+    }
+    team_sync<kBlockTeam>();
+    if (kBlockTeam || active) {
+        const float2* res = fft_row<S, kBlockTeam>(buf, tmp, plan, tw, lane, team);
+        if (active) {
+            float2* dst = out + row * n;
+            for (int i = lane; i < n; i += team) dst[i] = res[i];
+        }
+    }
+}
+
+// Column pass.  A workgroup owns a set of columns: W contiguous columns
+// starting at c0 and, when `paired` is set, the W columns starting at
+// c0 + N/2 as well (then the fft_shift maps the set onto itself, which makes
+// an in-place shifted store safe).  The columns are loaded into an LDS tile
+// (transposed, one padded LDS row per column), transformed by the teams, and
+// stored row by row, optionally shifted and scaled.
+template <int S, bool kBlockTeam>
+__global__ void __launch_bounds__(kFftThreads)
+k_fft_cols(ColArgs a, FftPlan plan, int team, const float2* __restrict__ tw) {
+    extern __shared__ float4 cols_lds4[];
+    const int n = plan.n;
+    const int ld = n + 1;  // padded LDS row
+    const int wt = a.paired ? 2 * a.width : a.width;
+    float2* tile = (float2*)cols_lds4;                 // [wt][ld]
+    const int teams = kBlockTeam ? 1 : (int)blockDim.x / team;
+    const int t_id = kBlockTeam ? 0 : (int)threadIdx.x / team;
+    const int lane = kBlockTeam ? (int)threadIdx.x : (int)threadIdx.x % team;
+    float2* tmp = tile + (size_t)wt * ld + (size_t)t_id * n;
+    const int span = a.paired ? n / 2 : n;
+    const int n_cb = (span + a.width - 1) / a.width;
+    const int cb = blockIdx.x % n_cb;
+    const int64_t img = blockIdx.x / n_cb;
+    const int c0 = cb * a.width;
+    const int w_here = min(a.width, span - c0);
+    const float2* src = a.in + img * a.in_img_stride;
+    const int64_t oimg = a.out_img_offset + img;
+    float2* dst = a.out + img * a.out_img_stride;
+    const int* rs = a.row_start ? a.row_start + (oimg % a.n_ch) * (int64_t)(n + 1) : nullptr;
+
+    // load: element (r, w) of the set; columns c0+w and c0+N/2+w
+    const int total = n * wt;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int r = i / wt;
+        const int wl = i % wt;
+        const int half = wl / a.width;  // 0 or 1
+        const int w = wl % a.width;
+        float2 v = make_float2(0.f, 0.f);
+        if (w < w_here && (!rs || rs[r + 1] != rs[r]))
+            v = src[(int64_t)r * n + c0 + half * (n / 2) + w];
+        tile[(size_t)wl * ld + r] = v;
+    }
+    __syncthreads();
+    const int ncols = a.paired ? 2 * w_here : w_here;
+    const int rounds = kBlockTeam ? ncols : (ncols + teams - 1) / teams;
+    for (int k = 0; k < rounds; k++) {
+        const int cidx = kBlockTeam ? k : k * teams + t_id;
+        if (!kBlockTeam && cidx >= ncols) break;
+        const int wl = a.paired ? (cidx < w_here ? cidx : a.width + (cidx - w_here)) : cidx;
+        float2* col = tile + (size_t)wl * ld;
+        const float2* res = fft_row<S, kBlockTeam>(col, tmp, plan, tw, lane, team);
+        if (res != col) {
+            for (int i = lane; i < n; i += team) col[i] = res[i];
+            team_sync<kBlockTeam>();
+        }
+    }
+    __syncthreads();
+    float scale = 1.0f;
+    if (a.sums) scale = (float)(1.0 / a.sums[oimg]);
+    else if (a.isums) scale = (float)(1.0 / (double)a.isums[oimg]);
+    const bool do_scale = a.sums || a.isums;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int r = i / wt;
+        const int wl = i % wt;
+        const int half = wl / a.width;
+        const int w = wl % a.width;
+        if (w >= w_here) continue;
+        const int c = c0 + half * (n / 2) + w;
+        float2 v = tile[(size_t)wl * ld + r];
+        if (do_scale) v = cscale(v, scale);
+        const int orow = a.shift ? shift_index(r, n) : r;
+        const int ocol = a.shift ? shift_index(c, n) : c;
+        dst[(int64_t)orow * n + ocol] = v;
+    }
+}
+
+// ---------------------------------------------- (d) sums, shift+norm, averaging
+
+// sums[img] = sum of values of one image, in double.  One block per image.
+__global__ void __launch_bounds__(1024)
+k_vector_sum(const float* __restrict__ values, int64_t image_size, double* __restrict__ sums) {
+    __shared__ double part[16];
+    const float* v = values + (int64_t)blockIdx.x * image_size;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < image_size; i += blockDim.x) s += (double)v[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+hipError_t launch_vector_sum(const float* values, int64_t image_size, int n_images, double* sums,
+                             hipStream_t stream) {
+    if (n_images == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vector_sum, dim3(n_images), dim3(1024), 0, stream, values, image_size, sums);
+    return hipGetLastError();
+}
+
+// In-place fft_shift + scale.  Pixel p = (y, x) and its image q =
+// (shift(y), shift(x)) form a 2-cycle (or p == q); the smaller index swaps.
+__global__ void k_shift_norm(float2* __restrict__ img, int nx, int ny, int64_t n_total,
+                             const double* __restrict__ sums, int n_sums) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t per = (int64_t)nx * ny;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_total; i += stride) {
+        const int64_t im = i / per;
+        const int64_t p = i % per;
+        const int y = (int)(p / nx), x = (int)(p % nx);
+        const int64_t q = (int64_t)shift_index(y, ny) * nx + shift_index(x, nx);
+        if (q < p) continue;
+        const float s = sums ? (float)(1.0 / sums[im % n_sums]) : 1.0f;
+        float2* base = img + im * per;
+        const float2 a = base[p];
+        const float2 b = base[q];
+        if (q == p) {
+            base[p] = sums ? cscale(a, s) : a;
+        } else {
+            base[p] = sums ? cscale(b, s) : b;
+            base[q] = sums ? cscale(a, s) : a;
+        }
+    }
+}
+
+hipError_t launch_shift_norm(float* images, int nx, int ny, int n_images, const double* sums,
+                             int n_sums, hipStream_t stream) {
+    const int64_t n = (int64_t)nx * ny * n_images;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shift_norm, dim3(grid_for(n, 256, 65536)), dim3(256), 0, stream,
+                       (float2*)images, nx, ny, n, sums, n_sums);
+    return hipGetLastError();
+}
+
+// Mean over images per pixel, summed in image order in FP32
+// (src/gpu/gpu_utils.cpp:143-150; real part only for src/utils.cpp:4-22).
+__global__ void k_average(const float2* __restrict__ images, int64_t image_size, int n_images,
+                          int real_only, float2* __restrict__ out) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < image_size; i += stride) {
+        float2 t = images[i];
+        for (int k = 1; k < n_images; k++) {
+            const float2 v = images[i + (int64_t)k * image_size];
+            t.x += v.x;
+            t.y += v.y;
+        }
+        const float n = (float)n_images;
+        out[i] = real_only ? make_float2(t.x / n, 0.f) : make_float2(t.x / n, t.y / n);
+    }
+}
+
+hipError_t launch_average(const float* images, int64_t image_size, int n_images, int real_only,
+                          float* avg, hipStream_t stream) {
+    if (image_size == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_average, dim3(grid_for(image_size, 256, 65536)), dim3(256), 0, stream,
+                       (const float2*)images, image_size, n_images, real_only, (float2*)avg);
+    return hipGetLastError();
+}
+
+// ------------------------------------- batched: gridding fused with row FFT
+
+// Workgroup = (tile of RB rows, image).  Plan entries of the tile are
+// gathered (pol, phase correction, NaN/MAX_VIS), accumulated with LDS atomics
+// into the tile, each non-empty row is transformed in LDS and written to the
+// row-pass buffer `inter` (image index relative to the chunk).  Empty rows are
+// not written: the column pass reads the plan and treats them as zero.  The
+// number of gridded cells per image goes to isums (integer atomics).
+template <int S, bool kBlockTeam>
+__global__ void __launch_bounds__(kFftThreads)
+k_grid_rows_fft(GatherArgs g, const int* __restrict__ row_start,
+                const uint32_t* __restrict__ entries, FftPlan plan, int team, int rb,
+                int64_t img0, int n_img, float2* __restrict__ inter, uint32_t* __restrict__ isums,
+                const float2* __restrict__ tw) {
+    extern __shared__ float4 grf_lds4[];
+    const int n = plan.n;
+    float2* tile = (float2*)grf_lds4;                 // [rb][n]
+    const int teams = kBlockTeam ? 1 : (int)blockDim.x / team;
+    const int t_id = kBlockTeam ? 0 : (int)threadIdx.x / team;
+    const int lane = kBlockTeam ? (int)threadIdx.x : (int)threadIdx.x % team;
+    float2* tmp = tile + (size_t)rb * n + (size_t)t_id * n;
+    __shared__ int rs[kMaxTileRows + 1];
+    __shared__ int red[kFftThreads / 64];
+
+    const int n_rb = (n + rb - 1) / rb;
+    // XCD-aware order: consecutive workgroup ids land on different XCDs, so
+    // the ids are remapped to give each XCD a contiguous run of (image, tile)
+    // pairs and keep an image's visibilities in one L2.
+    const unsigned nwg = gridDim.x;
+    unsigned bid = blockIdx.x;
+    if ((nwg & 7u) == 0) bid = (bid & 7u) * (nwg >> 3) + (bid >> 3);
+    const int tile_id = (int)(bid % n_rb);
+    const int li = (int)(bid / n_rb);
+    const int64_t img = img0 + li;
+    const int ch = (int)(img % g.n_ch);
+    const int r0 = tile_id * rb;
+    const int nr = min(rb, n - r0);
+    const int* rsg = row_start + (int64_t)ch * (n + 1);
+    const uint32_t* ent = entries + (int64_t)ch * 2 * g.n_bl;
+    if ((int)threadIdx.x <= nr) rs[threadIdx.x] = rsg[r0 + threadIdx.x];
+    __syncthreads();
+    if (rs[nr] == rs[0]) return;  // uniform: no entries in this tile
+    for (int i = threadIdx.x; i < nr * n; i += blockDim.x) tile[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+    int n_ok = 0;
+    for (int e = rs[0] + threadIdx.x; e < rs[nr]; e += blockDim.x) {
+        int u_idx;
+        float2 v;
+        if (!gather_entry(g, img, ch, ent[e], u_idx, v)) continue;
+        const int r = row_of_entry(rs, nr, e);
+        atomicAdd(&tile[r * n + u_idx].x, v.x);
+        atomicAdd(&tile[r * n + u_idx].y, v.y);
+        n_ok++;
+    }
+    const int tot = block_sum_int(n_ok, red);
+    if (threadIdx.x == 0 && tot) atomicAdd(&isums[img], (uint32_t)tot);
+    __syncthreads();
+    const int rounds = kBlockTeam ? nr : (nr + teams - 1) / teams;
+    for (int k = 0; k < rounds; k++) {
+        const int r = kBlockTeam ? k : k * teams + t_id;
+        if (!kBlockTeam && r >= nr) break;
+        if (rs[r + 1] == rs[r]) continue;  // uniform per team
+        float2* row = tile + (size_t)r * n;
+        const float2* res = fft_row<S, kBlockTeam>(row, tmp, plan, tw, lane, team);
+        float2* dst = inter + ((int64_t)li * n + r0 + r) * n;
+        for (int i = lane; i < n; i += team) dst[i] = res[i];
+        team_sync<kBlockTeam>();
+    }
+}
+
+// ------------------------------------------------------------ launchers
+
+static int fft_team_for(int n) {
+    // lanes per row: about 8-16 values per lane, one wave or the whole block
+    if (n <= 128) return 16;
+    if (n <= 512) return 32;
+    if (n <= 1024) return 64;
+    return kFftThreads;
+}
+
+bool make_fft_plan(int n, FftPlan& plan) {
+    if (n < 2 || n > kMaxPixels) return false;
+    plan.n = n;
+    plan.n_passes = 0;
+    int rem = n, span = 1;
+    while (rem > 1) {
+        int r;
+        if (rem % 16 == 0) r = 16;
+        else if (rem % 8 == 0) r = 8;
+        else if (rem % 4 == 0) r = 4;
+        else if (rem % 2 == 0) r = 2;
+        else if (rem % 3 == 0) r = 3;
+        else if (rem % 5 == 0) r = 5;
+        else return false;
+        if (plan.n_passes >= kMaxFftPasses) return false;
+        plan.radix[plan.n_passes] = r;
+        plan.span[plan.n_passes] = span;
+        plan.n_passes++;
+        span *= r;
+        rem /= r;
+    }
+    return true;
+}
+
+template <int S>
+static hipError_t fft_rows_t(const float* in, float* out, const FftPlan& plan, int64_t n_rows,
+                             const float* tw, hipStream_t stream) {
+    const int n = plan.n;
+    const int team = fft_team_for(n);
+    if (team == kFftThreads) {
+        const size_t lds = sizeof(float2) * 2 * (size_t)n;
+        if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
+        hipLaunchKernelGGL((k_fft_rows<S, true>), dim3((unsigned)n_rows), dim3(kFftThreads), lds,
+                           stream, (const float2*)in, (float2*)out, plan, team, n_rows,
+                           (const float2*)tw);
+    } else {
+        const int teams = kFftThreads / team;
+        const size_t lds = sizeof(float2) * 2 * (size_t)n * teams;
+        if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
+        const unsigned blocks = (unsigned)((n_rows + teams - 1) / teams);
+        hipLaunchKernelGGL((k_fft_rows<S, false>), dim3(blocks), dim3(kFftThreads), lds, stream,
+                           (const float2*)in, (float2*)out, plan, team, n_rows, (const float2*)tw);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fft_rows(const float* in, float* out, const FftPlan& plan, int64_t n_rows,
+                           int sign, const float* tw, hipStream_t stream) {
+    if (n_rows == 0) return hipSuccess;
+    return sign > 0 ? fft_rows_t<1>(in, out, plan, n_rows, tw, stream)
+                    : fft_rows_t<-1>(in, out, plan, n_rows, tw, stream);
+}
+
+// Column width: the largest divisor of the column span that fits in the LDS
+// budget next to the team scratch rows, at most kMaxColWidth.
+static int col_width_for(int n, int teams, bool paired) {
+    const int64_t budget = g_lds_budget - (int64_t)teams * n * 8;
+    int w = (int)(budget / ((int64_t)(n + 1) * 8 * (paired ? 2 : 1)));
+    w = std::min(w, kMaxColWidth);
+    const int span = paired ? n / 2 : n;
+    w = std::max(1, std::min(w, span));
+    while (span % w) w--;
+    return w;
+}
+
+bool cols_paired_fits(int n) {
+    const int team = fft_team_for(n);
+    const int teams = team == kFftThreads ? 1 : kFftThreads / team;
+    const size_t lds = sizeof(float2) * (2 * (size_t)(n + 1) + (size_t)teams * n);
+    return n % 2 == 0 && lds_fits(lds);
+}
+
+template <int S>
+static hipError_t fft_cols_t(ColArgs a, const FftPlan& plan, int n_images, const float* tw,
+                             hipStream_t stream) {
+    const int n = plan.n;
+    const int team = fft_team_for(n);
+    const bool block_team = team == kFftThreads;
+    const int teams = block_team ? 1 : kFftThreads / team;
+    a.width = col_width_for(n, teams, a.paired != 0);
+    const int span = a.paired ? n / 2 : n;
+    const int n_cb = (span + a.width - 1) / a.width;
+    const int wt = a.paired ? 2 * a.width : a.width;
+    const size_t lds = sizeof(float2) * ((size_t)wt * (n + 1) + (size_t)teams * n);
+    if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)((int64_t)n_cb * n_images));
+    if (block_team)
+        hipLaunchKernelGGL((k_fft_cols<S, true>), grid, dim3(kFftThreads), lds, stream, a, plan,
+                           team, (const float2*)tw);
+    else
+        hipLaunchKernelGGL((k_fft_cols<S, false>), grid, dim3(kFftThreads), lds, stream, a, plan,
+                           team, (const float2*)tw);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft_cols(const ColArgs& a, const FftPlan& plan, int n_images, int sign,
+                           const float* tw, hipStream_t stream) {
+    if (n_images == 0) return hipSuccess;
+    return sign > 0 ? fft_cols_t<1>(a, plan, n_images, tw, stream)
+                    : fft_cols_t<-1>(a, plan, n_images, tw, stream);
+}
+
+int grid_rows_tile_rows(int n) {
+    const int team = fft_team_for(n);
+    const int teams = team == kFftThreads ? 1 : kFftThreads / team;
+    const int64_t budget = g_lds_budget - (int64_t)teams * n * 8;
+    int rb = (int)(budget / ((int64_t)n * 8));
+    rb = std::min(rb, kMaxTileRows);
+    if (rb >= teams) rb = (rb / teams) * teams;
+    return std::max(1, std::min(rb, n));
+}
+
+hipError_t launch_grid_rows_fft(const GatherArgs& g, const int* row_start, const uint32_t* entries,
+                                const FftPlan& plan, int64_t img0, int n_img, float* inter,
+                                uint32_t* isums, const float* tw, hipStream_t stream) {
+    if (n_img == 0) return hipSuccess;
+    const int n = plan.n;
+    const int team = fft_team_for(n);
+    const bool block_team = team == kFftThreads;
+    const int teams = block_team ? 1 : kFftThreads / team;
+    const int rb = grid_rows_tile_rows(n);
+    const int n_rb = (n + rb - 1) / rb;
+    const size_t lds = sizeof(float2) * ((size_t)rb * n + (size_t)teams * n);
+    if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)((int64_t)n_rb * n_img));
+    if (block_team)
+        hipLaunchKernelGGL((k_grid_rows_fft<1, true>), grid, dim3(kFftThreads), lds, stream, g,
+                           row_start, entries, plan, team, rb, img0, n_img, (float2*)inter, isums,
+                           (const float2*)tw);
+    else
+        hipLaunchKernelGGL((k_grid_rows_fft<1, false>), grid, dim3(kFftThreads), lds, stream, g,
+                           row_start, entries, plan, team, rb, img0, n_img, (float2*)inter, isums,
+                           (const float2*)tw);
+    return hipGetLastError();
+}
+
+__global__ void k_isums_to_double(const uint32_t* __restrict__ isums, double* __restrict__ sums,
+                                  int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sums[i] = (double)isums[i];
+}
+
+hipError_t launch_isums_to_double(const uint32_t* isums, double* sums, int n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_isums_to_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                       isums, sums, n);
+    return hipGetLastError();
+}
+
+hipError_t set_kernel_attributes() {
+    // Allow the dynamic LDS sizes used above.
+    const int max_lds = (int)g_lds_limit;
+    hipError_t e = hipSuccess;
+    auto set = [&](const void* f) {
+        hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        if (r != hipSuccess && e == hipSuccess) e = r;
+    };
+    set((const void*)k_fft_rows<1, true>);
+    set((const void*)k_fft_rows<1, false>);
+    set((const void*)k_fft_rows<-1, true>);
+    set((const void*)k_fft_rows<-1, false>);
+    set((const void*)k_fft_cols<1, true>);
+    set((const void*)k_fft_cols<1, false>);
+    set((const void*)k_fft_cols<-1, true>);
+    set((const void*)k_fft_cols<-1, false>);
+    set((const void*)k_grid_rows_fft<1, true>);
+    set((const void*)k_grid_rows_fft<1, false>);
+    set((const void*)k_grid_tiles);
+    set((const void*)k_plan_rows);
+    return e;
+}
+
+}  // namespace blink

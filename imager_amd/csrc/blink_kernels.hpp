@@ -1,0 +1,90 @@
+// Host-side launch interface of the gfx950 kernels (blink_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "blink_fft.hpp"
+
+namespace blink {
+
+constexpr int kMaxPixels = 4096;      // largest grid side (plan packs u in 13 bits)
+constexpr int kMaxBaselines = 1 << 18;  // plan packs the baseline in 18 bits
+constexpr int kPlanThreads = 1024;
+constexpr int kTileThreads = 256;
+constexpr int kFftThreads = 256;
+constexpr int kMaxTileRows = 16;
+constexpr int kMaxColWidth = 16;
+
+// LDS bytes a workgroup may allocate dynamically (device limit), and the
+// share the tile-sizing heuristics aim for.  Set once per process from the
+// device properties (gfx950: 160 KiB per CU).
+extern int64_t g_lds_limit;
+extern int64_t g_lds_budget;
+
+struct GatherArgs {
+    const float4* vis;        // [t][ch][bl][2 x float4]
+    const float* w_m;
+    const double* cable_m;
+    const double* freq_hz;
+    unsigned corrections;
+    int pol;
+    int n_bl, n_ch;
+};
+
+struct ColArgs {
+    const float2* in;          // [img][N][N], images of this launch
+    float2* out;               // [img][N][N], images of this launch
+    int64_t in_img_stride;     // elements
+    int64_t out_img_stride;
+    int64_t out_img_offset;    // global index of the launch's first image (sums, plan)
+    int width;                 // W, set by the launcher
+    int paired;                // also load columns c0 + N/2 .. (+W)
+    int shift;                 // fft_shift on store
+    const double* sums;        // scale (float)(1/sums[img]) on store, or null
+    const uint32_t* isums;     // integer sums alternative, or null
+    const int* row_start;      // plan rows: skip empty rows on load, or null
+    int n_ch;
+};
+
+bool make_fft_plan(int n, FftPlan& plan);
+// Whether the in-place (paired) shifted column pass fits in LDS for side n.
+bool cols_paired_fits(int n);
+int grid_rows_tile_rows(int n);
+hipError_t set_kernel_attributes();
+
+hipError_t launch_zero(void* ptr, int64_t bytes, hipStream_t stream);
+hipError_t launch_phase_correct(float* vis, int n_ant, int n_ch, int n_t, const float* w_m,
+                                const double* cable_m, const double* freq_hz, unsigned which,
+                                hipStream_t stream);
+hipError_t launch_uv_cells(const float* u_m, const float* v_m, const int* flags, int n_bl,
+                           const double* freq_hz, int n_ch, double du, double dv, double min_uv,
+                           int n_pixels, int* cells, hipStream_t stream);
+hipError_t launch_grid_atomic(const float* vis, int n_ant, int n_ch, int n_t, const float* u_m,
+                              const float* v_m, const int* flags, const double* freq_hz, double du,
+                              double dv, double min_uv, int n_pixels, int pol, float* grids,
+                              float* counts, hipStream_t stream);
+hipError_t launch_plan_rows(const float* u_m, const float* v_m, const int* flags, int n_bl,
+                            const double* freq_hz, int n_ch, double du, double dv, double min_uv,
+                            int n_pixels, int* row_start, uint32_t* entries, hipStream_t stream);
+hipError_t launch_grid_tiles(const GatherArgs& g, const int* row_start, const uint32_t* entries,
+                             int n_images, int n_pixels, float* grids, float* counts,
+                             hipStream_t stream);
+hipError_t launch_uniform_weight(float* grids, const float* counts, int64_t n_cells,
+                                 hipStream_t stream);
+hipError_t launch_fft_rows(const float* in, float* out, const FftPlan& plan, int64_t n_rows,
+                           int sign, const float* tw, hipStream_t stream);
+hipError_t launch_fft_cols(const ColArgs& a, const FftPlan& plan, int n_images, int sign,
+                           const float* tw, hipStream_t stream);
+hipError_t launch_vector_sum(const float* values, int64_t image_size, int n_images, double* sums,
+                             hipStream_t stream);
+hipError_t launch_shift_norm(float* images, int nx, int ny, int n_images, const double* sums,
+                             int n_sums, hipStream_t stream);
+hipError_t launch_average(const float* images, int64_t image_size, int n_images, int real_only,
+                          float* avg, hipStream_t stream);
+hipError_t launch_grid_rows_fft(const GatherArgs& g, const int* row_start, const uint32_t* entries,
+                                const FftPlan& plan, int64_t img0, int n_img, float* inter,
+                                uint32_t* isums, const float* tw, hipStream_t stream);
+hipError_t launch_isums_to_double(const uint32_t* isums, double* sums, int n, hipStream_t stream);
+
+}  // namespace blink

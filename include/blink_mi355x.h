@@ -1,0 +1,205 @@
+/*
+ * blink_mi355x.h -- C ABI of the MI355X (gfx950) visibility -> dirty-image
+ * library (libblink_mi355x.so).
+ *
+ * The reference imager (PaCER-BLINK-Project/imager) has no C ABI: its GPU
+ * path is a C++ class, CPacerImagerHip (src/gpu/pacer_imager_hip.h:12-58),
+ * that overrides four virtuals of CPacerImager (src/pacer_imager.h:124-141)
+ * and calls free functions in src/gpu/{corrections_gpu,gridding_gpu,
+ * gpu_utils}.h plus hipFFT.  Every entry point below replaces one of those
+ * functions; the comment on each names the reference interface it replaces.
+ *
+ * Conventions (all entry points):
+ *   - Plain pointers and sizes only.  Every array pointer is DEVICE memory
+ *     (hipMalloc / torch) unless the comment says "host".
+ *   - complex64 data is interleaved float pairs (re, im), the layout of
+ *     std::complex<float> and hipfftComplex.
+ *   - Visibilities: vis[t][ch][bl][pol], pol = XX, XY, YX, YY,
+ *     bl = a1*(a1+1)/2 + a2 with a2 <= a1, n_bl = n_ant*(n_ant+1)/2
+ *     (autos included).  Grids, counts and images: [t][ch][v][u] (N x N each).
+ *   - `stream` is a hipStream_t (NULL = the null stream).  Work is enqueued
+ *     asynchronously on it; nothing synchronises unless stated.
+ *   - Return value: BLINK_OK (0) or an error code; blink_last_error() gives
+ *     the message.  BLINK_ERR_INVALID_ARGUMENT corresponds to the reference's
+ *     std::invalid_argument, BLINK_ERR_RUNTIME to std::runtime_error.
+ *   - A context owns its scratch memory, FFT twiddle tables and captured
+ *     graphs.  Contexts are not thread-safe: use one per host thread.
+ */
+#ifndef BLINK_MI355X_H
+#define BLINK_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLINK_API __attribute__((visibility("default")))
+
+typedef struct blink_ctx blink_ctx;
+
+enum blink_status {
+    BLINK_OK = 0,
+    BLINK_ERR_INVALID_ARGUMENT = 1,
+    BLINK_ERR_RUNTIME = 2,
+    BLINK_ERR_HIP = 3,
+    BLINK_ERR_UNSUPPORTED = 4
+};
+
+/* Polarisation to image, same values as enum class Polarization
+ * (src/gridding.hpp:4). */
+enum blink_pol { BLINK_POL_XX = 0, BLINK_POL_YY = 1, BLINK_POL_I = 2, BLINK_POL_V = 3 };
+
+/* Weighting: natural, or the reference's "U" (src/pacer_imager.cpp:421-428). */
+enum blink_weighting { BLINK_WEIGHTING_NATURAL = 0, BLINK_WEIGHTING_UNIFORM = 1 };
+
+/* Phase corrections fused into blink_image_batch. */
+enum blink_corrections { BLINK_CORR_GEOMETRIC = 1u, BLINK_CORR_CABLE = 2u };
+
+/* FFT direction: BACKWARD is exp(+2 pi i jk/N) (FFTW_BACKWARD / hipFFT
+ * BACKWARD, src/gpu/pacer_imager_hip.cpp:95), unnormalised. */
+enum blink_fft_direction { BLINK_FFT_FORWARD = -1, BLINK_FFT_BACKWARD = 1 };
+
+/* Gridding algorithm for blink_gridding. */
+enum blink_grid_algo {
+    BLINK_GRID_TILED = 0,  /* per-channel row plan + LDS-privatised row tiles (default) */
+    BLINK_GRID_ATOMIC = 1  /* one thread per visibility, global atomics (plain path) */
+};
+
+/* Largest grid side supported by the in-house FFT and the gridding plan. */
+#define BLINK_MAX_PIXELS 4096
+
+/* ---------------------------------------------------------------- context */
+
+/* Creates a context on HIP device `device`.  No reference counterpart (the
+ * reference keeps this state in CPacerImagerHip members,
+ * src/gpu/pacer_imager_hip.h:14-29). */
+BLINK_API int blink_create(int device, blink_ctx** out);
+BLINK_API void blink_destroy(blink_ctx* ctx);
+/* Message of the last failed call on this context (never NULL). */
+BLINK_API const char* blink_last_error(const blink_ctx* ctx);
+/* Library version string. */
+BLINK_API const char* blink_version(void);
+/* 1 when n_pixels is supported by blink_fft2d_c2c / blink_image. */
+BLINK_API int blink_fft_supported(int n_pixels);
+
+/* ------------------------------------------------- (a) phase corrections */
+
+/* In place: V <- V * exp(+i 2 pi w[bl] nu[ch] / c) for all 4 pol products.
+ * Replaces apply_geometric_corrections_gpu (src/gpu/corrections_gpu.cpp:90-101)
+ * and apply_geometric_corrections_cpu (src/corrections.cpp:7-39, using w[bl]).
+ * The argument is formed and reduced modulo 2 pi in FP64, sin/cos in FP32. */
+BLINK_API int blink_apply_geometric_corrections(blink_ctx* ctx, float* vis, int n_ant,
+                                                int n_ch, int n_t, const float* w_m,
+                                                const double* freq_hz, void* stream);
+
+/* In place: V <- V * exp(-i 2 pi (L[a2] - L[a1]) nu[ch] / c).
+ * Replaces apply_cable_lengths_corrections_gpu (src/gpu/corrections_gpu.cpp:75-87)
+ * and apply_cable_lengths_corrections_cpu (src/corrections.cpp:41-74). */
+BLINK_API int blink_apply_cable_corrections(blink_ctx* ctx, float* vis, int n_ant, int n_ch,
+                                            int n_t, const double* cable_m,
+                                            const double* freq_hz, void* stream);
+
+/* --------------------------------------------- (b) nearest-neighbour gridding */
+
+/* Grids every unflagged, finite (|re|,|im| < 1e12) visibility of the chosen
+ * polarisation at its cell and its conjugate at the mirrored cell, adding
+ * into grids[t][ch][N][N] (complex64) and counts[t][ch][N][N] (float, per
+ * image).  Accumulates: the caller zeroes grids/counts first (blink_image
+ * zeroes them after imaging, as the reference does).  Cells follow
+ * src/pacer_imager.cpp:289,332-369 bit for bit.  weighting = UNIFORM divides
+ * each cell by its count afterwards (0/0 = NaN in empty cells, as the
+ * reference).  Replaces gridding_gpu (src/gpu/gridding_gpu.h:12-18) and
+ * CPacerImager::gridding (src/pacer_imager.cpp:250-432). */
+BLINK_API int blink_gridding(blink_ctx* ctx, const float* vis, int n_ant, int n_ch, int n_t,
+                             const float* u_m, const float* v_m, const int* bl_flags,
+                             const double* freq_hz, double delta_u, double delta_v,
+                             double min_uv, int n_pixels, int pol, int weighting, int algo,
+                             float* grids, float* counts, void* stream);
+
+/* Cell indices of every (channel, baseline): cells[(ch*n_bl + bl)*4 + k] =
+ * u, v, u_conj, v_conj, or -1 for flagged / min_uv-cut baselines.  The
+ * index computation of calculate_pos (src/gpu/gridding_gpu.cpp:24-46);
+ * exposed so that indices can be checked bit for bit. */
+BLINK_API int blink_uv_cells(blink_ctx* ctx, const float* u_m, const float* v_m,
+                             const int* bl_flags, int n_bl, const double* freq_hz, int n_ch,
+                             double delta_u, double delta_v, double min_uv, int n_pixels,
+                             int* cells, void* stream);
+
+/* --------------------------------------------------------------- (c) FFT */
+
+/* Batched, unnormalised 2-D complex FFT of `batch` N x N arrays, in-house
+ * LDS-staged Stockham passes (no hipFFT/rocFFT).  in == out is allowed.
+ * N = 2^a 3^b 5^c, 2 <= N <= BLINK_MAX_PIXELS.  Replaces gpufftPlanMany +
+ * gpufftExecC2C (src/gpu/pacer_imager_hip.cpp:83-99). */
+BLINK_API int blink_fft2d_c2c(blink_ctx* ctx, const float* in, float* out, int n_pixels,
+                              int batch, int direction, void* stream);
+
+/* ------------------------------------------------- (d) shift, normalisation */
+
+/* sums[img] = sum of values[img*image_size ...] in double (exact for the
+ * integer-valued counts).  Replaces vector_sum_gpu (src/gpu/gpu_utils.cpp:19-71). */
+BLINK_API int blink_vector_sum(blink_ctx* ctx, const float* values, int64_t image_size,
+                               int n_images, double* sums, void* stream);
+
+/* In place on images[n_images][ny][nx]: fft_shift (src/pacer_imager.cpp:110-128,
+ * odd sizes included) and multiply by (float)(1/sums[img % n_sums]); sums may
+ * be NULL (no normalisation).  Replaces fft_shift_and_norm_gpu
+ * (src/gpu/gpu_utils.cpp:128-140). */
+BLINK_API int blink_fft_shift_and_norm(blink_ctx* ctx, float* images, int nx, int ny,
+                                       int n_images, const double* sums, int n_sums,
+                                       void* stream);
+
+/* Full imaging step on gridded data: backward FFT of grids, sum of counts,
+ * multiply by (float)(1/sum), fft_shift, written to images; then, when
+ * zero_grids != 0, zeroes grids and counts for the next gridding round.
+ * Replaces CPacerImagerHip::image (src/gpu/pacer_imager_hip.cpp:72-127) and
+ * CPacerImager::image (src/pacer_imager.cpp:134-208), without the Images
+ * metadata. */
+BLINK_API int blink_image(blink_ctx* ctx, float* grids, float* counts, int n_pixels,
+                          int n_images, float* images, int zero_grids, void* stream);
+
+/* avg = complex mean of n_images images (image_averaging_gpu,
+ * src/gpu/gpu_utils.cpp:143-161); real_only != 0 averages the real part
+ * only and leaves the imaginary part 0 (image_averaging_cpu, src/utils.cpp:4-22). */
+BLINK_API int blink_image_averaging(blink_ctx* ctx, const float* images, int64_t image_size,
+                                    int n_images, int real_only, float* avg, void* stream);
+
+/* ------------------------------------------------ batched visibilities -> images */
+
+/* One set of images that share an array, a grid size and a polarisation:
+ * n_t x n_ch images from vis[t][ch][bl][4].  Every pointer is device memory. */
+typedef struct blink_image_set {
+    const float* vis;        /* [n_t][n_ch][n_bl][4] complex64, read only */
+    int n_ant, n_ch, n_t;
+    const float* u_m;        /* [n_bl] metres */
+    const float* v_m;        /* [n_bl] metres */
+    const float* w_m;        /* [n_bl] metres, read when BLINK_CORR_GEOMETRIC */
+    const int* bl_flags;     /* [n_bl], non-zero = flagged */
+    const double* freq_hz;   /* [n_ch] */
+    const double* cable_m;   /* [n_ant], read when BLINK_CORR_CABLE */
+    unsigned corrections;    /* BLINK_CORR_* bits, applied on the fly (vis is not modified) */
+    double delta_u, delta_v, min_uv;
+    int n_pixels;
+    int pol;                 /* blink_pol */
+    float* images;           /* [n_t][n_ch][N][N] complex64 output */
+    double* count_sums;      /* optional [n_t*n_ch] output: sum of counts per image */
+} blink_image_set;
+
+/* Visibilities -> normalised, shifted dirty images for every set, with the
+ * phase corrections, gridding, FFT, shift and normalisation fused into a
+ * few kernels per set, the whole call captured once as a hipGraph and
+ * replayed when it is made again with identical arguments.  Results equal
+ * apply_*_corrections + gridding + image of the reference (CPacerImager::run,
+ * src/pacer_imager.cpp:440-526) for natural weighting. */
+BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
+                                void* stream);
+
+/* Releases every captured graph and scratch buffer of the context. */
+BLINK_API int blink_release_cache(blink_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BLINK_MI355X_H */

@@ -1,0 +1,393 @@
+"""GPU parity of every stage, called through the C ABI, against the CPU oracle.
+
+Bars: cell indices and per-cell counts bit-exact; images within 1e-5
+peak-normalised of the oracle's reference-precision (FP32) image, with the
+error to the FP64 oracle image printed beside it; phase corrections within
+1e-6 of |V| of the FP64-maths reference rotation.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from imager_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+IMAGE_TOL = 1e-5          # peak-normalised, vs the FP32 oracle (reference precision)
+CORR_TOL = 1e-6           # |dV| / |V| for the phase corrections
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _obs_dev(o):
+    return dict(vis=_dev(o.vis), u=_dev(o.u), v=_dev(o.v), w=_dev(o.w), flags=_dev(o.flags),
+                freq=_dev(o.freq), cable=_dev(o.cable))
+
+
+# ---------------------------------------------------------------- cells
+
+@pytest.mark.parametrize("case", ["eda2_180", "mwa_1024", "small_odd"])
+def test_uv_cells_bit_exact(ctx, case):
+    import torch
+    if case == "eda2_180":
+        o = synth.eda2_station(n_ch=4, freq=np.linspace(150e6, 190e6, 4))
+    elif case == "mwa_1024":
+        o = synth.mwa_array(n_ch=16, n_t=1)
+    else:
+        o = synth.small(n_ant=30, n_ch=5, n_t=1, n_pixels=45, seed=3)
+    _say("case", case, o.n_ant, o.n_ch, o.n_pixels)
+    ref = oracle.cells(o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v, o.min_uv, o.n_pixels)
+    d = _obs_dev(o)
+    cells = torch.empty((o.n_ch, o.n_bl, 4), dtype=torch.int32, device="cuda")
+    ctx.uv_cells(d["u"], d["v"], d["flags"], o.n_bl, d["freq"], o.n_ch, o.delta_u, o.delta_v,
+                 o.min_uv, o.n_pixels, cells)
+    assert np.array_equal(_host(cells), ref)
+
+
+def test_uv_cells_exact_halves_and_min_uv(ctx):
+    import torch
+    c = 299792458.0
+    u = np.array([2.5, -2.5, 0.0, 1.0, 3.0, -7.0, -8.0, 9.0], np.float32)
+    v = np.array([0.0, 0.0, -0.5, 1.0, 0.0, 0.0, 0.0, 0.0], np.float32)
+    flags = np.zeros(8, np.int32)
+    freq = np.array([c])
+    for min_uv in (-np.inf, 2.0):
+        ref = oracle.cells(u, v, flags, freq, 1.0, 1.0, min_uv, 8)
+        cells = torch.empty((1, 8, 4), dtype=torch.int32, device="cuda")
+        ctx.uv_cells(_dev(u), _dev(v), _dev(flags), 8, _dev(freq), 1, 1.0, 1.0, min_uv, 8, cells)
+        assert np.array_equal(_host(cells), ref)
+
+
+# ---------------------------------------------------------- corrections
+
+def test_phase_corrections(ctx):
+    o = synth.small(n_ant=40, n_ch=6, n_t=3, n_pixels=64, seed=21, height_sigma_m=5.0)
+    d = _obs_dev(o)
+    ref = oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq)
+    ctx.apply_geometric_corrections(d["vis"], o.n_ant, o.n_ch, o.n_t, d["w"], d["freq"])
+    got = _host(d["vis"])
+    err = np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30))
+    _say("geometric max rel err", err)
+    assert err < CORR_TOL
+    ref2 = oracle.cable_corrections(ref, o.n_ant, o.cable, o.freq)
+    ctx.apply_cable_corrections(d["vis"], o.n_ant, o.n_ch, o.n_t, d["cable"], d["freq"])
+    got2 = _host(d["vis"])
+    err2 = np.max(np.abs(got2 - ref2) / np.maximum(np.abs(ref2), 1e-30))
+    _say("cable max rel err", err2)
+    assert err2 < 2 * CORR_TOL
+
+
+def test_phase_corrections_remove_w_phase(ctx):
+    """Closed form: a pure w-phase exp(-2 pi i w/lambda) corrects to 1."""
+    rng = np.random.default_rng(9)
+    n_ant = 64
+    x, y, z = rng.normal(0, 300, n_ant), rng.normal(0, 300, n_ant), rng.normal(0, 10, n_ant)
+    u, v, w = synth.zenith_uvw(x, y, z)
+    freq = np.array([170e6, 200e6])
+    lam = 299792458.0 / freq
+    vis = np.exp(-2j * np.pi * w.astype(np.float64)[None, :] / lam[:, None])
+    vis = np.repeat(vis[None, :, :, None], 4, axis=3).astype(np.complex64)
+    dv = _dev(vis)
+    ctx.apply_geometric_corrections(dv, n_ant, 2, 1, _dev(w), _dev(freq))
+    assert np.max(np.abs(_host(dv) - 1.0)) < 1e-6
+
+
+# ------------------------------------------------------------- gridding
+
+@pytest.mark.parametrize("algo", [0, 1])
+@pytest.mark.parametrize("pol", ["XX", "YY", "I", "V"])
+def test_gridding_counts_exact_grid_close(ctx, algo, pol):
+    import torch
+    o = synth.small(n_ant=48, n_ch=3, n_t=2, n_pixels=96, seed=4)
+    _say("gridding algo", algo, "pol", pol)
+    g_ref, c_ref, _ = oracle.gridding(o.vis, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u,
+                                      o.delta_v, o.min_uv, o.n_pixels, pol)
+    d = _obs_dev(o)
+    shape = (o.n_t, o.n_ch, o.n_pixels, o.n_pixels)
+    grids = torch.zeros(shape, dtype=torch.complex64, device="cuda")
+    counts = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    ctx.gridding(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["flags"], d["freq"],
+                 o.delta_u, o.delta_v, o.min_uv, o.n_pixels, pol, 0, grids, counts, algo=algo)
+    assert np.array_equal(_host(counts), c_ref)
+    err = np.max(np.abs(_host(grids) - g_ref))
+    assert err <= 1e-5 * np.max(np.abs(g_ref))
+
+
+def test_gridding_accumulates_and_uniform(ctx):
+    import torch
+    o = synth.small(n_ant=32, n_ch=2, n_t=1, n_pixels=64, seed=8)
+    d = _obs_dev(o)
+    shape = (o.n_t, o.n_ch, o.n_pixels, o.n_pixels)
+    grids = torch.zeros(shape, dtype=torch.complex64, device="cuda")
+    counts = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    args = (d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["flags"], d["freq"], o.delta_u,
+            o.delta_v, o.min_uv, o.n_pixels, "XX")
+    ctx.gridding(*args, 0, grids, counts)
+    ctx.gridding(*args, 0, grids, counts)   # grids accumulate across calls
+    g1, c1, _ = oracle.gridding(o.vis, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v,
+                                o.min_uv, o.n_pixels, "XX")
+    assert np.array_equal(_host(counts), 2 * c1)
+    assert np.max(np.abs(_host(grids) - 2 * g1)) <= 1e-5 * np.max(np.abs(2 * g1))
+    grids.zero_()
+    counts.zero_()
+    ctx.gridding(*args, 1, grids, counts)
+    gu, cu, _ = oracle.gridding(o.vis, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v,
+                                o.min_uv, o.n_pixels, "XX", uniform=True)
+    gg = _host(grids)
+    assert np.array_equal(np.isnan(gg), np.isnan(gu))
+    occ = ~np.isnan(gu)
+    assert np.max(np.abs(gg[occ] - gu[occ])) <= 1e-5 * np.max(np.abs(gu[occ]))
+
+
+def test_gridding_skips_nan_and_maxvis(ctx):
+    import torch
+    o = synth.small(n_ant=20, n_ch=1, n_t=1, n_pixels=40, seed=12)
+    vis = o.vis.copy()
+    good = np.where(o.flags == 0)[0]
+    vis[0, 0, good[0], 0] = np.nan
+    vis[0, 0, good[1], 0] = 3e12
+    vis[0, 0, good[2], 0] = 1.00000007e12j
+    vis[0, 0, good[3], 0] = 1e12j
+    _, c_ref, _ = oracle.gridding(vis, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v,
+                                  o.min_uv, o.n_pixels, "XX")
+    for algo in (0, 1):
+        grids = torch.zeros((1, 1, 40, 40), dtype=torch.complex64, device="cuda")
+        counts = torch.zeros((1, 1, 40, 40), dtype=torch.float32, device="cuda")
+        ctx.gridding(_dev(vis), o.n_ant, 1, 1, _dev(o.u), _dev(o.v), _dev(o.flags), _dev(o.freq),
+                     o.delta_u, o.delta_v, o.min_uv, 40, "XX", 0, grids, counts, algo=algo)
+        assert np.array_equal(_host(counts), c_ref)
+
+
+# ------------------------------------------------------------------ FFT
+
+@pytest.mark.parametrize("n,batch", [(2, 3), (8, 2), (15, 2), (45, 1), (64, 3), (128, 2),
+                                     (180, 3), (256, 2), (512, 1), (1024, 2), (2048, 1),
+                                     (4096, 1)])
+@pytest.mark.parametrize("direction", [1, -1])
+def test_fft2d_matches_numpy(ctx, n, batch, direction):
+    import torch
+    _say("fft", n, batch, direction)
+    rng = np.random.default_rng(n + batch)
+    x = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n)))
+    ref = np.fft.ifft2(x) * n * n if direction > 0 else np.fft.fft2(x)
+    dx = _dev(x.astype(np.complex64))
+    out = torch.empty_like(dx)
+    ctx.fft2d_c2c(dx, out, n, batch, direction)
+    got = _host(out)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    _say("  rel err", err)
+    assert err < 3e-6
+    # in place
+    ctx.fft2d_c2c(dx, dx, n, batch, direction)
+    assert np.array_equal(_host(dx), got)
+
+
+# ---------------------------------------------- shift, norm, sums, averaging
+
+def test_fft_shift_and_norm_kats(ctx):
+    import torch
+    # tests/gpu_utils_test.cpp:60-87: 3 images of 2x2, fnorm = 1
+    a = np.tile(np.array([0, 1, 2, 3], np.complex64), 3)
+    da = _dev(a)
+    sums = _dev(np.ones(3))
+    ctx.fft_shift_and_norm(da, 2, 2, 3, sums, 3)
+    assert np.array_equal(_host(da), np.tile(np.array([3, 2, 1, 0], np.complex64), 3))
+    # tests/imager_test.cpp:30-37 odd size 3x2 (the reference GPU throws here;
+    # the CPU swap semantics are kept)
+    b = _dev(np.arange(6).astype(np.complex64))
+    ctx.fft_shift_and_norm(b, 3, 2, 1, None)
+    assert np.array_equal(_host(b), np.arange(6)[::-1].astype(np.complex64))
+    _ = torch
+
+
+def test_fft_shift_and_norm_random(ctx):
+    rng = np.random.default_rng(5)
+    for n in (6, 7, 180):
+        imgs = (rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))).astype(np.complex64)
+        sums = np.array([12345.0, 7.0])
+        d = _dev(imgs)
+        ctx.fft_shift_and_norm(d, n, n, 2, _dev(sums), 2)
+        got = _host(d)
+        for k in range(2):
+            f = np.float32(1.0 / sums[k])
+            ref = oracle.fft_shift(imgs[k] * f)
+            assert np.array_equal(got[k], ref)
+
+
+def test_vector_sum_kat(ctx):
+    import torch
+    # tests/gpu_utils_test.cpp:20-57: 5 x 3000 values rand() % 50, exact
+    rng = np.random.default_rng(50)
+    vals = rng.integers(0, 50, (5, 3000)).astype(np.float32)
+    sums = torch.empty(5, dtype=torch.float64, device="cuda")
+    ctx.vector_sum(_dev(vals), 3000, 5, sums)
+    assert np.array_equal(_host(sums), vals.astype(np.float64).sum(1))
+
+
+def test_image_averaging_kat(ctx):
+    import torch
+    # tests/gpu_utils_test.cpp:91-124
+    n_img, side = 4, 512
+    vals = (np.arange(side * side) % 100).astype(np.float32)
+    imgs = np.repeat(vals[None], n_img, 0).astype(np.complex64)
+    out = torch.empty(side * side, dtype=torch.complex64, device="cuda")
+    ctx.image_averaging(_dev(imgs), side * side, n_img, out)
+    assert np.array_equal(_host(out), vals.astype(np.complex64))
+    rng = np.random.default_rng(1)
+    imgs = (rng.standard_normal((3, 100)) + 1j * rng.standard_normal((3, 100))).astype(np.complex64)
+    out = torch.empty(100, dtype=torch.complex64, device="cuda")
+    ctx.image_averaging(_dev(imgs), 100, 3, out)
+    assert np.array_equal(_host(out), oracle.image_averaging_complex(imgs))
+    ctx.image_averaging(_dev(imgs), 100, 3, out, real_only=True)
+    assert np.array_equal(_host(out), oracle.image_averaging_real(imgs))
+
+
+# ---------------------------------------------------------------- image
+
+@pytest.mark.parametrize("n", [64, 180, 256, 45])
+def test_image_from_grids(ctx, n):
+    import torch
+    o = synth.small(n_ant=40, n_ch=2, n_t=2, n_pixels=n, seed=n)
+    _say("image", n)
+    g, c, _ = oracle.gridding(o.vis, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v,
+                              o.min_uv, n, "XX")
+    ref = oracle.image(g, c)
+    g64, c64, _ = oracle.gridding(o.vis, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u,
+                                  o.delta_v, o.min_uv, n, "XX", precision="f64")
+    ref64 = oracle.image(g64, c64)
+    dg, dc = _dev(g), _dev(c)
+    imgs = torch.empty_like(dg)
+    ctx.image(dg, dc, n, o.n_images, imgs, zero_grids=True)
+    got = _host(imgs)
+    e32 = oracle.peak_normalised_error(got, ref)
+    e64 = oracle.peak_normalised_error(got, ref64)
+    _say(f"  N={n} err vs FP32 oracle {e32[0]:.2e} (rms {e32[1]:.2e}), vs FP64 {e64[0]:.2e}")
+    assert e32[0] < IMAGE_TOL
+    assert not _host(dg).any() and not _host(dc).any()   # zeroed for the next round
+
+
+def test_end_to_end_plain_path(ctx):
+    """corrections -> gridding -> image through the stage entry points, as the
+    reference's grid()/image() call them, vs the oracle's CPU pipeline."""
+    import torch
+    o = synth.small(n_ant=64, n_ch=4, n_t=2, n_pixels=128, seed=31, height_sigma_m=2.0)
+    ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, o.n_pixels)
+    ref64 = oracle.run_f64(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                           o.delta_u, o.delta_v, o.min_uv, o.n_pixels)
+    d = _obs_dev(o)
+    ctx.apply_geometric_corrections(d["vis"], o.n_ant, o.n_ch, o.n_t, d["w"], d["freq"])
+    ctx.apply_cable_corrections(d["vis"], o.n_ant, o.n_ch, o.n_t, d["cable"], d["freq"])
+    shape = (o.n_t, o.n_ch, o.n_pixels, o.n_pixels)
+    grids = torch.zeros(shape, dtype=torch.complex64, device="cuda")
+    counts = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    ctx.gridding(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["flags"], d["freq"],
+                 o.delta_u, o.delta_v, o.min_uv, o.n_pixels, "XX", 0, grids, counts)
+    imgs = torch.empty_like(grids)
+    ctx.image(grids, counts, o.n_pixels, o.n_images, imgs)
+    got = _host(imgs)
+    e32 = oracle.peak_normalised_error(got, ref)
+    e64 = oracle.peak_normalised_error(got, ref64)
+    _say(f"plain path err vs FP32 oracle {e32[0]:.2e}, vs FP64 oracle {e64[0]:.2e}")
+    assert e32[0] < IMAGE_TOL
+
+
+# -------------------------------------------------------------- batched
+
+def _batch(ctx, o, corrections=3, pol="XX"):
+    import torch
+    from imager_amd import capi
+    d = _obs_dev(o)
+    imgs = torch.empty((o.n_t, o.n_ch, o.n_pixels, o.n_pixels), dtype=torch.complex64,
+                       device="cuda")
+    sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
+    st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                       d["freq"], d["cable"], corrections, o.delta_u, o.delta_v, o.min_uv,
+                       o.n_pixels, pol, imgs, sums)
+    return d, imgs, sums, st
+
+
+@pytest.mark.parametrize("n,n_ant,pol", [(128, 48, "XX"), (180, 64, "I"), (256, 64, "V"),
+                                         (1024, 128, "XX"), (2048, 64, "YY"), (4096, 32, "XX")])
+def test_image_batch_matches_reference_pipeline(ctx, n, n_ant, pol):
+    o = synth.small(n_ant=n_ant, n_ch=3, n_t=2, n_pixels=n, seed=n + n_ant, height_sigma_m=2.0)
+    _say("batch", n, n_ant, pol)
+    ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, n, pol)
+    ref64 = oracle.run_f64(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                           o.delta_u, o.delta_v, o.min_uv, n, pol)
+    d, imgs, sums, st = _batch(ctx, o, 3, pol)
+    ctx.image_batch([st])
+    got = _host(imgs)
+    e32 = oracle.peak_normalised_error(got, ref)
+    e64 = oracle.peak_normalised_error(got, ref64)
+    _say(f"  err vs FP32 oracle {e32[0]:.2e} (rms {e32[1]:.2e}), vs FP64 {e64[0]:.2e}")
+    assert e32[0] < IMAGE_TOL
+    _, c_ref, _ = oracle.gridding(oracle.cable_corrections(
+        oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq), o.n_ant, o.cable, o.freq),
+        o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v, o.min_uv, n, pol)
+    assert np.array_equal(_host(sums), c_ref.reshape(o.n_images, -1).sum(1).astype(np.float64))
+
+
+def test_image_batch_graph_replays_are_identical(ctx):
+    import torch
+    o = synth.small(n_ant=64, n_ch=4, n_t=2, n_pixels=256, seed=77)
+    d, imgs, sums, st = _batch(ctx, o)
+    ctx.image_batch([st])
+    first = _host(imgs).copy()
+    for k in range(3):
+        imgs.fill_(float("nan"))
+        scratch = torch.randn(1 << 20, device="cuda")   # eager work between replays
+        scratch.mul_(2.0)
+        ctx.image_batch([st])
+        got = _host(imgs)
+        assert not np.isnan(got).any()
+        assert oracle.peak_normalised_error(got, first)[0] < 1e-6, k
+
+
+def test_image_batch_two_sets(ctx):
+    a = synth.small(n_ant=40, n_ch=2, n_t=1, n_pixels=128, seed=1)
+    b = synth.small(n_ant=56, n_ch=3, n_t=2, n_pixels=180, seed=2)
+    da, ia, sa, sta = _batch(ctx, a, 0)
+    db, ib, sb, stb = _batch(ctx, b, 0)
+    ctx.image_batch([sta, stb])
+    for o, im in ((a, ia), (b, ib)):
+        ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, False,
+                                False, o.delta_u, o.delta_v, o.min_uv, o.n_pixels)
+        assert oracle.peak_normalised_error(_host(im), ref)[0] < IMAGE_TOL
+
+
+# ------------------------------------------------------------ error paths
+
+def test_error_paths(ctx):
+    import torch
+    from imager_amd import capi
+    x = torch.zeros((1, 11, 11), dtype=torch.complex64, device="cuda")
+    with pytest.raises(capi.BlinkUnsupported):
+        ctx.fft2d_c2c(x, x, 11, 1)
+    g = torch.zeros((1, 1, 8, 8), dtype=torch.complex64, device="cuda")
+    c = torch.zeros((1, 1, 8, 8), dtype=torch.float32, device="cuda")
+    o = synth.small(n_ant=4, n_ch=1, n_t=1, n_pixels=8)
+    d = _obs_dev(o)
+    with pytest.raises(capi.BlinkInvalidArgument):
+        ctx.gridding(d["vis"], 4, 1, 1, d["u"], d["v"], d["flags"], d["freq"], 1.0, 1.0, 0.0, 8,
+                     7, 0, g, c)
+    with pytest.raises(capi.BlinkInvalidArgument):
+        ctx.image(g, c, 8, 1, g)   # images aliasing grids
+    o2 = synth.small(n_ant=8, n_ch=1, n_t=1, n_pixels=45)
+    _, _, _, st = _batch(ctx, o2)
+    with pytest.raises(capi.BlinkInvalidArgument):
+        ctx.image_batch([st])      # odd grid side in the batched path
