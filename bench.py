@@ -1,0 +1,311 @@
+#!/usr/bin/env python3
+"""Benchmark of the visibility -> dirty-image path on one MI355X.
+
+A step images one batch of synthetic visibilities, already resident in HBM:
+phase corrections (geometric + cable), nearest-neighbour gridding, 2-D
+backward FFT, fft_shift and normalisation, for every (channel, timestep) of
+the workload, through blink_image_batch (replayed hipGraph).  Prints ONE JSON
+line (the last line of stdout).
+
+Default workload: 128-tile MWA-like array (8256 baselines), 64 channels
+170-200 MHz x 10 timesteps = 640 images of 1024^2.
+
+Also measured, outside the timed region:
+  * per-kernel GPU time (HIP events recorded by the library around each launch,
+    on the stream it launches on) over K steps with the same kernels launched
+    directly -> roofline of the dominant kernel;
+  * the plain FP32 GPU path: one image at a time through the stage entry points
+    (corrections, atomic gridding, FFT, shift+norm), on a sample of images;
+  * parity of sample images against the CPU oracle (reference FP32 semantics)
+    and the FP64 oracle;
+  * the CPU baseline: the oracle's CPU/OpenMP restatement of the reference
+    pipeline on a bounded sample of the same workload, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "imaged channels/sec, 128-ant 1024² grid; % HBM roofline vs plain FP32 GPU path"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name: str):
+    from imager_amd import synth
+    if name == "mwa128_64ch10t_1024":
+        return synth.mwa_array(n_ch=64, n_t=10, n_pixels=1024, seed=1002, height_sigma_m=0.5)
+    if name == "mwa128_768ch_1024":
+        return synth.mwa_array(n_ch=768, n_t=1, n_pixels=1024, seed=1006,
+                               freq=synth.mwa_fine_channels(768), height_sigma_m=0.5)
+    if name == "eda2_1ch_180":
+        return synth.eda2_station(n_ch=1, n_t=1, n_pixels=180, seed=1001)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def algorithmic_bytes(kernel: str, n_bl: int, n_pix: int, n_img: int) -> float | None:
+    """Per-launch algorithmic HBM bytes of the batched-path kernels, from the
+    per-image figures: gridding = 8 n_bl (one pol) + 8 n_bl (u, v) + 12 N^2
+    (grid + count write), 2-D FFT = 16 N^2 (read grid + write image), shift and
+    normalisation fused = 0.  The row kernel grids and does the row FFTs, the
+    column kernel does the column FFTs with the fused shift+norm store."""
+    g = 16.0 * n_bl + 12.0 * n_pix * n_pix
+    f = 16.0 * n_pix * n_pix
+    if kernel == "grid_rows_fft":
+        return g * n_img
+    if kernel == "fft_cols":
+        return f * n_img
+    return None
+
+
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch measured with rocprofv3 PMC counters, when a summary
+    for this kernel and workload has been recorded under profiles/."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(obs, budget_s: float):
+    """Oracle CPU pipeline (reference FP32 semantics, OpenMP over images and FFT
+    rows/columns) on the first images of the workload."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+
+    def run(n_t, n_ch):
+        t0 = time.perf_counter()
+        oracle.run_f32(obs.vis[:n_t, :n_ch], obs.n_ant, obs.u, obs.v, obs.w, obs.flags,
+                       obs.freq[:n_ch], obs.cable, True, True, obs.delta_u, obs.delta_v,
+                       obs.min_uv, obs.n_pixels, "XX", threads)
+        return time.perf_counter() - t0
+
+    per_img = run(1, 1)
+    want = int(max(1, budget_s / max(per_img, 1e-4)))
+    n_ch = min(obs.n_ch, want)
+    n_t = min(obs.n_t, max(1, want // n_ch))
+    dt = run(n_t, n_ch)
+    k = n_t * n_ch
+    return {"value": k / dt, "unit": "images/s", "cores": threads, "kind": "port",
+            "sample": f"{n_t} timestep(s) x {n_ch} channel(s) = {k} images of the workload "
+                      f"({obs.n_ant} antennas, {obs.n_pixels}^2): geometric+cable corrections, "
+                      f"gridding, FFT, shift+norm in oracle/blink_oracle.c with {threads} OpenMP "
+                      f"threads, {dt:.1f} s"}
+
+
+def fft_mkl_rate(n_pix: int, seconds: float = 3.0):
+    """Batched 2-D FFT images/s on this host's CPU through torch (MKL), an
+    FFTW-class bound for the FFT stage of the CPU baseline."""
+    import torch
+    x = torch.randn(8, n_pix, n_pix, dtype=torch.complex64)
+    torch.fft.ifft2(x)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        torch.fft.ifft2(x)
+        n += x.shape[0]
+    return n / (time.perf_counter() - t0), torch.get_num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mwa128_64ch10t_1024")
+    ap.add_argument("--plain-images", type=int, default=64,
+                    help="images run through the plain one-image-at-a-time GPU path")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only warmup + timed steps (for rocprofv3 runs)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: every image is independent and this build runs on "
+              f"one MI355X; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported.",
+              file=sys.stderr, flush=True)
+        sys.exit(2)
+
+    import torch
+    import oracle
+    from imager_amd import capi
+
+    t_gen = time.perf_counter()
+    obs = make_workload(args.workload)
+    log(f"workload {args.workload}: {obs.n_ant} ant, {obs.n_bl} baselines, {obs.n_ch} ch x "
+        f"{obs.n_t} t = {obs.n_images} images of {obs.n_pixels}^2 "
+        f"(generated in {time.perf_counter() - t_gen:.1f} s)")
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    d = dict(vis=dev(obs.vis), u=dev(obs.u), v=dev(obs.v), w=dev(obs.w), flags=dev(obs.flags),
+             freq=dev(obs.freq), cable=dev(obs.cable))
+    N = obs.n_pixels
+    images = torch.empty((obs.n_t, obs.n_ch, N, N), dtype=torch.complex64, device="cuda")
+    sums = torch.empty(obs.n_images, dtype=torch.float64, device="cuda")
+    ctx = capi.Context(0)
+    corr = capi.CORR_GEOMETRIC | capi.CORR_CABLE
+    st = capi.make_set(d["vis"], obs.n_ant, obs.n_ch, obs.n_t, d["u"], d["v"], d["w"],
+                       d["flags"], d["freq"], d["cable"], corr, obs.delta_u, obs.delta_v,
+                       obs.min_uv, N, "XX", images, sums)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        ctx.image_batch([st], stream)
+    torch.cuda.synchronize()
+    log(f"warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.image_batch([st], stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = obs.n_images * args.steps / elapsed
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} images/s")
+    if args.profile_only:
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "images/s",
+                          "ms_per_step": ms_per_step, "profile_only": True}), flush=True)
+        ctx.close()
+        return
+
+    # per-kernel GPU time, HIP events on the launch stream, same K steps
+    ctx.profile_reset()
+    ctx.profile_enable(True)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.image_batch([st], stream)
+    torch.cuda.synchronize()
+    eager_ms = 1e3 * (time.perf_counter() - t1) / args.steps
+    ctx.profile_enable(False)
+    prof = ctx.profile()
+    kernels = {}
+    imgs_per_launch = {}
+    n_launch_rows = prof.get("grid_rows_fft", (0, 0))[0] // args.steps
+    chunk = -(-obs.n_images // max(n_launch_rows, 1))
+    for name, (n, ms) in prof.items():
+        kernels[name] = {"launches_per_step": n / args.steps, "avg_ms": ms / max(n, 1),
+                         "ms_per_step": ms / args.steps}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    log("kernels: " + ", ".join(f"{k} {v['ms_per_step']:.3f} ms/step ({v['avg_ms']*1e3:.1f} us x "
+                               f"{v['launches_per_step']:.0f})" for k, v in kernels.items()))
+    imgs_per_launch = chunk if dom in ("grid_rows_fft", "fft_cols") else obs.n_images
+    alg = algorithmic_bytes(dom, obs.n_bl, N, imgs_per_launch)
+    roof = None
+    if alg is not None:
+        avg_s = kernels[dom]["avg_ms"] * 1e-3
+        ach = alg / avg_s / 1e9
+        traffic = read_traffic(dom, args.workload)
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "images_per_launch": imgs_per_launch, "alg_bytes_per_launch": alg}
+    pipe_bytes = (16.0 * obs.n_bl + 28.0 * N * N)
+    pipeline = {"alg_bytes_per_image": pipe_bytes,
+                "achieved_gbs": value * pipe_bytes / 1e9,
+                "frac": value * pipe_bytes / 1e9 / HBM_PEAK_GBS}
+
+    # plain path: one image at a time through the stage entry points
+    plain = None
+    if args.plain_images > 0:
+        n_s = min(args.plain_images, obs.n_images)
+        n_bl = obs.n_bl
+        vis_s = d["vis"].reshape(obs.n_images, n_bl, 4)[:n_s].clone()
+        grid = torch.zeros((N, N), dtype=torch.complex64, device="cuda")
+        cnt = torch.zeros((N, N), dtype=torch.float32, device="cuda")
+        img = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+
+        def plain_image(i):
+            ch = i % obs.n_ch
+            v1 = vis_s[i]
+            f1 = d["freq"][ch:ch + 1]
+            ctx.apply_geometric_corrections(v1, obs.n_ant, 1, 1, d["w"], f1, stream)
+            ctx.apply_cable_corrections(v1, obs.n_ant, 1, 1, d["cable"], f1, stream)
+            ctx.gridding(v1, obs.n_ant, 1, 1, d["u"], d["v"], d["flags"], f1, obs.delta_u,
+                         obs.delta_v, obs.min_uv, N, "XX", 0, grid, cnt, algo=capi.GRID_ATOMIC,
+                         stream=stream)
+            ctx.image(grid, cnt, N, 1, img, zero_grids=True, stream=stream)
+
+        for i in range(min(4, n_s)):
+            plain_image(i)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for i in range(n_s):
+            plain_image(i)
+        torch.cuda.synchronize()
+        dtp = time.perf_counter() - tp
+        plain = {"value": n_s / dtp, "unit": "images/s", "images": n_s,
+                 "path": "per image: geometric + cable corrections, atomic gridding, "
+                         "row/column FFT, fused shift+norm (stage entry points)"}
+        plain["batched_speedup"] = value / plain["value"]
+        log(f"plain path: {plain['value']:.1f} images/s; batched is {plain['batched_speedup']:.1f}x")
+
+    # parity of sample images against the oracle
+    parity = None
+    if not args.no_parity:
+        got = images.cpu().numpy()
+        sample = [(0, 0), (obs.n_t - 1, obs.n_ch - 1)]
+        e32s, e64s = [], []
+        for (t, ch) in sample:
+            v1 = obs.vis[t:t + 1, ch:ch + 1]
+            a = (obs.n_ant, obs.u, obs.v, obs.w, obs.flags, obs.freq[ch:ch + 1], obs.cable, True,
+                 True, obs.delta_u, obs.delta_v, obs.min_uv, N, "XX")
+            r32, _ = oracle.run_f32(v1, *a)
+            r64 = oracle.run_f64(v1, *a)
+            e32s.append(oracle.peak_normalised_error(got[t, ch], r32[0, 0]))
+            e64s.append(oracle.peak_normalised_error(got[t, ch], r64[0, 0]))
+        parity = {"images": [f"t{t}_ch{c}" for t, c in sample],
+                  "max_err_vs_fp32_oracle": max(e[0] for e in e32s),
+                  "max_err_vs_fp64_oracle": max(e[0] for e in e64s),
+                  "rms_err_vs_fp64_oracle": max(e[1] for e in e64s),
+                  "tolerance": 1e-5}
+        log(f"parity: {parity}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(obs, args.cpu_seconds)
+        try:
+            rate, thr = fft_mkl_rate(N)
+            cpu["fft_stage_mkl_images_per_s"] = rate
+            cpu["fft_stage_mkl_threads"] = thr
+        except Exception as e:  # torch CPU FFT is optional context
+            cpu["fft_stage_mkl_error"] = str(e)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "images/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded point sources + noise)",
+        "config": {"workload": args.workload, "n_ant": obs.n_ant, "n_baselines": obs.n_bl,
+                   "n_ch": obs.n_ch, "n_t": obs.n_t, "n_pixels": N,
+                   "images_per_step": obs.n_images, "pol": "XX",
+                   "corrections": "geometric+cable", "path": "blink_image_batch (hipGraph)"},
+        "roofline": roof,
+        "pipeline_roofline": pipeline,
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+        "kernels": kernels,
+        "eager_ms_per_step": eager_ms,
+        "parity": parity,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

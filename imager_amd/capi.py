@@ -39,6 +39,7 @@ EXPORTED_SYMBOLS = (
     "blink_apply_geometric_corrections", "blink_apply_cable_corrections", "blink_gridding",
     "blink_uv_cells", "blink_fft2d_c2c", "blink_vector_sum", "blink_fft_shift_and_norm",
     "blink_image", "blink_image_averaging", "blink_image_batch", "blink_release_cache",
+    "blink_profile_enable", "blink_profile_get", "blink_profile_kernels", "blink_profile_reset",
 )
 
 
@@ -101,6 +102,11 @@ def load():
         "blink_image_averaging": (i, [vp, vp, i64, i, i, vp, vp]),
         "blink_image_batch": (i, [vp, ctypes.POINTER(ImageSet), i, vp]),
         "blink_release_cache": (i, [vp]),
+        "blink_profile_enable": (i, [vp, i]),
+        "blink_profile_get": (i, [vp, ctypes.c_char_p, ctypes.POINTER(i64),
+                                  ctypes.POINTER(ctypes.c_double)]),
+        "blink_profile_kernels": (i, [vp, ctypes.c_char_p, i]),
+        "blink_profile_reset": (i, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -239,6 +245,27 @@ class Context:
 
     def release_cache(self):
         self._check(load().blink_release_cache(self._h))
+
+    # measurement hooks
+    def profile_enable(self, on=True):
+        self._check(load().blink_profile_enable(self._h, 1 if on else 0))
+
+    def profile_reset(self):
+        self._check(load().blink_profile_reset(self._h))
+
+    def profile(self) -> dict:
+        """{kernel: (launches, total_ms)} since the last reset (waits for the events)."""
+        L = load()
+        buf = ctypes.create_string_buffer(4096)
+        self._check(L.blink_profile_kernels(self._h, buf, len(buf)))
+        out = {}
+        for name in filter(None, buf.value.decode().split(",")):
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            self._check(L.blink_profile_get(self._h, name.encode(), ctypes.byref(n),
+                                            ctypes.byref(ms)))
+            out[name] = (n.value, ms.value)
+        return out
 
 
 def make_set(vis, n_ant, n_ch, n_t, u_m, v_m, w_m, bl_flags, freq_hz, cable_m, corrections,

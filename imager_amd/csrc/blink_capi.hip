@@ -40,6 +40,16 @@ struct blink_ctx {
     hipStream_t capture_stream = nullptr;
     bool use_graphs = true;
 
+    // measurement hooks: event pairs around every launch while enabled
+    struct ProfRec {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    bool prof_on = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<ProfRec> prof_pending;
+    std::map<std::string, std::pair<int64_t, double>> prof_totals;
+
     int fail(int code, const char* fmt, ...) {
         char buf[512];
         va_list ap;
@@ -61,6 +71,66 @@ struct blink_ctx {
 #define CHECK_ARG(cond, ...)                                                \
     do {                                                                    \
         if (!(cond)) return ctx->fail(BLINK_ERR_INVALID_ARGUMENT, __VA_ARGS__); \
+    } while (0)
+
+namespace {
+
+hipEvent_t prof_take(blink_ctx* ctx) {
+    hipEvent_t e = nullptr;
+    if (!ctx->ev_pool.empty()) {
+        e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+hipEvent_t prof_open(blink_ctx* ctx, hipStream_t s) {
+    if (!ctx->prof_on) return nullptr;
+    hipEvent_t a = prof_take(ctx);
+    if (a && hipEventRecord(a, s) != hipSuccess) {
+        ctx->ev_pool.push_back(a);
+        return nullptr;
+    }
+    return a;
+}
+
+void prof_close(blink_ctx* ctx, hipStream_t s, const char* name, hipEvent_t a) {
+    if (!a) return;
+    hipEvent_t b = prof_take(ctx);
+    if (!b || hipEventRecord(b, s) != hipSuccess) {
+        ctx->ev_pool.push_back(a);
+        if (b) ctx->ev_pool.push_back(b);
+        return;
+    }
+    ctx->prof_pending.push_back({name, a, b});
+}
+
+int prof_collect(blink_ctx* ctx) {
+    for (auto& r : ctx->prof_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto& t = ctx->prof_totals[r.name];
+            t.first += 1;
+            t.second += ms;
+        }
+        ctx->ev_pool.push_back(r.a);
+        ctx->ev_pool.push_back(r.b);
+    }
+    ctx->prof_pending.clear();
+    return BLINK_OK;
+}
+
+}  // namespace
+
+// Launch wrapper: checks the launch and, while profiling, brackets it with
+// events on its stream.
+#define LAUNCH(name, stream, expr)                                     \
+    do {                                                               \
+        hipEvent_t pa_ = prof_open(ctx, (stream));                     \
+        HIP_TRY(expr);                                                 \
+        prof_close(ctx, (stream), (name), pa_);                        \
     } while (0)
 
 namespace {
@@ -135,7 +205,7 @@ int fft2d(blink_ctx* ctx, const float* in, float* out, int n, int batch, int sig
     if (rc) return rc;
     const float* tw;
     if ((rc = twiddles(ctx, n, &tw))) return rc;
-    HIP_TRY(launch_fft_rows(in, out, plan, (int64_t)batch * n, sign, tw, stream));
+    LAUNCH("fft_rows", stream, launch_fft_rows(in, out, plan, (int64_t)batch * n, sign, tw, stream));
     // The shifted store is fused into the in-place column pass when a
     // workgroup can hold a column block and its shift partner; otherwise a
     // separate shift+norm pass follows.
@@ -149,8 +219,8 @@ int fft2d(blink_ctx* ctx, const float* in, float* out, int n, int batch, int sig
     a.shift = fused ? 1 : 0;
     a.sums = fused ? sums : nullptr;
     a.n_ch = 1;
-    HIP_TRY(launch_fft_cols(a, plan, batch, sign, tw, stream));
-    if (shift && !fused) HIP_TRY(launch_shift_norm(out, n, n, batch, sums, batch, stream));
+    LAUNCH("fft_cols", stream, launch_fft_cols(a, plan, batch, sign, tw, stream));
+    if (shift && !fused) LAUNCH("shift_norm", stream, launch_shift_norm(out, n, n, batch, sums, batch, stream));
     return BLINK_OK;
 }
 
@@ -224,6 +294,8 @@ BLINK_API void blink_destroy(blink_ctx* ctx) {
     (void)hipDeviceSynchronize();
     blink_release_cache(ctx);
     for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+    prof_collect(ctx);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     delete ctx;
 }
@@ -255,8 +327,9 @@ static int phase_correct(blink_ctx* ctx, float* vis, int n_ant, int n_ch, int n_
     CHECK_ARG(!(which & 1u) || w_m, "w_m must be non-null");
     CHECK_ARG(!(which & 2u) || cable_m, "cable_m must be non-null");
     if ((rc = set_device(ctx))) return rc;
-    HIP_TRY(launch_phase_correct(vis, n_ant, n_ch, n_t, w_m, cable_m, freq_hz, which,
-                                 (hipStream_t)stream));
+    LAUNCH("phase_correct", (hipStream_t)stream,
+           launch_phase_correct(vis, n_ant, n_ch, n_t, w_m, cable_m, freq_hz, which,
+                                (hipStream_t)stream));
     return BLINK_OK;
 }
 
@@ -283,8 +356,9 @@ BLINK_API int blink_uv_cells(blink_ctx* ctx, const float* u_m, const float* v_m,
     CHECK_ARG(u_m && v_m && bl_flags && freq_hz && cells, "null pointer argument");
     int rc = set_device(ctx);
     if (rc) return rc;
-    HIP_TRY(launch_uv_cells(u_m, v_m, bl_flags, n_bl, freq_hz, n_ch, delta_u, delta_v, min_uv,
-                            n_pixels, cells, (hipStream_t)stream));
+    LAUNCH("uv_cells", (hipStream_t)stream,
+           launch_uv_cells(u_m, v_m, bl_flags, n_bl, freq_hz, n_ch, delta_u, delta_v, min_uv,
+                           n_pixels, cells, (hipStream_t)stream));
     return BLINK_OK;
 }
 
@@ -299,8 +373,9 @@ static int build_plan(blink_ctx* ctx, const float* u_m, const float* v_m, const 
     if ((rc = scratch(ctx, a.c_str(), sizeof(int) * (size_t)n_ch * (n_pixels + 1), &rs))) return rc;
     if ((rc = scratch(ctx, b.c_str(), sizeof(uint32_t) * 2 * (size_t)n_ch * std::max(n_bl, 1), &en)))
         return rc;
-    HIP_TRY(launch_plan_rows(u_m, v_m, flags, n_bl, freq_hz, n_ch, du, dv, min_uv, n_pixels,
-                             (int*)rs, (uint32_t*)en, stream));
+    LAUNCH("plan_rows", stream,
+           launch_plan_rows(u_m, v_m, flags, n_bl, freq_hz, n_ch, du, dv, min_uv, n_pixels,
+                            (int*)rs, (uint32_t*)en, stream));
     *row_start = (int*)rs;
     *entries = (uint32_t*)en;
     return BLINK_OK;
@@ -326,8 +401,9 @@ BLINK_API int blink_gridding(blink_ctx* ctx, const float* vis, int n_ant, int n_
     const int n_bl = n_ant * (n_ant + 1) / 2;
     const int n_img = n_ch * n_t;
     if (algo == BLINK_GRID_ATOMIC) {
-        HIP_TRY(launch_grid_atomic(vis, n_ant, n_ch, n_t, u_m, v_m, bl_flags, freq_hz, delta_u,
-                                   delta_v, min_uv, n_pixels, pol, grids, counts, s));
+        LAUNCH("grid_atomic", s,
+               launch_grid_atomic(vis, n_ant, n_ch, n_t, u_m, v_m, bl_flags, freq_hz, delta_u,
+                                  delta_v, min_uv, n_pixels, pol, grids, counts, s));
     } else {
         int* rs;
         uint32_t* en;
@@ -341,10 +417,11 @@ BLINK_API int blink_gridding(blink_ctx* ctx, const float* vis, int n_ant, int n_
         g.pol = pol;
         g.n_bl = n_bl;
         g.n_ch = n_ch;
-        HIP_TRY(launch_grid_tiles(g, rs, en, n_img, n_pixels, grids, counts, s));
+        LAUNCH("grid_tiles", s, launch_grid_tiles(g, rs, en, n_img, n_pixels, grids, counts, s));
     }
     if (weighting == BLINK_WEIGHTING_UNIFORM)
-        HIP_TRY(launch_uniform_weight(grids, counts, (int64_t)n_img * n_pixels * n_pixels, s));
+        LAUNCH("uniform_weight", s,
+               launch_uniform_weight(grids, counts, (int64_t)n_img * n_pixels * n_pixels, s));
     return BLINK_OK;
 }
 
@@ -372,7 +449,8 @@ BLINK_API int blink_vector_sum(blink_ctx* ctx, const float* values, int64_t imag
     CHECK_ARG(image_size >= 0 && n_images >= 0, "invalid sizes");
     int rc = set_device(ctx);
     if (rc) return rc;
-    HIP_TRY(launch_vector_sum(values, image_size, n_images, sums, (hipStream_t)stream));
+    LAUNCH("vector_sum", (hipStream_t)stream,
+           launch_vector_sum(values, image_size, n_images, sums, (hipStream_t)stream));
     return BLINK_OK;
 }
 
@@ -385,7 +463,8 @@ BLINK_API int blink_fft_shift_and_norm(blink_ctx* ctx, float* images, int nx, in
     CHECK_ARG(!sums || n_sums >= 1, "n_sums must be >= 1 when sums are given");
     int rc = set_device(ctx);
     if (rc) return rc;
-    HIP_TRY(launch_shift_norm(images, nx, ny, n_images, sums, n_sums, (hipStream_t)stream));
+    LAUNCH("shift_norm", (hipStream_t)stream,
+           launch_shift_norm(images, nx, ny, n_images, sums, n_sums, (hipStream_t)stream));
     return BLINK_OK;
 }
 
@@ -404,13 +483,13 @@ BLINK_API int blink_image(blink_ctx* ctx, float* grids, float* counts, int n_pix
     void* sums;
     if ((rc = scratch(ctx, "image_sums", sizeof(double) * (size_t)n_images, &sums))) return rc;
     const int64_t gs = (int64_t)n_pixels * n_pixels;
-    HIP_TRY(launch_vector_sum(counts, gs, n_images, (double*)sums, s));
+    LAUNCH("vector_sum", s, launch_vector_sum(counts, gs, n_images, (double*)sums, s));
     if ((rc = fft2d(ctx, grids, images, n_pixels, n_images, BLINK_FFT_BACKWARD, true,
                     (const double*)sums, s)))
         return rc;
     if (zero_grids) {
-        HIP_TRY(launch_zero(grids, (int64_t)sizeof(float) * 2 * gs * n_images, s));
-        HIP_TRY(launch_zero(counts, (int64_t)sizeof(float) * gs * n_images, s));
+        LAUNCH("zero", s, launch_zero(grids, (int64_t)sizeof(float) * 2 * gs * n_images, s));
+        LAUNCH("zero", s, launch_zero(counts, (int64_t)sizeof(float) * gs * n_images, s));
     }
     return BLINK_OK;
 }
@@ -422,7 +501,8 @@ BLINK_API int blink_image_averaging(blink_ctx* ctx, const float* images, int64_t
     CHECK_ARG(image_size >= 0 && n_images >= 1, "invalid sizes");
     int rc = set_device(ctx);
     if (rc) return rc;
-    HIP_TRY(launch_average(images, image_size, n_images, real_only, avg, (hipStream_t)stream));
+    LAUNCH("average", (hipStream_t)stream,
+           launch_average(images, image_size, n_images, real_only, avg, (hipStream_t)stream));
     return BLINK_OK;
 }
 
@@ -491,9 +571,11 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
         FftPlan plan;
         make_fft_plan(n, plan);
         const float* tw = ctx->twiddles[n];
-        HIP_TRY(launch_zero(sc[i].isums, sizeof(uint32_t) * (int64_t)n_img, s));
-        HIP_TRY(launch_plan_rows(st.u_m, st.v_m, st.bl_flags, n_bl, st.freq_hz, st.n_ch, st.delta_u,
-                                 st.delta_v, st.min_uv, n, sc[i].row_start, sc[i].entries, s));
+        LAUNCH("zero", s, launch_zero(sc[i].isums, sizeof(uint32_t) * (int64_t)n_img, s));
+        LAUNCH("plan_rows", s,
+               launch_plan_rows(st.u_m, st.v_m, st.bl_flags, n_bl, st.freq_hz, st.n_ch,
+                                st.delta_u, st.delta_v, st.min_uv, n, sc[i].row_start,
+                                sc[i].entries, s));
         GatherArgs g{};
         g.vis = (const float4*)st.vis;
         g.w_m = st.w_m;
@@ -506,8 +588,9 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
         const int64_t gs = (int64_t)n * n;
         for (int img0 = 0; img0 < n_img; img0 += sc[i].chunk) {
             const int cnt = std::min(sc[i].chunk, n_img - img0);
-            HIP_TRY(launch_grid_rows_fft(g, sc[i].row_start, sc[i].entries, plan, img0, cnt,
-                                         sc[i].inter, sc[i].isums, tw, s));
+            LAUNCH("grid_rows_fft", s,
+                   launch_grid_rows_fft(g, sc[i].row_start, sc[i].entries, plan, img0, cnt,
+                                        sc[i].inter, sc[i].isums, tw, s));
             ColArgs a{};
             a.in = (const float2*)sc[i].inter;
             a.out = (float2*)(st.images + 2 * gs * img0);
@@ -518,9 +601,11 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
             a.isums = sc[i].isums;
             a.row_start = sc[i].row_start;
             a.n_ch = st.n_ch;
-            HIP_TRY(launch_fft_cols(a, plan, cnt, BLINK_FFT_BACKWARD, tw, s));
+            LAUNCH("fft_cols", s, launch_fft_cols(a, plan, cnt, BLINK_FFT_BACKWARD, tw, s));
         }
-        if (st.count_sums) HIP_TRY(launch_isums_to_double(sc[i].isums, st.count_sums, n_img, s));
+        if (st.count_sums)
+            LAUNCH("isums_to_double", s,
+                   launch_isums_to_double(sc[i].isums, st.count_sums, n_img, s));
     }
     return BLINK_OK;
 }
@@ -547,7 +632,7 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
     std::vector<SetScratch> sc;
     if ((rc = reserve_batch(ctx, sets, n_sets, sc))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    if (!ctx->use_graphs) return enqueue_batch(ctx, sets, n_sets, sc, s);
+    if (!ctx->use_graphs || ctx->prof_on) return enqueue_batch(ctx, sets, n_sets, sc, s);
 
     std::string key((const char*)sets, sizeof(blink_image_set) * (size_t)n_sets);
     auto it = ctx->graphs.find(key);
@@ -573,6 +658,47 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
         it = ctx->graphs.emplace(key, ge).first;
     }
     HIP_TRY(hipGraphLaunch(it->second.exec, s));
+    return BLINK_OK;
+}
+
+// ------------------------------------------------------ measurement hooks
+
+BLINK_API int blink_profile_enable(blink_ctx* ctx, int enable) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    ctx->prof_on = enable != 0;
+    return BLINK_OK;
+}
+
+BLINK_API int blink_profile_get(blink_ctx* ctx, const char* kernel, int64_t* launches,
+                                double* total_ms) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(kernel && launches && total_ms, "null pointer argument");
+    prof_collect(ctx);
+    auto it = ctx->prof_totals.find(kernel);
+    *launches = it == ctx->prof_totals.end() ? 0 : it->second.first;
+    *total_ms = it == ctx->prof_totals.end() ? 0.0 : it->second.second;
+    return BLINK_OK;
+}
+
+BLINK_API int blink_profile_kernels(blink_ctx* ctx, char* buf, int buf_len) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(buf && buf_len > 0, "need an output buffer");
+    prof_collect(ctx);
+    std::string names;
+    for (auto& kv : ctx->prof_totals) {
+        if (!names.empty()) names += ",";
+        names += kv.first;
+    }
+    CHECK_ARG((int)names.size() < buf_len, "buffer too small (%d bytes needed)",
+              (int)names.size() + 1);
+    memcpy(buf, names.c_str(), names.size() + 1);
+    return BLINK_OK;
+}
+
+BLINK_API int blink_profile_reset(blink_ctx* ctx) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    prof_collect(ctx);
+    ctx->prof_totals.clear();
     return BLINK_OK;
 }
 

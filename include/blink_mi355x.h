@@ -199,6 +199,25 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
 /* Releases every captured graph and scratch buffer of the context. */
 BLINK_API int blink_release_cache(blink_ctx* ctx);
 
+/* ------------------------------------------------------- measurement hooks */
+
+/* No reference counterpart (the reference times hipFFT with hipEvents inline,
+ * src/gpu/pacer_imager_hip.cpp:79-101).  enable != 0: every kernel this
+ * context launches is bracketed by a pair of HIP events recorded on the
+ * stream it is launched on, and blink_image_batch runs its kernels directly
+ * instead of replaying a graph.  enable == 0 stops recording (the totals are
+ * kept until blink_profile_reset). */
+BLINK_API int blink_profile_enable(blink_ctx* ctx, int enable);
+/* Number of launches and total GPU milliseconds of the kernel `kernel`
+ * (e.g. "grid_rows_fft", "fft_cols") since the last reset; waits for the
+ * recorded events.  Unknown names give 0 launches. */
+BLINK_API int blink_profile_get(blink_ctx* ctx, const char* kernel, int64_t* launches,
+                                double* total_ms);
+/* Comma-separated names of every kernel recorded since the last reset
+ * (written into buf, at most buf_len bytes including the terminating 0). */
+BLINK_API int blink_profile_kernels(blink_ctx* ctx, char* buf, int buf_len);
+BLINK_API int blink_profile_reset(blink_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif
