@@ -265,7 +265,7 @@ BLINK_API int blink_create(int device, blink_ctx** out) {
     if (lds <= 0) lds = 64 * 1024;
     g_lds_limit = std::min<int64_t>(lds, 160 * 1024) - 4 * 1024;
     g_lds_budget = g_lds_limit - 12 * 1024;
-    if (set_kernel_attributes() != hipSuccess) {
+    if (set_kernel_attributes() != hipSuccess || set_batch_kernel_attributes() != hipSuccess) {
         (void)hipGetLastError();
     }
     if (hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking) != hipSuccess) {
@@ -586,8 +586,18 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
         g.n_bl = n_bl;
         g.n_ch = st.n_ch;
         const int64_t gs = (int64_t)n * n;
+        const bool reg = batch_reg_supported(n);
         for (int img0 = 0; img0 < n_img; img0 += sc[i].chunk) {
             const int cnt = std::min(sc[i].chunk, n_img - img0);
+            if (reg) {
+                LAUNCH("grid_rows_fft", s,
+                       launch_grid_rows_reg(g, sc[i].row_start, sc[i].entries, n, img0, cnt,
+                                            sc[i].inter, sc[i].isums, tw, s));
+                LAUNCH("fft_cols", s,
+                       launch_cols_reg(sc[i].inter, st.images, n, img0, cnt, st.n_ch,
+                                       sc[i].row_start, sc[i].isums, tw, s));
+                continue;
+            }
             LAUNCH("grid_rows_fft", s,
                    launch_grid_rows_fft(g, sc[i].row_start, sc[i].entries, plan, img0, cnt,
                                         sc[i].inter, sc[i].isums, tw, s));
