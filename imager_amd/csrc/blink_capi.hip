@@ -26,7 +26,12 @@ struct DeviceBuffer {
 struct GraphEntry {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
 };
+
+// Captured graphs kept per context; the least recently used is dropped
+// beyond this.
+constexpr size_t kMaxGraphs = 16;
 
 }  // namespace
 
@@ -37,6 +42,7 @@ struct blink_ctx {
     std::map<int, float*> twiddles;               // N -> device table of N float2
     std::map<std::string, DeviceBuffer> scratch;  // named grow-only buffers
     std::map<std::string, GraphEntry> graphs;     // descriptor bytes -> graph
+    uint64_t graph_clock = 0;
     hipStream_t capture_stream = nullptr;
     bool use_graphs = true;
 
@@ -665,8 +671,19 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
             (void)hipGraphDestroy(graph);
             return ctx->fail(BLINK_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         }
+        if (ctx->graphs.size() >= kMaxGraphs) {
+            auto lru = ctx->graphs.begin();
+            for (auto g = ctx->graphs.begin(); g != ctx->graphs.end(); ++g)
+                if (g->second.last_use < lru->second.last_use) lru = g;
+            // a graph still executing is destroyed only after it completes
+            HIP_TRY(hipDeviceSynchronize());
+            (void)hipGraphExecDestroy(lru->second.exec);
+            (void)hipGraphDestroy(lru->second.graph);
+            ctx->graphs.erase(lru);
+        }
         it = ctx->graphs.emplace(key, ge).first;
     }
+    it->second.last_use = ++ctx->graph_clock;
     HIP_TRY(hipGraphLaunch(it->second.exec, s));
     return BLINK_OK;
 }

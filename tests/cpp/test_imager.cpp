@@ -1,0 +1,203 @@
+// The shape of the reference's tests/imager_test.cpp for CPacerImagerMI355X:
+// the fft_shift known-answer tests, then imaging runs checked against the CPU
+// oracle (oracle/blink_oracle.c) on identical synthetic inputs.
+#include <cstring>
+#include <stdexcept>
+
+#include "blink_oracle.h"
+#include "common.hpp"
+#include "pacer_imager_mi355x.h"
+
+// tests/imager_test.cpp:21-40
+void test_fft_shift_simple() {
+    std::complex<float> one[]{{0, 0}, {1, 0}, {2, 0}, {3, 0}};
+    std::complex<float> ref_one[]{{3, 0}, {2, 0}, {1, 0}, {0, 0}};
+    fft_shift(one, 2, 2);
+    for (int x = 0; x < 4; x++)
+        if (one[x] != ref_one[x]) throw TestFailed("test_fft_shift_simple: test 1, wrong output.");
+    std::complex<float> two[]{{0, 0}, {1, 0}, {2, 0}, {3, 0}, {4, 0}, {5, 0}};
+    std::complex<float> ref_two[]{{5, 0}, {4, 0}, {3, 0}, {2, 0}, {1, 0}, {0, 0}};
+    fft_shift(two, 3, 2);
+    for (int x = 0; x < 6; x++)
+        if (two[x] != ref_two[x]) throw TestFailed("test_fft_shift_simple: test 2, wrong output.");
+}
+
+struct Array {
+    int n_ant;
+    std::vector<double> x, y, z, cable;
+};
+
+static Array make_array(int n_ant, double radius, uint64_t seed) {
+    SplitMix64 g(seed);
+    Array a{n_ant, {}, {}, {}, {}};
+    for (int i = 0; i < n_ant; i++) {
+        const double r = radius * std::sqrt(g.uniform()), th = 6.283185307179586 * g.uniform();
+        a.x.push_back(r * std::cos(th));
+        a.y.push_back(r * std::sin(th));
+        a.z.push_back(0.5 * g.normal());
+        a.cable.push_back(50.0 + 250.0 * g.uniform());
+    }
+    return a;
+}
+
+// Point sources + noise, all 4 pol products, [t][ch][bl][4].
+static Visibilities make_vis(const Array& a, int n_ch, int n_t, const std::vector<double>& freq,
+                             uint64_t seed, ObservationInfo& info) {
+    const int n_bl = a.n_ant * (a.n_ant + 1) / 2;
+    SplitMix64 g(seed);
+    double l[4], m[4], s[4];
+    for (int k = 0; k < 4; k++) {
+        l[k] = 0.6 * (g.uniform() - 0.5);
+        m[k] = 0.6 * (g.uniform() - 0.5);
+        s[k] = 1.0 + 9.0 * g.uniform();
+    }
+    info.nAntennas = a.n_ant;
+    info.nFrequencies = n_ch;
+    info.nTimesteps = n_t;
+    info.frequencyResolution = 0.04;
+    info.coarseChannelBandwidth = 1.28;
+    info.coarseChannel = 133;
+    MemoryBuffer<std::complex<float>> buf((size_t)n_t * n_ch * n_bl * 4);
+    for (int t = 0; t < n_t; t++)
+        for (int ch = 0; ch < n_ch; ch++) {
+            const double lam = 299792458.0 / freq[ch];
+            for (int bl = 0; bl < n_bl; bl++) {
+                const int a1 = (int)(-0.5 + std::sqrt(0.25 + 2.0 * bl)), a2 = bl - (a1 + 1) * a1 / 2;
+                const double u = -(a.x[a1] - a.x[a2]), v = -(a.y[a1] - a.y[a2]);
+                std::complex<double> xx = 0;
+                for (int k = 0; k < 4; k++)
+                    xx += s[k] * std::exp(std::complex<double>(0, -6.283185307179586 * (u * l[k] + v * m[k]) / lam));
+                std::complex<float>* rec = buf.data() + (((size_t)t * n_ch + ch) * n_bl + bl) * 4;
+                for (int p = 0; p < 4; p++)
+                    rec[p] = std::complex<float>((float)(0.1 * g.normal()), (float)(0.1 * g.normal()));
+                rec[0] += std::complex<float>(xx);
+                rec[3] += std::complex<float>(0.9 * xx);
+            }
+        }
+    return Visibilities(std::move(buf), info, 1, 1);
+}
+
+static void check_against_oracle(CPacerImagerMI355X& imager, Images& images, const Array& a,
+                                 const std::vector<float>& vis0, const std::vector<double>& freq,
+                                 int n_ch, int n_t, bool geom, bool cable, int pol, double tol,
+                                 const char* what) {
+    const int n = imager.n_pixels;
+    const size_t gs = (size_t)n * n, n_img = (size_t)n_ch * n_t;
+    const int n_bl = a.n_ant * (a.n_ant + 1) / 2;
+    std::vector<float> u(n_bl), v(n_bl), w(n_bl);
+    oracle_zenith_uvw(a.n_ant, a.x.data(), a.y.data(), a.z.data(), u.data(), v.data(), w.data());
+    std::vector<int> flags(n_bl);
+    oracle_baseline_flags(a.n_ant, imager.m_FlaggedAntennas.data(),
+                          (int)imager.m_FlaggedAntennas.size(), flags.data());
+    double um, vm, du, dv;
+    oracle_uv_max(u.data(), v.data(), flags.data(), n_bl, &um, &vm);
+    oracle_delta_uv(imager.oversampling_factor, um, vm, n, &du, &dv);
+    if (du != imager.delta_u || dv != imager.delta_v) throw TestFailed(std::string(what) + ": delta_u/v differ");
+    std::vector<float> vis(vis0), grids(2 * gs * n_img), counts(gs * n_img), ref(2 * gs * n_img);
+    oracle_run_f32(vis.data(), a.n_ant, n_ch, n_t, u.data(), v.data(), w.data(), flags.data(),
+                   freq.data(), a.cable.data(), geom, cable, du, dv, imager.min_uv, n, pol,
+                   grids.data(), counts.data(), ref.data(), 0);
+    images.to_cpu();
+    const double err = peak_normalised_error(images.data(), (const std::complex<float>*)ref.data(), gs * n_img);
+    std::cout << what << ": peak-normalised error vs oracle " << err << std::endl;
+    if (!(err < tol)) throw TestFailed(std::string(what) + ": image differs from the oracle");
+}
+
+// tests/imager_test.cpp:42-58 images real MWA data with the CPU and GPU imagers
+// and compares nothing; here the imager is run on synthetic data and the
+// images are compared with the oracle's CPU restatement of the reference.
+void test_imager_run() {
+    const int n_ch = 3, n_t = 2, N = 128;
+    Array a = make_array(48, 17.5, 11);
+    std::vector<double> freq;
+    for (int ch = 0; ch < n_ch; ch++) freq.push_back((133 * 1.28 - 0.64 + 0.04 * ch) * 1e6);
+    ObservationInfo info;
+    Visibilities xcorr = make_vis(a, n_ch, n_t, freq, 5, info);
+    std::vector<float> vis0((float*)xcorr.data(), (float*)xcorr.data() + 2 * xcorr.size());
+    std::vector<int> flagged{3, 17};
+    CPacerImagerMI355X imager("", N, flagged, false, Polarization::XX, 2.0f);
+    imager.SetAntennaPositions(a.x, a.y, a.z, a.cable);
+    Images images = imager.run(xcorr);
+    if (images.size() != (size_t)n_ch * n_t || images.side_size != (unsigned)N)
+        throw TestFailed("test_imager_run: wrong image shape");
+    check_against_oracle(imager, images, a, vis0, freq, n_ch, n_t, true, true, ORACLE_POL_XX, 1e-5,
+                         "run (corrections in place + gridding + image)");
+    // visibilities were corrected in place, as the reference does
+    xcorr.to_cpu();
+    std::vector<float> corrected(vis0);
+    std::vector<float> w(a.n_ant * (a.n_ant + 1) / 2), u(w.size()), v(w.size());
+    oracle_zenith_uvw(a.n_ant, a.x.data(), a.y.data(), a.z.data(), u.data(), v.data(), w.data());
+    oracle_geometric_corrections(corrected.data(), a.n_ant, n_ch, n_t, w.data(), freq.data());
+    oracle_cable_corrections(corrected.data(), a.n_ant, n_ch, n_t, a.cable.data(), freq.data());
+    double worst = 0;
+    for (size_t i = 0; i + 1 < corrected.size(); i += 2) {
+        const std::complex<double> g(((float*)xcorr.data())[i], ((float*)xcorr.data())[i + 1]);
+        const std::complex<double> r(corrected[i], corrected[i + 1]);
+        worst = std::max(worst, std::abs(g - r) / std::max(std::abs(r), 1e-30));
+    }
+    if (!(worst < 2e-6)) throw TestFailed("test_imager_run: corrected visibilities differ");
+}
+
+void test_imager_batched_and_pols() {
+    const int n_ch = 2, n_t = 2, N = 256;
+    Array a = make_array(64, 17.5, 21);
+    std::vector<double> freq{159.375e6, 160.0e6};
+    ObservationInfo info;
+    Visibilities xcorr = make_vis(a, n_ch, n_t, freq, 6, info);
+    std::vector<float> vis0((float*)xcorr.data(), (float*)xcorr.data() + 2 * xcorr.size());
+    const Polarization pols[] = {Polarization::XX, Polarization::YY, Polarization::I, Polarization::V};
+    for (int p = 0; p < 4; p++) {
+        CPacerImagerMI355X imager("", N, {}, false, pols[p], 2.0f);
+        imager.SetAntennaPositions(a.x, a.y, a.z, a.cable);
+        imager.m_fFrequencyMHz = -1;
+        // get_frequency_hz: coarse channel 133, so override with explicit values
+        Visibilities copy(MemoryBuffer<std::complex<float>>(xcorr.size()), info, 1, 1);
+        std::memcpy((void*)copy.data(), vis0.data(), sizeof(float) * vis0.size());
+        copy.obsInfo.frequencyResolution = 0.625;
+        copy.obsInfo.coarseChannelBandwidth = 1.25;
+        copy.obsInfo.coarseChannel = 128;  // 159.375 MHz + 0.625 ch
+        Images images = imager.run_batched(copy);
+        check_against_oracle(imager, images, a, vis0, freq, n_ch, n_t, true, true, p, 1e-5,
+                             "run_batched");
+    }
+}
+
+// src/pacer_imager.cpp:135: image() before grid() throws std::runtime_error.
+void test_error_paths() {
+    CPacerImagerMI355X imager("", 64, {}, false);
+    ObservationInfo info;
+    bool thrown = false;
+    try {
+        imager.image(info);
+    } catch (const std::runtime_error&) {
+        thrown = true;
+    }
+    if (!thrown) throw TestFailed("image() before grid() must throw std::runtime_error");
+    Array a = make_array(8, 10, 3);
+    imager.SetAntennaPositions(a.x, a.y, a.z, a.cable);
+    std::vector<double> freq{150e6};
+    Visibilities xcorr = make_vis(a, 1, 1, freq, 1, info);
+    imager.m_nConvolvingKernelSize = 7;
+    thrown = false;
+    try {
+        imager.run(xcorr);
+    } catch (const std::runtime_error&) {
+        thrown = true;
+    }
+    if (!thrown) throw TestFailed("convolutional gridding must be refused");
+}
+
+int main() {
+    CPacerImagerMI355X::SetDebugLevel(IMAGER_WARNING_LEVEL);
+    try {
+        RUN_TEST(test_fft_shift_simple);
+        RUN_TEST(test_imager_run);
+        RUN_TEST(test_imager_batched_and_pols);
+        RUN_TEST(test_error_paths);
+    } catch (std::exception& ex) {
+        std::cerr << ex.what() << std::endl;
+        return 1;
+    }
+    std::cout << "All tests passed." << std::endl;
+    return 0;
+}
