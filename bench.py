@@ -135,6 +135,9 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="only warmup + timed steps (for rocprofv3 runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -147,6 +150,16 @@ def main():
     import torch
     import oracle
     from imager_amd import capi
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 20
+        x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n, "launches": 3}), flush=True)
+        return
 
     t_gen = time.perf_counter()
     obs = make_workload(args.workload)
@@ -301,6 +314,7 @@ def main():
         "plain_gpu": plain,
         "kernels": kernels,
         "eager_ms_per_step": eager_ms,
+        "batch_tiles": ctx.batch_tiles(N),
         "parity": parity,
     }
     ctx.close()

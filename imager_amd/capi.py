@@ -40,6 +40,7 @@ EXPORTED_SYMBOLS = (
     "blink_uv_cells", "blink_fft2d_c2c", "blink_vector_sum", "blink_fft_shift_and_norm",
     "blink_image", "blink_image_averaging", "blink_image_batch", "blink_release_cache",
     "blink_profile_enable", "blink_profile_get", "blink_profile_kernels", "blink_profile_reset",
+    "blink_batch_tiles",
 )
 
 
@@ -107,6 +108,7 @@ def load():
                                   ctypes.POINTER(ctypes.c_double)]),
         "blink_profile_kernels": (i, [vp, ctypes.c_char_p, i]),
         "blink_profile_reset": (i, [vp]),
+        "blink_batch_tiles": (i, [vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -245,6 +247,13 @@ class Context:
 
     def release_cache(self):
         self._check(load().blink_release_cache(self._h))
+
+    def batch_tiles(self, n_pixels):
+        """(rows per row tile, columns per column block) of the batched kernels."""
+        r, c = ctypes.c_int(), ctypes.c_int()
+        self._check(load().blink_batch_tiles(self._h, int(n_pixels), ctypes.byref(r),
+                                             ctypes.byref(c)))
+        return r.value, c.value
 
     # measurement hooks
     def profile_enable(self, on=True):

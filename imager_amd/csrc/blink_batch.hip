@@ -4,11 +4,19 @@
 //   k_grid_rows_reg : (row tile, image) -> plan entries gathered (pol, phase
 //                     correction, NaN/MAX_VIS) and accumulated in LDS, each
 //                     non-empty row transformed, the tile written TRANSPOSED
-//                     to the row-pass buffer inter[img][u][v];
+//                     to the row-pass buffer inter[img][x][v];
 //   k_cols_reg      : (column block, image) -> each column read contiguously
 //                     from inter (rows with no plan entries read as zero),
 //                     transformed, scaled by (float)(1/sum counts) and stored
 //                     fft-shifted into images[img][y][x].
+//
+// Hermitian half: every visibility is gridded at (u, v) and conjugated at
+// (-u, -v), so the grid satisfies G[-v][-u] = conj(G[v][u]) and the row
+// transforms R_v(x) = sum_u G[v][u] w^(ux) satisfy R_{-v}(x) = conj(R_v(x)).
+// Only rows v = 0 .. N/2 are gridded and transformed; the column pass reads
+// R_v(x) for v <= N/2 and uses conj(R_{N-v}(x)) for the others.  The count
+// sum stays exact: an entry in a row 0 < v < N/2 stands for itself and for its
+// conjugate partner in row N - v, so it counts twice.
 #include "blink_fft_reg.hpp"
 #include "blink_gather.hpp"
 #include "blink_kernels.hpp"
@@ -19,27 +27,39 @@ namespace blink {
 // of float2, so that walking down a column of the tile hits distinct banks.
 __host__ __device__ constexpr int tile_stride(int n) { return padded_len(n) + 1; }
 
-template <int N>
-struct BatchShape;
-// lanes per row (L), rows per row tile / columns per column block (RB)
-template <> struct BatchShape<128> { static constexpr int L = 8, RB = 16; };
-template <> struct BatchShape<256> { static constexpr int L = 16, RB = 16; };
-template <> struct BatchShape<512> { static constexpr int L = 32, RB = 16; };
-template <> struct BatchShape<1024> { static constexpr int L = 64, RB = 16; };
-template <> struct BatchShape<2048> { static constexpr int L = 64, RB = 8; };
+// Column stride of the row-pass buffer inter[img][x][v], v = 0 .. N/2:
+// N/2 + 16 elements keeps every column 128-byte aligned.
+__host__ __device__ constexpr int half_stride(int n) { return n / 2 + 16; }
 
+// Lanes per row: 16 values per lane up to N = 1024, 32 at N = 2048.
 template <int N>
+struct BatchLanes;
+template <> struct BatchLanes<128> { static constexpr int L = 8; };
+template <> struct BatchLanes<256> { static constexpr int L = 16; };
+template <> struct BatchLanes<512> { static constexpr int L = 32; };
+template <> struct BatchLanes<1024> { static constexpr int L = 64; };
+template <> struct BatchLanes<2048> { static constexpr int L = 64; };
+
+// (N, RB): lanes per row L and RB rows per row tile (or columns per column
+// block); one team of L lanes per row, RB * L threads per workgroup.
+template <int N, int RB_>
+struct BatchShape {
+    static constexpr int L = BatchLanes<N>::L;
+    static constexpr int RB = RB_;
+};
+
+template <int N, int RB>
 constexpr size_t batch_lds_bytes() {
-    return sizeof(float2) * (size_t)BatchShape<N>::RB * tile_stride(N);
+    return sizeof(float2) * (size_t)RB * tile_stride(N);
 }
 
-template <int N>
-__global__ void __launch_bounds__(BatchShape<N>::RB * BatchShape<N>::L)
+template <int N, int RB_>
+__global__ void __launch_bounds__(RB_ * BatchLanes<N>::L)
 k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
                 const uint32_t* __restrict__ entries, int64_t img0, int n_img,
                 float2* __restrict__ inter, uint32_t* __restrict__ isums,
                 const float2* __restrict__ tw) {
-    constexpr int L = BatchShape<N>::L, RB = BatchShape<N>::RB, TS = tile_stride(N);
+    constexpr int L = BatchLanes<N>::L, RB = RB_, TS = tile_stride(N);
     constexpr int NT = RB * L;
     using F = RegFft<N, L>;
     extern __shared__ float4 rows_reg_lds4[];
@@ -47,18 +67,21 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
     __shared__ int rs[RB + 1];
     __shared__ int red[NT / 64 + 1];
 
-    constexpr int n_rb = N / RB;
+    constexpr int H = N / 2;
+    constexpr int n_rb = (H + 1 + RB - 1) / RB;  // tiles covering rows 0 .. N/2
     const unsigned nwg = gridDim.x;
     unsigned bid = blockIdx.x;
     if ((nwg & 7u) == 0) bid = (bid & 7u) * (nwg >> 3) + (bid >> 3);  // XCD-contiguous images
     const int tile_id = (int)(bid % n_rb);
     const int li = (int)(bid / n_rb);
     const int64_t img = img0 + li;
-    const int ch = (int)(img % g.n_ch);
+    const int ch = (int)img % g.n_ch;
     const int r0 = tile_id * RB;
+    const int nr = min(RB, H + 1 - r0);  // rows of this tile that are <= N/2
     const int tid = threadIdx.x;
     const int* rsg = row_start + (int64_t)ch * (N + 1);
-    if (tid <= RB) rs[tid] = rsg[r0 + tid];
+    if (tid <= nr) rs[tid] = rsg[r0 + tid];
+    if (tid > nr && tid <= RB) rs[tid] = rsg[r0 + nr];
     __syncthreads();
     if (rs[RB] == rs[0]) return;  // uniform: nothing lands in this tile
 
@@ -75,7 +98,8 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
         float2* c = &tile[r * TS + pad_index(u_idx)];
         atomicAdd(&c->x, v.x);
         atomicAdd(&c->y, v.y);
-        n_ok++;
+        const int row = r0 + r;
+        n_ok += (row == 0 || row == H) ? 1 : 2;
     }
     const int tot = block_sum_int(n_ok, red);
     if (tid == 0 && tot) atomicAdd(&isums[img], (uint32_t)tot);
@@ -92,19 +116,20 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
     }
     __syncthreads();
     // transposed store: inter[li][c][r0 + rr], RB consecutive rows per column
-    float2* dst = inter + (int64_t)li * N * N + r0;
+    constexpr int HS = half_stride(N);
+    float2* dst = inter + (int64_t)li * N * HS + r0;
     for (int i = tid; i < RB * N; i += NT) {
         const int c = i / RB, rr = i % RB;
-        if (rs[rr + 1] != rs[rr]) dst[(int64_t)c * N + rr] = tile[rr * TS + pad_index(c)];
+        if (rs[rr + 1] != rs[rr]) dst[(int64_t)c * HS + rr] = tile[rr * TS + pad_index(c)];
     }
 }
 
-template <int N>
-__global__ void __launch_bounds__(BatchShape<N>::RB * BatchShape<N>::L)
+template <int N, int CB_>
+__global__ void __launch_bounds__(CB_ * BatchLanes<N>::L)
 k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_t img0,
            int n_ch, const int* __restrict__ row_start, const uint32_t* __restrict__ isums,
            const float2* __restrict__ tw) {
-    constexpr int L = BatchShape<N>::L, CB = BatchShape<N>::RB, TS = tile_stride(N);
+    constexpr int L = BatchLanes<N>::L, CB = CB_, TS = tile_stride(N);
     constexpr int NT = CB * L;
     using F = RegFft<N, L>;
     extern __shared__ float4 cols_reg_lds4[];
@@ -116,15 +141,26 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
     const int li = blockIdx.x / n_cb;
     const int64_t img = img0 + li;
     const int tid = threadIdx.x;
-    const int* rsg = row_start + (img % n_ch) * (int64_t)(N + 1);
+    const int* rsg = row_start + (int64_t)((int)img % n_ch) * (N + 1);
     for (int i = tid; i < N; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
     __syncthreads();
 
     const int team = tid / L, lane = tid % L;
     const int c0 = cb * CB;
-    const float2* src = inter + ((int64_t)li * N + c0 + team) * N;
+    constexpr int HS = half_stride(N);
+    const float2* src = inter + ((int64_t)li * N + c0 + team) * HS;
     float2 v[F::V];
-    F::load(v, lane, [&](int i) { return nonempty[i] ? src[i] : make_float2(0.f, 0.f); });
+    // Row i > N/2 is the conjugate of stored row N - i.  Rows without plan
+    // entries read element 0 (one cached line) and are cleared with a bit
+    // mask, so every load is unconditional and all are in flight together.
+    F::load(v, lane, [&](int i) {
+        const int m = nonempty[i] ? -1 : 0;
+        const int j = i <= N / 2 ? i : N - i;
+        const float2 x = src[j & m];
+        const float y = i <= N / 2 ? x.y : -x.y;
+        return make_float2(__int_as_float(__float_as_int(x.x) & m),
+                           __int_as_float(__float_as_int(y) & m));
+    });
     float2* row = tile + team * TS;
     F::template run<1>(v, row, lane, tw);
     const float s = (float)(1.0 / (double)isums[img]);
@@ -144,31 +180,39 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
     }
 }
 
-template <int N>
+template <int N, int RB>
 static hipError_t launch_rows_reg_t(const GatherArgs& g, const int* row_start,
                                     const uint32_t* entries, int64_t img0, int n_img, float* inter,
                                     uint32_t* isums, const float* tw, hipStream_t stream) {
-    constexpr int NT = BatchShape<N>::RB * BatchShape<N>::L;
-    constexpr size_t lds = batch_lds_bytes<N>();
+    constexpr int NT = RB * BatchLanes<N>::L;
+    constexpr size_t lds = batch_lds_bytes<N, RB>();
     if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
-    const dim3 grid((unsigned)((int64_t)(N / BatchShape<N>::RB) * n_img));
-    hipLaunchKernelGGL(k_grid_rows_reg<N>, grid, dim3(NT), lds, stream, g, row_start, entries,
-                       img0, n_img, (float2*)inter, isums, (const float2*)tw);
+    const dim3 grid((unsigned)((int64_t)((N / 2 + 1 + RB - 1) / RB) * n_img));
+    hipLaunchKernelGGL((k_grid_rows_reg<N, RB>), grid, dim3(NT), lds, stream, g, row_start,
+                       entries, img0, n_img, (float2*)inter, isums, (const float2*)tw);
     return hipGetLastError();
 }
 
-template <int N>
+template <int N, int CB>
 static hipError_t launch_cols_reg_t(const float* inter, float* images, int64_t img0, int n_img,
                                     int n_ch, const int* row_start, const uint32_t* isums,
                                     const float* tw, hipStream_t stream) {
-    constexpr int NT = BatchShape<N>::RB * BatchShape<N>::L;
-    constexpr size_t lds = batch_lds_bytes<N>();
+    constexpr int NT = CB * BatchLanes<N>::L;
+    constexpr size_t lds = batch_lds_bytes<N, CB>();
     if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
-    const dim3 grid((unsigned)((int64_t)(N / BatchShape<N>::RB) * n_img));
-    hipLaunchKernelGGL(k_cols_reg<N>, grid, dim3(NT), lds, stream, (const float2*)inter,
+    const dim3 grid((unsigned)((int64_t)(N / CB) * n_img));
+    hipLaunchKernelGGL((k_cols_reg<N, CB>), grid, dim3(NT), lds, stream, (const float2*)inter,
                        (float2*)images, img0, n_ch, row_start, isums, (const float2*)tw);
     return hipGetLastError();
 }
+
+// Tile variants (rows per row tile = columns per column block) per size.
+#define BLINK_BATCH_VARIANTS(X) \
+    X(128, 16) X(128, 32) \
+    X(256, 8) X(256, 16) X(256, 32) \
+    X(512, 8) X(512, 16) \
+    X(1024, 4) X(1024, 8) X(1024, 16) \
+    X(2048, 4) X(2048, 8)
 
 bool batch_reg_supported(int n) {
     switch (n) {
@@ -179,32 +223,37 @@ bool batch_reg_supported(int n) {
     }
 }
 
-hipError_t launch_grid_rows_reg(const GatherArgs& g, const int* row_start, const uint32_t* entries,
-                                int n, int64_t img0, int n_img, float* inter, uint32_t* isums,
-                                const float* tw, hipStream_t stream) {
-    if (n_img == 0) return hipSuccess;
-    switch (n) {
-        case 128: return launch_rows_reg_t<128>(g, row_start, entries, img0, n_img, inter, isums, tw, stream);
-        case 256: return launch_rows_reg_t<256>(g, row_start, entries, img0, n_img, inter, isums, tw, stream);
-        case 512: return launch_rows_reg_t<512>(g, row_start, entries, img0, n_img, inter, isums, tw, stream);
-        case 1024: return launch_rows_reg_t<1024>(g, row_start, entries, img0, n_img, inter, isums, tw, stream);
-        case 2048: return launch_rows_reg_t<2048>(g, row_start, entries, img0, n_img, inter, isums, tw, stream);
-        default: return hipErrorNotSupported;
-    }
+int batch_reg_variants(int n, int* tiles, int max_tiles) {
+    int k = 0;
+#define X(NN, T) \
+    if (n == NN && k < max_tiles) tiles[k++] = T;
+    BLINK_BATCH_VARIANTS(X)
+#undef X
+    return k;
 }
 
-hipError_t launch_cols_reg(const float* inter, float* images, int n, int64_t img0, int n_img,
-                           int n_ch, const int* row_start, const uint32_t* isums, const float* tw,
-                           hipStream_t stream) {
+hipError_t launch_grid_rows_reg(const GatherArgs& g, const int* row_start, const uint32_t* entries,
+                                int n, int tile, int64_t img0, int n_img, float* inter,
+                                uint32_t* isums, const float* tw, hipStream_t stream) {
     if (n_img == 0) return hipSuccess;
-    switch (n) {
-        case 128: return launch_cols_reg_t<128>(inter, images, img0, n_img, n_ch, row_start, isums, tw, stream);
-        case 256: return launch_cols_reg_t<256>(inter, images, img0, n_img, n_ch, row_start, isums, tw, stream);
-        case 512: return launch_cols_reg_t<512>(inter, images, img0, n_img, n_ch, row_start, isums, tw, stream);
-        case 1024: return launch_cols_reg_t<1024>(inter, images, img0, n_img, n_ch, row_start, isums, tw, stream);
-        case 2048: return launch_cols_reg_t<2048>(inter, images, img0, n_img, n_ch, row_start, isums, tw, stream);
-        default: return hipErrorNotSupported;
-    }
+#define X(NN, T) \
+    if (n == NN && tile == T) \
+        return launch_rows_reg_t<NN, T>(g, row_start, entries, img0, n_img, inter, isums, tw, stream);
+    BLINK_BATCH_VARIANTS(X)
+#undef X
+    return hipErrorNotSupported;
+}
+
+hipError_t launch_cols_reg(const float* inter, float* images, int n, int tile, int64_t img0,
+                           int n_img, int n_ch, const int* row_start, const uint32_t* isums,
+                           const float* tw, hipStream_t stream) {
+    if (n_img == 0) return hipSuccess;
+#define X(NN, T) \
+    if (n == NN && tile == T) \
+        return launch_cols_reg_t<NN, T>(inter, images, img0, n_img, n_ch, row_start, isums, tw, stream);
+    BLINK_BATCH_VARIANTS(X)
+#undef X
+    return hipErrorNotSupported;
 }
 
 hipError_t set_batch_kernel_attributes() {
@@ -214,16 +263,11 @@ hipError_t set_batch_kernel_attributes() {
         hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (r != hipSuccess && e == hipSuccess) e = r;
     };
-    set((const void*)k_grid_rows_reg<128>);
-    set((const void*)k_grid_rows_reg<256>);
-    set((const void*)k_grid_rows_reg<512>);
-    set((const void*)k_grid_rows_reg<1024>);
-    set((const void*)k_grid_rows_reg<2048>);
-    set((const void*)k_cols_reg<128>);
-    set((const void*)k_cols_reg<256>);
-    set((const void*)k_cols_reg<512>);
-    set((const void*)k_cols_reg<1024>);
-    set((const void*)k_cols_reg<2048>);
+#define X(NN, T)                                        \
+    set((const void*)k_grid_rows_reg<NN, T>);           \
+    set((const void*)k_cols_reg<NN, T>);
+    BLINK_BATCH_VARIANTS(X)
+#undef X
     return e;
 }
 

@@ -45,6 +45,9 @@ struct blink_ctx {
     uint64_t graph_clock = 0;
     hipStream_t capture_stream = nullptr;
     bool use_graphs = true;
+    bool autotune = true;
+    // tile choice of the batched kernels per grid side: {row tile, column block}
+    std::map<int, std::pair<int, int>> tiles;
 
     // measurement hooks: event pairs around every launch while enabled
     struct ProfRec {
@@ -280,6 +283,8 @@ BLINK_API int blink_create(int device, blink_ctx** out) {
     }
     const char* ng = getenv("BLINK_NO_GRAPH");
     ctx->use_graphs = !(ng && ng[0] == '1');
+    const char* at = getenv("BLINK_AUTOTUNE");
+    ctx->autotune = !(at && at[0] == '0');
     // Twiddle tables of the common grid sides, so that later calls do not
     // allocate.
     static const int common[] = {128, 180, 256, 512, 1024, 2048, 4096};
@@ -567,6 +572,87 @@ static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
     return BLINK_OK;
 }
 
+// Default tiles when the autotuner is off: 16 rows / columns up to N = 1024.
+static std::pair<int, int> default_tiles(int n) {
+    int t[8];
+    const int k = batch_reg_variants(n, t, 8);
+    int best = t[0];
+    for (int i = 0; i < k; i++)
+        if (t[i] <= (n >= 2048 ? 8 : 16)) best = std::max(best, t[i]);
+    return {best, best};
+}
+
+// Average GPU milliseconds of `reps` launches made by fn() on stream s.
+}  // extern "C"
+template <class F>
+static int time_launches(blink_ctx* ctx, hipStream_t s, int reps, F fn, float* ms) {
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    hipError_t e = fn();  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(a, s);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = fn();
+    if (e == hipSuccess) e = hipEventRecord(b, s);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (e != hipSuccess) return ctx->fail(BLINK_ERR_HIP, "autotune: %s", hipGetErrorString(e));
+    *ms = t / reps;
+    return BLINK_OK;
+}
+extern "C" {
+
+// Chooses the row-tile and column-block sizes of the batched kernels for one
+// set's grid side by timing every built variant on the set's first chunk.
+static int autotune_set(blink_ctx* ctx, const blink_image_set& st, const SetScratch& sc,
+                        hipStream_t s) {
+    const int n = st.n_pixels;
+    int variants[8];
+    const int nv = batch_reg_variants(n, variants, 8);
+    const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
+    const int n_img = st.n_ch * st.n_t;
+    const int cnt = std::min(sc.chunk, n_img);
+    const float* tw = ctx->twiddles[n];
+    HIP_TRY(launch_plan_rows(st.u_m, st.v_m, st.bl_flags, n_bl, st.freq_hz, st.n_ch, st.delta_u,
+                             st.delta_v, st.min_uv, n, sc.row_start, sc.entries, s));
+    GatherArgs g{};
+    g.vis = (const float4*)st.vis;
+    g.w_m = st.w_m;
+    g.cable_m = st.cable_m;
+    g.freq_hz = st.freq_hz;
+    g.corrections = st.corrections & 3u;
+    g.pol = st.pol;
+    g.n_bl = n_bl;
+    g.n_ch = st.n_ch;
+    int best_r = variants[0], best_c = variants[0];
+    float best = 1e30f;
+    for (int v = 0; v < nv; v++) {
+        float ms;
+        int rc = time_launches(ctx, s, 3, [&] {
+            return launch_grid_rows_reg(g, sc.row_start, sc.entries, n, variants[v], 0, cnt,
+                                        sc.inter, sc.isums, tw, s);
+        }, &ms);
+        if (rc) return rc;
+        if (ms < best) best = ms, best_r = variants[v];
+    }
+    HIP_TRY(launch_grid_rows_reg(g, sc.row_start, sc.entries, n, best_r, 0, cnt, sc.inter,
+                                 sc.isums, tw, s));
+    best = 1e30f;
+    for (int v = 0; v < nv; v++) {
+        float ms;
+        int rc = time_launches(ctx, s, 3, [&] {
+            return launch_cols_reg(sc.inter, st.images, n, variants[v], 0, cnt, st.n_ch,
+                                   sc.row_start, sc.isums, tw, s);
+        }, &ms);
+        if (rc) return rc;
+        if (ms < best) best = ms, best_c = variants[v];
+    }
+    ctx->tiles[n] = {best_r, best_c};
+    return BLINK_OK;
+}
+
 static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
                          const std::vector<SetScratch>& sc, hipStream_t s) {
     for (int i = 0; i < n_sets; i++) {
@@ -596,11 +682,13 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
         for (int img0 = 0; img0 < n_img; img0 += sc[i].chunk) {
             const int cnt = std::min(sc[i].chunk, n_img - img0);
             if (reg) {
+                auto it = ctx->tiles.find(n);
+                const std::pair<int, int> t = it != ctx->tiles.end() ? it->second : default_tiles(n);
                 LAUNCH("grid_rows_fft", s,
-                       launch_grid_rows_reg(g, sc[i].row_start, sc[i].entries, n, img0, cnt,
-                                            sc[i].inter, sc[i].isums, tw, s));
+                       launch_grid_rows_reg(g, sc[i].row_start, sc[i].entries, n, t.first, img0,
+                                            cnt, sc[i].inter, sc[i].isums, tw, s));
                 LAUNCH("fft_cols", s,
-                       launch_cols_reg(sc[i].inter, st.images, n, img0, cnt, st.n_ch,
+                       launch_cols_reg(sc[i].inter, st.images, n, t.second, img0, cnt, st.n_ch,
                                        sc[i].row_start, sc[i].isums, tw, s));
                 continue;
             }
@@ -648,6 +736,13 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
     std::vector<SetScratch> sc;
     if ((rc = reserve_batch(ctx, sets, n_sets, sc))) return rc;
     hipStream_t s = (hipStream_t)stream;
+    if (ctx->autotune) {
+        for (int i = 0; i < n_sets; i++) {
+            const int n = sets[i].n_pixels;
+            if (!batch_reg_supported(n) || ctx->tiles.count(n)) continue;
+            if ((rc = autotune_set(ctx, sets[i], sc[i], s))) return rc;
+        }
+    }
     if (!ctx->use_graphs || ctx->prof_on) return enqueue_batch(ctx, sets, n_sets, sc, s);
 
     std::string key((const char*)sets, sizeof(blink_image_set) * (size_t)n_sets);
@@ -726,6 +821,23 @@ BLINK_API int blink_profile_reset(blink_ctx* ctx) {
     if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
     prof_collect(ctx);
     ctx->prof_totals.clear();
+    return BLINK_OK;
+}
+
+BLINK_API int blink_batch_tiles(blink_ctx* ctx, int n_pixels, int* row_tile, int* col_tile) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(row_tile && col_tile, "null pointer argument");
+    auto it = ctx->tiles.find(n_pixels);
+    if (it != ctx->tiles.end()) {
+        *row_tile = it->second.first;
+        *col_tile = it->second.second;
+    } else if (batch_reg_supported(n_pixels)) {
+        const std::pair<int, int> t = default_tiles(n_pixels);
+        *row_tile = -t.first;
+        *col_tile = -t.second;
+    } else {
+        *row_tile = *col_tile = 0;
+    }
     return BLINK_OK;
 }
 

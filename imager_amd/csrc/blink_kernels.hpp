@@ -90,12 +90,14 @@ hipError_t launch_isums_to_double(const uint32_t* isums, double* sums, int n, hi
 // Register-FFT batched kernels (power-of-two sides 128..2048); the row-pass
 // buffer is transposed: inter[img][u][v].
 bool batch_reg_supported(int n);
+// Tile sizes (rows per row tile / columns per column block) built for side n.
+int batch_reg_variants(int n, int* tiles, int max_tiles);
 hipError_t set_batch_kernel_attributes();
 hipError_t launch_grid_rows_reg(const GatherArgs& g, const int* row_start, const uint32_t* entries,
-                                int n, int64_t img0, int n_img, float* inter, uint32_t* isums,
-                                const float* tw, hipStream_t stream);
-hipError_t launch_cols_reg(const float* inter, float* images, int n, int64_t img0, int n_img,
-                           int n_ch, const int* row_start, const uint32_t* isums, const float* tw,
-                           hipStream_t stream);
+                                int n, int tile, int64_t img0, int n_img, float* inter,
+                                uint32_t* isums, const float* tw, hipStream_t stream);
+hipError_t launch_cols_reg(const float* inter, float* images, int n, int tile, int64_t img0,
+                           int n_img, int n_ch, const int* row_start, const uint32_t* isums,
+                           const float* tw, hipStream_t stream);
 
 }  // namespace blink

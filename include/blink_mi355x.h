@@ -218,6 +218,14 @@ BLINK_API int blink_profile_get(blink_ctx* ctx, const char* kernel, int64_t* lau
 BLINK_API int blink_profile_kernels(blink_ctx* ctx, char* buf, int buf_len);
 BLINK_API int blink_profile_reset(blink_ctx* ctx);
 
+/* Tile sizes of the batched kernels for grid side n_pixels: rows per row
+ * tile and columns per column block.  Positive when chosen by the built-in
+ * autotuner (the first blink_image_batch call on a new side times every
+ * built variant unless BLINK_AUTOTUNE=0 is set in the environment),
+ * negative for the defaults not yet tuned, 0 when the side uses the generic
+ * kernels. */
+BLINK_API int blink_batch_tiles(blink_ctx* ctx, int n_pixels, int* row_tile, int* col_tile);
+
 #ifdef __cplusplus
 }
 #endif
