@@ -122,6 +122,31 @@ def fft_mkl_rate(n_pix: int, seconds: float = 3.0):
     return n / (time.perf_counter() - t0), torch.get_num_threads()
 
 
+def memory_reference(mib: int = 2048):
+    """Write-only (fill) and copy bandwidth of this GPU with torch kernels, as a
+    practical reference next to the 8 TB/s peak used for the roofline."""
+    import torch
+    n = (mib << 20) // 4
+    x = torch.empty(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    res = {}
+    for name, fn, nbytes in (("fill_gbs", lambda: x.fill_(2.0), 4 * n),
+                             ("copy_gbs", lambda: y.copy_(x), 8 * n)):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(5):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        res[name] = 5 * nbytes / (a.elapsed_time(b) * 1e-3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +340,7 @@ def main():
         "kernels": kernels,
         "eager_ms_per_step": eager_ms,
         "batch_tiles": ctx.batch_tiles(N),
+        "hbm_reference": memory_reference(),
         "parity": parity,
     }
     ctx.close()

@@ -48,9 +48,10 @@ struct BatchShape {
     static constexpr int RB = RB_;
 };
 
+// LDS of one workgroup: the tile and a copy of the twiddle table.
 template <int N, int RB>
 constexpr size_t batch_lds_bytes() {
-    return sizeof(float2) * (size_t)RB * tile_stride(N);
+    return sizeof(float2) * ((size_t)RB * tile_stride(N) + N);
 }
 
 template <int N, int RB_>
@@ -85,6 +86,8 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
     __syncthreads();
     if (rs[RB] == rs[0]) return;  // uniform: nothing lands in this tile
 
+    float2* twl = tile + RB * TS;  // twiddle table copy [N]
+    for (int i = tid; i < N; i += NT) twl[i] = tw[i];
     for (int i = tid; i < RB * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
     const uint32_t* ent = entries + (int64_t)ch * 2 * g.n_bl;
@@ -111,7 +114,7 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
         float2 v[F::V];
         F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
         team_sync<false>();
-        F::template run<1>(v, row, lane, tw);
+        F::template run<1>(v, row, lane, twl);
         F::store(v, lane, [&](int i, float2 x) { row[pad_index(i)] = x; });
     }
     __syncthreads();
@@ -141,8 +144,13 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
     const int li = blockIdx.x / n_cb;
     const int64_t img = img0 + li;
     const int tid = threadIdx.x;
+    const uint32_t count = isums[img];
     const int* rsg = row_start + (int64_t)((int)img % n_ch) * (N + 1);
-    for (int i = tid; i < N; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
+    float2* twl = tile + CB * TS;  // twiddle table copy [N]
+    for (int i = tid; i < N; i += NT) {
+        nonempty[i] = rsg[i + 1] != rsg[i];
+        twl[i] = tw[i];
+    }
     __syncthreads();
 
     const int team = tid / L, lane = tid % L;
@@ -162,8 +170,8 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
                            __int_as_float(__float_as_int(y) & m));
     });
     float2* row = tile + team * TS;
-    F::template run<1>(v, row, lane, tw);
-    const float s = (float)(1.0 / (double)isums[img]);
+    F::template run<1>(v, row, lane, twl);
+    const float s = (float)(1.0 / (double)count);
 #pragma unroll
     for (int k = 0; k < F::V; k++) v[k] = cscale(v[k], s);
     team_sync<false>();
