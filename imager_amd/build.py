@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 
-KERNEL_SOURCES = ["blink_kernels.hip", "blink_batch.hip", "blink_capi.hip"]
+KERNEL_SOURCES = ["blink_kernels.hip", "blink_batch.hip", "blink_dft_mfma.hip", "blink_capi.hip"]
 
 
 def _run(cmd):

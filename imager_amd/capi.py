@@ -31,6 +31,7 @@ WEIGHTING_NATURAL, WEIGHTING_UNIFORM = 0, 1
 CORR_GEOMETRIC, CORR_CABLE = 1, 2
 FFT_FORWARD, FFT_BACKWARD = -1, 1
 GRID_TILED, GRID_ATOMIC = 0, 1
+TRANSFORM_FFT, TRANSFORM_DFT_BF16X3 = 0, 1
 MAX_PIXELS = 4096
 
 # Every symbol include/blink_mi355x.h declares.
@@ -71,6 +72,7 @@ class ImageSet(ctypes.Structure):
         ("delta_u", ctypes.c_double), ("delta_v", ctypes.c_double), ("min_uv", ctypes.c_double),
         ("n_pixels", ctypes.c_int), ("pol", ctypes.c_int),
         ("images", ctypes.c_void_p), ("count_sums", ctypes.c_void_p),
+        ("transform", ctypes.c_int),
     ]
 
 
@@ -278,9 +280,10 @@ class Context:
 
 
 def make_set(vis, n_ant, n_ch, n_t, u_m, v_m, w_m, bl_flags, freq_hz, cable_m, corrections,
-             delta_u, delta_v, min_uv, n_pixels, pol, images, count_sums=None) -> ImageSet:
+             delta_u, delta_v, min_uv, n_pixels, pol, images, count_sums=None,
+             transform=TRANSFORM_FFT) -> ImageSet:
     return ImageSet(_ptr(vis), n_ant, n_ch, n_t, _ptr(u_m), _ptr(v_m), _ptr(w_m), _ptr(bl_flags),
                     _ptr(freq_hz), _ptr(cable_m), int(corrections), float(delta_u),
                     float(delta_v), float(min_uv), int(n_pixels),
                     POL[pol] if isinstance(pol, str) else int(pol), _ptr(images),
-                    _ptr(count_sums))
+                    _ptr(count_sums), int(transform))

@@ -40,6 +40,7 @@ struct blink_ctx {
     hipDeviceProp_t props{};
     std::string error;
     std::map<int, float*> twiddles;               // N -> device table of N float2
+    std::map<int, uint16_t*> dft_planes;          // N -> split DFT matrix planes
     std::map<std::string, DeviceBuffer> scratch;  // named grow-only buffers
     std::map<std::string, GraphEntry> graphs;     // descriptor bytes -> graph
     uint64_t graph_clock = 0;
@@ -197,6 +198,23 @@ int twiddles(blink_ctx* ctx, int n, const float** out) {
     return BLINK_OK;
 }
 
+// Split DFT matrix of side n (blink_dft_mfma.hip), built once per context.
+int dft_planes(blink_ctx* ctx, int n, const uint16_t** out) {
+    auto it = ctx->dft_planes.find(n);
+    if (it != ctx->dft_planes.end()) {
+        *out = it->second;
+        return BLINK_OK;
+    }
+    std::vector<uint16_t> h;
+    make_dft_planes(n, h);
+    uint16_t* d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(uint16_t) * h.size()));
+    HIP_TRY(hipMemcpy(d, h.data(), sizeof(uint16_t) * h.size(), hipMemcpyHostToDevice));
+    ctx->dft_planes[n] = d;
+    *out = d;
+    return BLINK_OK;
+}
+
 int check_fft_size(blink_ctx* ctx, int n, FftPlan& plan) {
     if (!make_fft_plan(n, plan))
         return ctx->fail(BLINK_ERR_UNSUPPORTED,
@@ -305,6 +323,7 @@ BLINK_API void blink_destroy(blink_ctx* ctx) {
     (void)hipDeviceSynchronize();
     blink_release_cache(ctx);
     for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+    for (auto& kv : ctx->dft_planes) (void)hipFree(kv.second);
     prof_collect(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
@@ -543,7 +562,8 @@ static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
         const int64_t n = st.n_pixels;
         const int n_img = st.n_ch * st.n_t;
         const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
-        const int64_t per = 8 * n * n;
+        // the DFT path keeps the grid and the first product (two buffers)
+        const int64_t per = (st.transform == BLINK_TRANSFORM_DFT_BF16X3 ? 16 : 8) * n * n;
         int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
         out[i].chunk = chunk;
         inter_bytes = std::max(inter_bytes, (size_t)(per * chunk));
@@ -568,6 +588,10 @@ static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
     for (int i = 0; i < n_sets; i++) {
         const float* tw;
         if ((rc = twiddles(ctx, sets[i].n_pixels, &tw))) return rc;
+        if (sets[i].transform == BLINK_TRANSFORM_DFT_BF16X3) {
+            const uint16_t* pl;
+            if ((rc = dft_planes(ctx, sets[i].n_pixels, &pl))) return rc;
+        }
     }
     return BLINK_OK;
 }
@@ -678,6 +702,27 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
         g.n_bl = n_bl;
         g.n_ch = st.n_ch;
         const int64_t gs = (int64_t)n * n;
+        if (st.transform == BLINK_TRANSFORM_DFT_BF16X3) {
+            // grid (overwrite) -> G D'^T -> (.) D'^T with scale, per chunk
+            const uint16_t* pl = ctx->dft_planes[n];
+            float* gbuf = sc[i].inter;
+            float* tbuf = sc[i].inter + 2 * gs * sc[i].chunk;
+            for (int img0 = 0; img0 < n_img; img0 += sc[i].chunk) {
+                const int cnt = std::min(sc[i].chunk, n_img - img0);
+                LAUNCH("grid_tiles", s,
+                       launch_grid_tiles(g, sc[i].row_start, sc[i].entries, cnt, n, gbuf, nullptr,
+                                         s, 1, sc[i].isums, img0));
+                LAUNCH("dft_gemm", s,
+                       launch_dft_gemm(gbuf, pl, tbuf, n, cnt, img0, nullptr, nullptr, s));
+                LAUNCH("dft_gemm", s,
+                       launch_dft_gemm(tbuf, pl, st.images + 2 * gs * img0, n, cnt, img0,
+                                       sc[i].isums, nullptr, s));
+            }
+            if (st.count_sums)
+                LAUNCH("isums_to_double", s,
+                       launch_isums_to_double(sc[i].isums, st.count_sums, n_img, s));
+            continue;
+        }
         const bool reg = batch_reg_supported(n);
         for (int img0 = 0; img0 < n_img; img0 += sc[i].chunk) {
             const int cnt = std::min(sc[i].chunk, n_img - img0);
@@ -727,6 +772,15 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
         CHECK_ARG(!(st.corrections & BLINK_CORR_GEOMETRIC) || st.w_m, "set %d: w_m is null", i);
         CHECK_ARG(!(st.corrections & BLINK_CORR_CABLE) || st.cable_m, "set %d: cable_m is null", i);
         CHECK_ARG(st.pol >= 0 && st.pol <= 3, "set %d: pol must be 0..3", i);
+        CHECK_ARG(st.transform == BLINK_TRANSFORM_FFT || st.transform == BLINK_TRANSFORM_DFT_BF16X3,
+                  "set %d: unknown transform %d", i, st.transform);
+        if (st.transform == BLINK_TRANSFORM_DFT_BF16X3) {
+            if (st.n_pixels < 2 || st.n_pixels > BLINK_MAX_DFT_PIXELS)
+                return ctx->fail(BLINK_ERR_UNSUPPORTED,
+                                 "set %d: the DFT transform needs 2 <= N <= %d, got %d", i,
+                                 BLINK_MAX_DFT_PIXELS, st.n_pixels);
+            continue;
+        }
         CHECK_ARG(st.n_pixels % 2 == 0, "set %d: the batched path needs an even grid side", i);
         FftPlan plan;
         if ((rc = check_fft_size(ctx, st.n_pixels, plan))) return rc;
@@ -739,7 +793,9 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
     if (ctx->autotune) {
         for (int i = 0; i < n_sets; i++) {
             const int n = sets[i].n_pixels;
-            if (!batch_reg_supported(n) || ctx->tiles.count(n)) continue;
+            if (sets[i].transform != BLINK_TRANSFORM_FFT || !batch_reg_supported(n) ||
+                ctx->tiles.count(n))
+                continue;
             if ((rc = autotune_set(ctx, sets[i], sc[i], s))) return rc;
         }
     }

@@ -272,14 +272,17 @@ __device__ __forceinline__ int row_of_entry(const int* rs, int nr, int e) {
 // the flush needs no atomics).
 __global__ void __launch_bounds__(kTileThreads)
 k_grid_tiles(GatherArgs g, const int* __restrict__ row_start, const uint32_t* __restrict__ entries,
-             int n_pixels, int rb, float2* __restrict__ grids, float* __restrict__ counts) {
+             int n_pixels, int rb, float2* __restrict__ grids, float* __restrict__ counts,
+             int overwrite, uint32_t* __restrict__ isums, int64_t img0) {
     extern __shared__ float4 tiles_lds4[];
     float2* tile = (float2*)tiles_lds4;                       // [rb][N]
     float* cnt = (float*)(tile + (size_t)rb * n_pixels);      // [rb][N]
     __shared__ int rs[kMaxTileRows + 1];
+    __shared__ int red[kTileThreads / 64];
     const int n_rb = (n_pixels + rb - 1) / rb;
     const int tile_id = blockIdx.x % n_rb;
-    const int64_t img = blockIdx.x / n_rb;
+    const int64_t li = blockIdx.x / n_rb;       // image within the launch
+    const int64_t img = img0 + li;              // image within the set
     const int ch = (int)(img % g.n_ch);
     const int r0 = tile_id * rb;
     const int nr = min(rb, n_pixels - r0);
@@ -287,13 +290,19 @@ k_grid_tiles(GatherArgs g, const int* __restrict__ row_start, const uint32_t* __
     const uint32_t* ent = entries + (int64_t)ch * 2 * g.n_bl;
     if (threadIdx.x <= nr) rs[threadIdx.x] = rsg[r0 + threadIdx.x];
     __syncthreads();
-    if (rs[nr] == rs[0]) return;  // nothing lands in this tile
+    const int64_t base = li * (int64_t)n_pixels * n_pixels + (int64_t)r0 * n_pixels;
     const int cells = nr * n_pixels;
+    if (rs[nr] == rs[0]) {  // nothing lands in this tile
+        if (overwrite)
+            for (int i = threadIdx.x; i < cells; i += blockDim.x) grids[base + i] = make_float2(0.f, 0.f);
+        return;
+    }
     for (int i = threadIdx.x; i < cells; i += blockDim.x) {
         tile[i] = make_float2(0.f, 0.f);
         cnt[i] = 0.f;
     }
     __syncthreads();
+    int n_ok = 0;
     for (int e = rs[0] + threadIdx.x; e < rs[nr]; e += blockDim.x) {
         int u_idx;
         float2 v;
@@ -303,11 +312,20 @@ k_grid_tiles(GatherArgs g, const int* __restrict__ row_start, const uint32_t* __
         atomicAdd(&tile[c].x, v.x);
         atomicAdd(&tile[c].y, v.y);
         atomicAdd(&cnt[c], 1.0f);
+        n_ok++;
+    }
+    if (isums) {
+        const int tot = block_sum_int(n_ok, red);
+        if (threadIdx.x == 0 && tot) atomicAdd(&isums[img], (uint32_t)tot);
     }
     __syncthreads();
-    const int64_t base = img * (int64_t)n_pixels * n_pixels + (int64_t)r0 * n_pixels;
     for (int i = threadIdx.x; i < cells; i += blockDim.x) {
         const int r = i / n_pixels;
+        if (overwrite) {
+            grids[base + i] = tile[i];
+            if (counts) counts[base + i] = cnt[i];
+            continue;
+        }
         if (rs[r + 1] == rs[r]) continue;  // empty row: nothing to add
         const float2 t = tile[i];
         float2 o = grids[base + i];
@@ -320,7 +338,7 @@ k_grid_tiles(GatherArgs g, const int* __restrict__ row_start, const uint32_t* __
 
 hipError_t launch_grid_tiles(const GatherArgs& g, const int* row_start, const uint32_t* entries,
                              int n_images, int n_pixels, float* grids, float* counts,
-                             hipStream_t stream) {
+                             hipStream_t stream, int overwrite, uint32_t* isums, int64_t img0) {
     if (n_images == 0) return hipSuccess;
     // 12 bytes per cell (complex + count) within the LDS budget.
     int rb = (int)(g_lds_budget / (12 * (int64_t)n_pixels));
@@ -329,7 +347,8 @@ hipError_t launch_grid_tiles(const GatherArgs& g, const int* row_start, const ui
     const size_t lds = (size_t)rb * n_pixels * 12;
     if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_grid_tiles, dim3((unsigned)((int64_t)n_rb * n_images)), dim3(kTileThreads),
-                       lds, stream, g, row_start, entries, n_pixels, rb, (float2*)grids, counts);
+                       lds, stream, g, row_start, entries, n_pixels, rb, (float2*)grids, counts,
+                       overwrite, isums, img0);
     return hipGetLastError();
 }
 

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "blink_fft.hpp"
 
 namespace blink {
@@ -67,9 +69,13 @@ hipError_t launch_grid_atomic(const float* vis, int n_ant, int n_ch, int n_t, co
 hipError_t launch_plan_rows(const float* u_m, const float* v_m, const int* flags, int n_bl,
                             const double* freq_hz, int n_ch, double du, double dv, double min_uv,
                             int n_pixels, int* row_start, uint32_t* entries, hipStream_t stream);
+// Row-tile gridding.  Accumulates into grids/counts, or with overwrite != 0
+// writes every cell of the n_images grids (counts may then be null); valid
+// visibilities are added to isums[img0 + image] when isums is given.
 hipError_t launch_grid_tiles(const GatherArgs& g, const int* row_start, const uint32_t* entries,
                              int n_images, int n_pixels, float* grids, float* counts,
-                             hipStream_t stream);
+                             hipStream_t stream, int overwrite = 0, uint32_t* isums = nullptr,
+                             int64_t img0 = 0);
 hipError_t launch_uniform_weight(float* grids, const float* counts, int64_t n_cells,
                                  hipStream_t stream);
 hipError_t launch_fft_rows(const float* in, float* out, const FftPlan& plan, int64_t n_rows,
@@ -86,6 +92,14 @@ hipError_t launch_grid_rows_fft(const GatherArgs& g, const int* row_start, const
                                 const FftPlan& plan, int64_t img0, int n_img, float* inter,
                                 uint32_t* isums, const float* tw, hipStream_t stream);
 hipError_t launch_isums_to_double(const uint32_t* isums, double* sums, int n, hipStream_t stream);
+
+// Split-precision DFT-as-GEMM (blink_dft_mfma.hip): out = (D' applied to the
+// rows of a), stored transposed per image, scaled by 1/sum when sums are given.
+hipError_t launch_dft_gemm(const float* a, const uint16_t* planes, float* out, int n, int64_t n_img,
+                           int64_t img0, const uint32_t* isums, const double* sums,
+                           hipStream_t stream);
+void make_dft_planes(int n, std::vector<uint16_t>& planes);
+size_t dft_planes_bytes(int n);
 
 // Register-FFT batched kernels (power-of-two sides 128..2048); the row-pass
 // buffer is transposed: inter[img][u][v].

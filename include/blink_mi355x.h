@@ -61,6 +61,12 @@ enum blink_corrections { BLINK_CORR_GEOMETRIC = 1u, BLINK_CORR_CABLE = 2u };
  * BACKWARD, src/gpu/pacer_imager_hip.cpp:95), unnormalised. */
 enum blink_fft_direction { BLINK_FFT_FORWARD = -1, BLINK_FFT_BACKWARD = 1 };
 
+/* 2-D transform of blink_image_batch.  FFT: in-house FP32 FFT.  DFT_BF16X3:
+ * image = D G D^T as two dense GEMMs on MFMA with every FP32 operand split
+ * into bf16 hi + lo and FP32 accumulation (N <= 512). */
+enum blink_transform { BLINK_TRANSFORM_FFT = 0, BLINK_TRANSFORM_DFT_BF16X3 = 1 };
+#define BLINK_MAX_DFT_PIXELS 512
+
 /* Gridding algorithm for blink_gridding. */
 enum blink_grid_algo {
     BLINK_GRID_TILED = 0,  /* per-channel row plan + LDS-privatised row tiles (default) */
@@ -185,6 +191,7 @@ typedef struct blink_image_set {
     int pol;                 /* blink_pol */
     float* images;           /* [n_t][n_ch][N][N] complex64 output */
     double* count_sums;      /* optional [n_t*n_ch] output: sum of counts per image */
+    int transform;           /* blink_transform: how the 2-D transform is computed */
 } blink_image_set;
 
 /* Visibilities -> normalised, shifted dirty images for every set, with the

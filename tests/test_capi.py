@@ -67,9 +67,9 @@ def test_image_set_struct_matches_header():
 #include <stddef.h>
 #include "blink_mi355x.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu\n", sizeof(blink_image_set), offsetof(blink_image_set, corrections),
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(blink_image_set), offsetof(blink_image_set, corrections),
          offsetof(blink_image_set, delta_u), offsetof(blink_image_set, images),
-         offsetof(blink_image_set, count_sums));
+         offsetof(blink_image_set, count_sums), offsetof(blink_image_set, transform));
   return 0;
 }
 '''
@@ -82,4 +82,4 @@ int main(void) {
                                         check=True).stdout.split()))
     S = capi.ImageSet
     assert vals == [ctypes.sizeof(S), S.corrections.offset, S.delta_u.offset, S.images.offset,
-                    S.count_sums.offset]
+                    S.count_sums.offset, S.transform.offset]

@@ -391,3 +391,59 @@ def test_error_paths(ctx):
     _, _, _, st = _batch(ctx, o2)
     with pytest.raises(capi.BlinkInvalidArgument):
         ctx.image_batch([st])      # odd grid side in the batched path
+
+
+# ------------------------------------------ split-precision DFT on MFMA
+
+@pytest.mark.parametrize("n,n_ant", [(64, 32), (180, 128), (256, 128), (512, 64), (45, 24)])
+def test_image_batch_dft_bf16x3(ctx, n, n_ant):
+    """image = D G D^T on MFMA with bf16 hi+lo operands: within 1e-5
+    (peak-normalised) of the FP32 oracle; its error to the FP64 oracle is
+    printed next to the FP32 FFT path's."""
+    import torch
+    from imager_amd import capi
+    o = synth.small(n_ant=n_ant, n_ch=3, n_t=2, n_pixels=n, seed=900 + n, height_sigma_m=1.0)
+    _say("dft", n, n_ant)
+    ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, n)
+    ref64 = oracle.run_f64(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                           o.delta_u, o.delta_v, o.min_uv, n)
+    d = _obs_dev(o)
+    imgs = torch.empty((o.n_t, o.n_ch, n, n), dtype=torch.complex64, device="cuda")
+    sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
+    st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                       d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, n, "XX", imgs,
+                       sums, transform=capi.TRANSFORM_DFT_BF16X3)
+    ctx.image_batch([st])
+    got = _host(imgs)
+    e32 = oracle.peak_normalised_error(got, ref)
+    e64 = oracle.peak_normalised_error(got, ref64)
+    line = f"  DFT bf16x3 N={n}: vs FP32 oracle {e32[0]:.2e}, vs FP64 max {e64[0]:.2e} rms {e64[1]:.2e}"
+    if n % 2 == 0:
+        imgs2 = torch.empty_like(imgs)
+        st2 = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                            d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, n, "XX",
+                            imgs2)
+        ctx.image_batch([st2])
+        f64 = oracle.peak_normalised_error(_host(imgs2), ref64)
+        line += f"; FP32 FFT vs FP64 max {f64[0]:.2e} rms {f64[1]:.2e}"
+    _say(line)
+    assert e32[0] < IMAGE_TOL
+    assert e64[0] < IMAGE_TOL
+    _, c_ref, _ = oracle.gridding(oracle.cable_corrections(
+        oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq), o.n_ant, o.cable, o.freq),
+        o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v, o.min_uv, n, "XX")
+    assert np.array_equal(_host(sums), c_ref.reshape(o.n_images, -1).sum(1).astype(np.float64))
+
+
+def test_image_batch_dft_rejects_large_grids(ctx):
+    import torch
+    from imager_amd import capi
+    o = synth.small(n_ant=8, n_ch=1, n_t=1, n_pixels=1024)
+    d = _obs_dev(o)
+    imgs = torch.empty((1, 1, 1024, 1024), dtype=torch.complex64, device="cuda")
+    st = capi.make_set(d["vis"], 8, 1, 1, d["u"], d["v"], d["w"], d["flags"], d["freq"],
+                       d["cable"], 0, o.delta_u, o.delta_v, o.min_uv, 1024, "XX", imgs,
+                       transform=capi.TRANSFORM_DFT_BF16X3)
+    with pytest.raises(capi.BlinkUnsupported):
+        ctx.image_batch([st])
