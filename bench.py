@@ -51,6 +51,10 @@ def make_workload(name: str):
                                freq=synth.mwa_fine_channels(768), height_sigma_m=0.5)
     if name == "eda2_1ch_180":
         return synth.eda2_station(n_ch=1, n_t=1, n_pixels=180, seed=1001)
+    if name in ("eda2_768ch_180", "eda2_768ch_256"):
+        n = int(name.rsplit("_", 1)[1])
+        return synth.eda2_station(n_ch=768, n_t=1, n_pixels=n, seed=1004,
+                                  freq=synth.mwa_fine_channels(768))
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -62,11 +66,23 @@ def algorithmic_bytes(kernel: str, n_bl: int, n_pix: int, n_img: int) -> float |
     column kernel does the column FFTs with the fused shift+norm store."""
     g = 16.0 * n_bl + 12.0 * n_pix * n_pix
     f = 16.0 * n_pix * n_pix
-    if kernel == "grid_rows_fft":
+    if kernel in ("grid_rows_fft", "grid_tiles"):
         return g * n_img
     if kernel == "fft_cols":
         return f * n_img
     return None
+
+
+def dft_flops(n_pix: int, n_img: int):
+    """(useful, executed) flops of ONE dft_gemm launch (one of the two complex
+    N x N x N products per image): useful 8 N^3 per image; executed = 12 MFMA
+    32x32x16 per wave and k-step of 16 over the padded tile grid."""
+    n_pad = -(-n_pix // 64) * 64
+    k_pad = -(-n_pix // 16) * 16
+    m_pad = -(-(n_img * n_pix) // 64) * 64
+    useful = 8.0 * n_pix ** 3 * n_img
+    executed = 12 * 2.0 * 32 * 32 * 16 * (n_pad // 32) * (m_pad // 32) * (k_pad // 16)
+    return useful, executed
 
 
 def read_traffic(kernel: str, workload: str):
@@ -153,6 +169,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mwa128_64ch10t_1024")
+    ap.add_argument("--transform", choices=["fft", "dft"], default="fft",
+                    help="2-D transform of the batched path: in-house FP32 FFT, or the "
+                         "split-precision (bf16 hi+lo) DFT-as-GEMM on MFMA (N <= 512)")
     ap.add_argument("--plain-images", type=int, default=64,
                     help="images run through the plain one-image-at-a-time GPU path")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -199,9 +218,10 @@ def main():
     sums = torch.empty(obs.n_images, dtype=torch.float64, device="cuda")
     ctx = capi.Context(0)
     corr = capi.CORR_GEOMETRIC | capi.CORR_CABLE
+    transform = capi.TRANSFORM_DFT_BF16X3 if args.transform == "dft" else capi.TRANSFORM_FFT
     st = capi.make_set(d["vis"], obs.n_ant, obs.n_ch, obs.n_t, d["u"], d["v"], d["w"],
                        d["flags"], d["freq"], d["cable"], corr, obs.delta_u, obs.delta_v,
-                       obs.min_uv, N, "XX", images, sums)
+                       obs.min_uv, N, "XX", images, sums, transform=transform)
     stream = torch.cuda.current_stream()
 
     for _ in range(args.warmup):
@@ -234,7 +254,8 @@ def main():
     prof = ctx.profile()
     kernels = {}
     imgs_per_launch = {}
-    n_launch_rows = prof.get("grid_rows_fft", (0, 0))[0] // args.steps
+    n_launch_rows = max(prof.get("grid_rows_fft", (0, 0))[0],
+                        prof.get("grid_tiles", (0, 0))[0]) // args.steps
     chunk = -(-obs.n_images // max(n_launch_rows, 1))
     for name, (n, ms) in prof.items():
         kernels[name] = {"launches_per_step": n / args.steps, "avg_ms": ms / max(n, 1),
@@ -242,10 +263,21 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     log("kernels: " + ", ".join(f"{k} {v['ms_per_step']:.3f} ms/step ({v['avg_ms']*1e3:.1f} us x "
                                f"{v['launches_per_step']:.0f})" for k, v in kernels.items()))
-    imgs_per_launch = chunk if dom in ("grid_rows_fft", "fft_cols") else obs.n_images
+    imgs_per_launch = chunk if dom in ("grid_rows_fft", "fft_cols", "grid_tiles", "dft_gemm") \
+        else obs.n_images
     alg = algorithmic_bytes(dom, obs.n_bl, N, imgs_per_launch)
     roof = None
-    if alg is not None:
+    if dom == "dft_gemm":
+        useful, executed = dft_flops(N, imgs_per_launch)
+        avg_s = kernels[dom]["avg_ms"] * 1e-3
+        ach = executed / avg_s / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s",
+                "frac": ach / 2500.0, "traffic": None, "kernel": dom,
+                "images_per_launch": imgs_per_launch, "executed_flops_per_launch": executed,
+                "useful_flops_per_launch": useful,
+                "useful_tflops": useful / avg_s / 1e12,
+                "note": "bf16 dense MFMA peak; executed = 3 split terms x 4 real products"}
+    elif alg is not None:
         avg_s = kernels[dom]["avg_ms"] * 1e-3
         ach = alg / avg_s / 1e9
         traffic = read_traffic(dom, args.workload)
@@ -327,12 +359,14 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "images/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.transform == "fft" else "bf16 hi+lo, f32 accumulate",
         "data": "synthetic (seeded point sources + noise)",
         "config": {"workload": args.workload, "n_ant": obs.n_ant, "n_baselines": obs.n_bl,
                    "n_ch": obs.n_ch, "n_t": obs.n_t, "n_pixels": N,
                    "images_per_step": obs.n_images, "pol": "XX",
-                   "corrections": "geometric+cable", "path": "blink_image_batch (hipGraph)"},
+                   "corrections": "geometric+cable", "path": "blink_image_batch (hipGraph)",
+                   "transform": "fft fp32" if args.transform == "fft" else "dft bf16 hi+lo on MFMA"},
         "roofline": roof,
         "pipeline_roofline": pipeline,
         "cpu_baseline": cpu,
