@@ -23,9 +23,15 @@
 
 namespace blink {
 
-// Row stride of the LDS tiles: the padded row plus one element, an odd number
-// of float2, so that walking down a column of the tile hits distinct banks.
-__host__ __device__ constexpr int tile_stride(int n) { return padded_len(n) + 1; }
+// Row stride (float2) of an LDS tile of `rows` rows: at least the padded row,
+// and congruent to 32 / rows modulo 32, so that a transposed walk (rows
+// fastest, then columns) puts the 32 lanes of a half-wave on 32 distinct
+// 8-byte bank pairs.
+__host__ __device__ constexpr int tile_stride(int n, int rows) {
+    const int want = 32 / rows;
+    const int p = padded_len(n);
+    return p + ((want - p % 32) + 32) % 32;
+}
 
 // Column stride of the row-pass buffer inter[img][x][v], v = 0 .. N/2:
 // N/2 + 16 elements keeps every column 128-byte aligned.
@@ -51,7 +57,7 @@ struct BatchShape {
 // LDS of one workgroup: the tile and a copy of the twiddle table.
 template <int N, int RB>
 constexpr size_t batch_lds_bytes() {
-    return sizeof(float2) * ((size_t)RB * tile_stride(N) + N);
+    return sizeof(float2) * ((size_t)RB * tile_stride(N, RB) + N);
 }
 
 template <int N, int RB_>
@@ -60,7 +66,7 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
                 const uint32_t* __restrict__ entries, int64_t img0, int n_img,
                 float2* __restrict__ inter, uint32_t* __restrict__ isums,
                 const float2* __restrict__ tw) {
-    constexpr int L = BatchLanes<N>::L, RB = RB_, TS = tile_stride(N);
+    constexpr int L = BatchLanes<N>::L, RB = RB_, TS = tile_stride(N, RB_);
     constexpr int NT = RB * L;
     using F = RegFft<N, L>;
     extern __shared__ float4 rows_reg_lds4[];
@@ -91,19 +97,16 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
     for (int i = tid; i < RB * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
     const uint32_t* ent = entries + (int64_t)ch * 2 * g.n_bl;
-    int n_ok = 0;
-    for (int e = rs[0] + tid; e < rs[RB]; e += NT) {
-        int u_idx;
-        float2 v;
-        if (!gather_entry(g, img, ch, ent[e], u_idx, v)) continue;
+    const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[RB], tid, NT,
+                                     [&](int e, int u_idx, float2 v) {
         int r = 0;
         while (r + 1 < RB && rs[r + 1] <= e) r++;
         float2* c = &tile[r * TS + pad_index(u_idx)];
         atomicAdd(&c->x, v.x);
         atomicAdd(&c->y, v.y);
         const int row = r0 + r;
-        n_ok += (row == 0 || row == H) ? 1 : 2;
-    }
+        return (row == 0 || row == H) ? 1 : 2;
+    });
     const int tot = block_sum_int(n_ok, red);
     if (tid == 0 && tot) atomicAdd(&isums[img], (uint32_t)tot);
     __syncthreads();
@@ -118,12 +121,19 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
         F::store(v, lane, [&](int i, float2 x) { row[pad_index(i)] = x; });
     }
     __syncthreads();
-    // transposed store: inter[li][c][r0 + rr], RB consecutive rows per column
+    // transposed store: inter[li][c][r0 + rr], RB consecutive rows per column,
+    // two rows (16 bytes) per thread; a pair is written when either row has
+    // entries (an empty row of the tile holds zeros).
     constexpr int HS = half_stride(N);
     float2* dst = inter + (int64_t)li * N * HS + r0;
-    for (int i = tid; i < RB * N; i += NT) {
-        const int c = i / RB, rr = i % RB;
-        if (rs[rr + 1] != rs[rr]) dst[(int64_t)c * HS + rr] = tile[rr * TS + pad_index(c)];
+    constexpr int RP = RB / 2;
+    for (int i = tid; i < RP * N; i += NT) {
+        const int c = i / RP, rr = 2 * (i % RP);
+        if (rs[rr + 2] != rs[rr]) {
+            const float2 a = tile[rr * TS + pad_index(c)];
+            const float2 b = tile[(rr + 1) * TS + pad_index(c)];
+            *(float4*)&dst[(int64_t)c * HS + rr] = make_float4(a.x, a.y, b.x, b.y);
+        }
     }
 }
 
@@ -132,7 +142,7 @@ __global__ void __launch_bounds__(CB_ * BatchLanes<N>::L)
 k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_t img0,
            int n_ch, const int* __restrict__ row_start, const uint32_t* __restrict__ isums,
            const float2* __restrict__ tw) {
-    constexpr int L = BatchLanes<N>::L, CB = CB_, TS = tile_stride(N);
+    constexpr int L = BatchLanes<N>::L, CB = CB_, TS = tile_stride(N, CB_);
     constexpr int NT = CB * L;
     using F = RegFft<N, L>;
     extern __shared__ float4 cols_reg_lds4[];
@@ -181,10 +191,13 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
     float2* out = images + img * (int64_t)N * N;
     constexpr int H = N / 2;
     const int oc0 = c0 < H ? c0 + H : c0 - H;  // shift of a block that does not straddle N/2
-    for (int i = tid; i < CB * N; i += NT) {
-        const int r = i / CB, w = i % CB;
+    constexpr int CP = CB / 2;  // two columns (16 bytes) per thread
+    for (int i = tid; i < CP * N; i += NT) {
+        const int r = i / CP, w = 2 * (i % CP);
         const int orow = r < H ? r + H : r - H;
-        out[(int64_t)orow * N + oc0 + w] = tile[w * TS + pad_index(r)];
+        const float2 a = tile[w * TS + pad_index(r)];
+        const float2 b = tile[(w + 1) * TS + pad_index(r)];
+        *(float4*)&out[(int64_t)orow * N + oc0 + w] = make_float4(a.x, a.y, b.x, b.y);
     }
 }
 
@@ -230,6 +243,8 @@ bool batch_reg_supported(int n) {
             return false;
     }
 }
+
+int64_t batch_inter_bytes(int n) { return (int64_t)sizeof(float2) * n * half_stride(n); }
 
 int batch_reg_variants(int n, int* tiles, int max_tiles) {
     int k = 0;

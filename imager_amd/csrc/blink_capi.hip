@@ -552,9 +552,16 @@ static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
                          std::vector<SetScratch>& out) {
     out.resize(n_sets);
     int rc;
-    // Row-pass buffer: a chunk of images whose row-transformed grids stay
-    // well inside the 256 MiB Infinity Cache between the two passes.
-    const int64_t inter_budget = 96ll << 20;
+    // Row-pass buffer: up to 4 GiB of row-transformed grids per chunk.  Fewer,
+    // larger launches win over chunks small enough to stay in the 256 MiB
+    // Infinity Cache between the passes: at 1024^2 one 640-image chunk runs
+    // 241k images/s, 12-image chunks 181k (launch ramp and tail of the row
+    // kernel).  BLINK_INTER_MIB overrides the budget, for tuning.
+    int64_t inter_budget = 4096ll << 20;
+    if (const char* e = getenv("BLINK_INTER_MIB")) {
+        const long long mib = atoll(e);
+        if (mib > 0) inter_budget = mib << 20;
+    }
     size_t inter_bytes = 0, rs_bytes = 0, en_bytes = 0, is_bytes = 0;
     std::vector<size_t> rs_off(n_sets), en_off(n_sets), is_off(n_sets);
     for (int i = 0; i < n_sets; i++) {
@@ -562,8 +569,11 @@ static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
         const int64_t n = st.n_pixels;
         const int n_img = st.n_ch * st.n_t;
         const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
-        // the DFT path keeps the grid and the first product (two buffers)
-        const int64_t per = (st.transform == BLINK_TRANSFORM_DFT_BF16X3 ? 16 : 8) * n * n;
+        // the DFT path keeps the grid and the first product (two buffers);
+        // the register path keeps rows 0 .. N/2 of each row-transformed grid
+        int64_t per = 8 * n * n;
+        if (st.transform == BLINK_TRANSFORM_DFT_BF16X3) per = 16 * n * n;
+        else if (batch_reg_supported((int)n)) per = batch_inter_bytes((int)n);
         int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
         out[i].chunk = chunk;
         inter_bytes = std::max(inter_bytes, (size_t)(per * chunk));

@@ -35,6 +35,64 @@ __device__ __forceinline__ bool gather_entry(const GatherArgs& g, int64_t img, i
     return true;
 }
 
+// Gathers the plan entries [e0, e1) of one image, `stride` threads apart, U
+// entries per thread at a time: the U entry words, then the U visibility
+// records and phase inputs are all requested before any is used, so a thread
+// has U independent load chains in flight instead of one.  Out-of-range slots
+// load entry e0 (always valid) and are discarded.  For every gridded
+// visibility sink(entry_index, u_idx, value) is called; the sum of its
+// return values is returned.
+template <int U, class Sink>
+__device__ __forceinline__ int gather_range(const GatherArgs& g, int64_t img, int ch,
+                                            const uint32_t* __restrict__ ent, int e0, int e1,
+                                            int tid, int stride, Sink sink) {
+    int total = 0;
+    const double nu = g.freq_hz[ch];
+    const float4* vis = g.vis + 2 * (img * g.n_bl);
+    for (int base = e0 + tid; base < e1; base += stride * U) {
+        int idx[U];
+        uint32_t ev[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = base + u * stride;
+            idx[u] = i < e1 ? i : -1;
+            ev[u] = ent[i < e1 ? i : e0];
+        }
+        float4 a[U], b[U];
+        float wv[U];
+        double ca[U], cb[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const unsigned bl = ev[u] >> 14;
+            a[u] = g.pol != 1 ? vis[2 * bl] : make_float4(0.f, 0.f, 0.f, 0.f);
+            b[u] = g.pol != 0 ? vis[2 * bl + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            wv[u] = (g.corrections & 1u) ? g.w_m[bl] : 0.f;
+            ca[u] = cb[u] = 0.0;
+            if (g.corrections & 2u) {
+                unsigned a1, a2;
+                baseline_antennas(bl, a1, a2);
+                ca[u] = g.cable_m[a1];
+                cb[u] = g.cable_m[a2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (idx[u] < 0) continue;
+            float2 v = select_pol(a[u], b[u], g.pol);
+            if (g.corrections) {
+                double turns = 0.0;
+                if (g.corrections & 1u) turns += reduce_turns(geometric_turns(wv[u], nu));
+                if (g.corrections & 2u) turns += reduce_turns(cable_turns(cb[u] - ca[u], nu));
+                v = cmul(v, unit_phasor(turns));
+            }
+            if (!vis_ok(v)) continue;
+            if (ev[u] & (1u << 13)) v.y = -v.y;
+            total += sink(idx[u], (int)(ev[u] & 0x1fffu), v);
+        }
+    }
+    return total;
+}
+
 // Block-wide sum of one int per thread; the result is valid in thread 0.
 __device__ __forceinline__ int block_sum_int(int x, int* scratch) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);

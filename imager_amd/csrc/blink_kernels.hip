@@ -302,18 +302,15 @@ k_grid_tiles(GatherArgs g, const int* __restrict__ row_start, const uint32_t* __
         cnt[i] = 0.f;
     }
     __syncthreads();
-    int n_ok = 0;
-    for (int e = rs[0] + threadIdx.x; e < rs[nr]; e += blockDim.x) {
-        int u_idx;
-        float2 v;
-        if (!gather_entry(g, img, ch, ent[e], u_idx, v)) continue;
+    const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[nr], (int)threadIdx.x,
+                                     (int)blockDim.x, [&](int e, int u_idx, float2 v) {
         const int r = row_of_entry(rs, nr, e);
         const int c = r * n_pixels + u_idx;
         atomicAdd(&tile[c].x, v.x);
         atomicAdd(&tile[c].y, v.y);
         atomicAdd(&cnt[c], 1.0f);
-        n_ok++;
-    }
+        return 1;
+    });
     if (isums) {
         const int tot = block_sum_int(n_ok, red);
         if (threadIdx.x == 0 && tot) atomicAdd(&isums[img], (uint32_t)tot);

@@ -104,6 +104,8 @@ size_t dft_planes_bytes(int n);
 // Register-FFT batched kernels (power-of-two sides 128..2048); the row-pass
 // buffer is transposed: inter[img][u][v].
 bool batch_reg_supported(int n);
+// Bytes of the row-pass buffer per image of side n.
+int64_t batch_inter_bytes(int n);
 // Tile sizes (rows per row tile / columns per column block) built for side n.
 int batch_reg_variants(int n, int* tiles, int max_tiles);
 hipError_t set_batch_kernel_attributes();
