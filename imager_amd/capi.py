@@ -41,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "blink_uv_cells", "blink_fft2d_c2c", "blink_vector_sum", "blink_fft_shift_and_norm",
     "blink_image", "blink_image_averaging", "blink_image_batch", "blink_release_cache",
     "blink_profile_enable", "blink_profile_get", "blink_profile_kernels", "blink_profile_reset",
-    "blink_batch_tiles",
+    "blink_batch_tiles", "blink_set_batch_tiles",
 )
 
 
@@ -111,6 +111,7 @@ def load():
         "blink_profile_kernels": (i, [vp, ctypes.c_char_p, i]),
         "blink_profile_reset": (i, [vp]),
         "blink_batch_tiles": (i, [vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "blink_set_batch_tiles": (i, [vp, i, i, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -256,6 +257,11 @@ class Context:
         self._check(load().blink_batch_tiles(self._h, int(n_pixels), ctypes.byref(r),
                                              ctypes.byref(c)))
         return r.value, c.value
+
+    def set_batch_tiles(self, n_pixels, row_tile, col_tile):
+        """Pins the batched kernels' tiles for one side (0, 0: back to the autotuner)."""
+        self._check(load().blink_set_batch_tiles(self._h, int(n_pixels), int(row_tile),
+                                                 int(col_tile)))
 
     # measurement hooks
     def profile_enable(self, on=True):

@@ -54,7 +54,8 @@ struct BatchShape {
     static constexpr int RB = RB_;
 };
 
-// LDS of one workgroup: the tile and a copy of the twiddle table.
+// LDS of one workgroup: RB tile rows (row-tile rows, or column teams) and a
+// copy of the twiddle table.
 template <int N, int RB>
 constexpr size_t batch_lds_bytes() {
     return sizeof(float2) * ((size_t)RB * tile_stride(N, RB) + N);
@@ -137,16 +138,24 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
     }
 }
 
+// Column pass.  The gridded plane is Hermitian by construction (every
+// visibility lands at (u, v) and its conjugate at (-u, -v)), so every image
+// is real and each column sequence c_x[v] = R_v(x) obeys c_x[N-v] = conj(c_x[v]).
+// One complex FFT therefore transforms two columns at once:
+//   z[v] = c_x[v] + i c_{x+1}[v]  ->  Z[y] = img_x[y] + i img_{x+1}[y],
+// with both image columns real; the imaginary parts of the output are 0.
+// CB output columns per workgroup, CB / 2 teams of L lanes.
 template <int N, int CB_>
-__global__ void __launch_bounds__(CB_ * BatchLanes<N>::L)
+__global__ void __launch_bounds__(CB_ / 2 * BatchLanes<N>::L)
 k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_t img0,
            int n_ch, const int* __restrict__ row_start, const uint32_t* __restrict__ isums,
            const float2* __restrict__ tw) {
-    constexpr int L = BatchLanes<N>::L, CB = CB_, TS = tile_stride(N, CB_);
-    constexpr int NT = CB * L;
+    constexpr int L = BatchLanes<N>::L, CB = CB_, TEAMS = CB_ / 2;
+    constexpr int TS = tile_stride(N, TEAMS);
+    constexpr int NT = TEAMS * L;
     using F = RegFft<N, L>;
     extern __shared__ float4 cols_reg_lds4[];
-    float2* tile = (float2*)cols_reg_lds4;  // [CB][TS]
+    float2* tile = (float2*)cols_reg_lds4;  // [TEAMS][TS]
     __shared__ unsigned char nonempty[N];
 
     constexpr int n_cb = N / CB;
@@ -156,7 +165,7 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
     const int tid = threadIdx.x;
     const uint32_t count = isums[img];
     const int* rsg = row_start + (int64_t)((int)img % n_ch) * (N + 1);
-    float2* twl = tile + CB * TS;  // twiddle table copy [N]
+    float2* twl = tile + TEAMS * TS;  // twiddle table copy [N]
     for (int i = tid; i < N; i += NT) {
         nonempty[i] = rsg[i + 1] != rsg[i];
         twl[i] = tw[i];
@@ -166,18 +175,20 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
     const int team = tid / L, lane = tid % L;
     const int c0 = cb * CB;
     constexpr int HS = half_stride(N);
-    const float2* src = inter + ((int64_t)li * N + c0 + team) * HS;
+    const float2* src = inter + ((int64_t)li * N + c0 + 2 * team) * HS;
     float2 v[F::V];
     // Row i > N/2 is the conjugate of stored row N - i.  Rows without plan
     // entries read element 0 (one cached line) and are cleared with a bit
     // mask, so every load is unconditional and all are in flight together.
     F::load(v, lane, [&](int i) {
         const int m = nonempty[i] ? -1 : 0;
-        const int j = i <= N / 2 ? i : N - i;
-        const float2 x = src[j & m];
-        const float y = i <= N / 2 ? x.y : -x.y;
-        return make_float2(__int_as_float(__float_as_int(x.x) & m),
-                           __int_as_float(__float_as_int(y) & m));
+        const int j = (i <= N / 2 ? i : N - i) & m;
+        const float sg = i <= N / 2 ? 1.f : -1.f;
+        const float2 a = src[j];       // column c0 + 2 team
+        const float2 b = src[HS + j];  // column c0 + 2 team + 1
+        const float re = a.x - sg * b.y, im = sg * a.y + b.x;
+        return make_float2(__int_as_float(__float_as_int(re) & m),
+                           __int_as_float(__float_as_int(im) & m));
     });
     float2* row = tile + team * TS;
     F::template run<1>(v, row, lane, twl);
@@ -187,17 +198,16 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
     team_sync<false>();
     F::store(v, lane, [&](int i, float2 x) { row[pad_index(i)] = x; });
     __syncthreads();
-    // shifted store: images[img][shift(r)][shift(c0 + w)], CB columns per row
+    // shifted store: images[img][shift(r)][shift(c0 + 2 t) + {0, 1}] =
+    // (Re Z_t[r], 0), (Im Z_t[r], 0) -- 16 bytes per thread, CB columns per row
     float2* out = images + img * (int64_t)N * N;
     constexpr int H = N / 2;
     const int oc0 = c0 < H ? c0 + H : c0 - H;  // shift of a block that does not straddle N/2
-    constexpr int CP = CB / 2;  // two columns (16 bytes) per thread
-    for (int i = tid; i < CP * N; i += NT) {
-        const int r = i / CP, w = 2 * (i % CP);
+    for (int i = tid; i < TEAMS * N; i += NT) {
+        const int r = i / TEAMS, t = i % TEAMS;
         const int orow = r < H ? r + H : r - H;
-        const float2 a = tile[w * TS + pad_index(r)];
-        const float2 b = tile[(w + 1) * TS + pad_index(r)];
-        *(float4*)&out[(int64_t)orow * N + oc0 + w] = make_float4(a.x, a.y, b.x, b.y);
+        const float2 z = tile[t * TS + pad_index(r)];
+        *(float4*)&out[(int64_t)orow * N + oc0 + 2 * t] = make_float4(z.x, 0.f, z.y, 0.f);
     }
 }
 
@@ -218,8 +228,8 @@ template <int N, int CB>
 static hipError_t launch_cols_reg_t(const float* inter, float* images, int64_t img0, int n_img,
                                     int n_ch, const int* row_start, const uint32_t* isums,
                                     const float* tw, hipStream_t stream) {
-    constexpr int NT = CB * BatchLanes<N>::L;
-    constexpr size_t lds = batch_lds_bytes<N, CB>();
+    constexpr int NT = CB / 2 * BatchLanes<N>::L;
+    constexpr size_t lds = batch_lds_bytes<N, CB / 2>();
     if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)((int64_t)(N / CB) * n_img));
     hipLaunchKernelGGL((k_cols_reg<N, CB>), grid, dim3(NT), lds, stream, (const float2*)inter,
@@ -227,13 +237,20 @@ static hipError_t launch_cols_reg_t(const float* inter, float* images, int64_t i
     return hipGetLastError();
 }
 
-// Tile variants (rows per row tile = columns per column block) per size.
-#define BLINK_BATCH_VARIANTS(X) \
+// Built variants: rows per row tile, and output columns per column block
+// (two columns per team).
+#define BLINK_ROW_VARIANTS(X) \
     X(128, 16) X(128, 32) \
     X(256, 8) X(256, 16) X(256, 32) \
     X(512, 8) X(512, 16) \
     X(1024, 4) X(1024, 8) X(1024, 16) \
     X(2048, 4) X(2048, 8)
+#define BLINK_COL_VARIANTS(X) \
+    X(128, 16) X(128, 32) X(128, 64) \
+    X(256, 16) X(256, 32) X(256, 64) \
+    X(512, 16) X(512, 32) X(512, 64) \
+    X(1024, 8) X(1024, 16) X(1024, 32) \
+    X(2048, 4) X(2048, 8) X(2048, 16)
 
 bool batch_reg_supported(int n) {
     switch (n) {
@@ -246,11 +263,20 @@ bool batch_reg_supported(int n) {
 
 int64_t batch_inter_bytes(int n) { return (int64_t)sizeof(float2) * n * half_stride(n); }
 
-int batch_reg_variants(int n, int* tiles, int max_tiles) {
+int batch_row_variants(int n, int* tiles, int max_tiles) {
     int k = 0;
 #define X(NN, T) \
     if (n == NN && k < max_tiles) tiles[k++] = T;
-    BLINK_BATCH_VARIANTS(X)
+    BLINK_ROW_VARIANTS(X)
+#undef X
+    return k;
+}
+
+int batch_col_variants(int n, int* tiles, int max_tiles) {
+    int k = 0;
+#define X(NN, T) \
+    if (n == NN && k < max_tiles) tiles[k++] = T;
+    BLINK_COL_VARIANTS(X)
 #undef X
     return k;
 }
@@ -262,7 +288,7 @@ hipError_t launch_grid_rows_reg(const GatherArgs& g, const int* row_start, const
 #define X(NN, T) \
     if (n == NN && tile == T) \
         return launch_rows_reg_t<NN, T>(g, row_start, entries, img0, n_img, inter, isums, tw, stream);
-    BLINK_BATCH_VARIANTS(X)
+    BLINK_ROW_VARIANTS(X)
 #undef X
     return hipErrorNotSupported;
 }
@@ -274,7 +300,7 @@ hipError_t launch_cols_reg(const float* inter, float* images, int n, int tile, i
 #define X(NN, T) \
     if (n == NN && tile == T) \
         return launch_cols_reg_t<NN, T>(inter, images, img0, n_img, n_ch, row_start, isums, tw, stream);
-    BLINK_BATCH_VARIANTS(X)
+    BLINK_COL_VARIANTS(X)
 #undef X
     return hipErrorNotSupported;
 }
@@ -286,10 +312,11 @@ hipError_t set_batch_kernel_attributes() {
         hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (r != hipSuccess && e == hipSuccess) e = r;
     };
-#define X(NN, T)                                        \
-    set((const void*)k_grid_rows_reg<NN, T>);           \
-    set((const void*)k_cols_reg<NN, T>);
-    BLINK_BATCH_VARIANTS(X)
+#define X(NN, T) set((const void*)k_grid_rows_reg<NN, T>);
+    BLINK_ROW_VARIANTS(X)
+#undef X
+#define X(NN, T) set((const void*)k_cols_reg<NN, T>);
+    BLINK_COL_VARIANTS(X)
 #undef X
     return e;
 }

@@ -3,8 +3,10 @@
 // kernels in blink_kernels.hip.
 #include "../../include/blink_mi355x.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -606,14 +608,18 @@ static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
     return BLINK_OK;
 }
 
-// Default tiles when the autotuner is off: 16 rows / columns up to N = 1024.
+// Default tiles when the autotuner is off: the largest built variant up to
+// 16 (8 at N = 2048), rows per row tile and output columns per column block.
 static std::pair<int, int> default_tiles(int n) {
-    int t[8];
-    const int k = batch_reg_variants(n, t, 8);
-    int best = t[0];
-    for (int i = 0; i < k; i++)
-        if (t[i] <= (n >= 2048 ? 8 : 16)) best = std::max(best, t[i]);
-    return {best, best};
+    const int cap = n >= 2048 ? 8 : 16;
+    int t[8], best[2];
+    for (int kind = 0; kind < 2; kind++) {
+        const int k = kind == 0 ? batch_row_variants(n, t, 8) : batch_col_variants(n, t, 8);
+        best[kind] = k > 0 ? t[0] : 0;
+        for (int i = 0; i < k; i++)
+            if (t[i] <= cap) best[kind] = std::max(best[kind], t[i]);
+    }
+    return {best[0], best[1]};
 }
 
 // Average GPU milliseconds of `reps` launches made by fn() on stream s.
@@ -643,8 +649,9 @@ extern "C" {
 static int autotune_set(blink_ctx* ctx, const blink_image_set& st, const SetScratch& sc,
                         hipStream_t s) {
     const int n = st.n_pixels;
-    int variants[8];
-    const int nv = batch_reg_variants(n, variants, 8);
+    int rows[8], cols[8];
+    const int nr = batch_row_variants(n, rows, 8);
+    const int nc = batch_col_variants(n, cols, 8);
     const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
     const int n_img = st.n_ch * st.n_t;
     const int cnt = std::min(sc.chunk, n_img);
@@ -660,28 +667,28 @@ static int autotune_set(blink_ctx* ctx, const blink_image_set& st, const SetScra
     g.pol = st.pol;
     g.n_bl = n_bl;
     g.n_ch = st.n_ch;
-    int best_r = variants[0], best_c = variants[0];
+    int best_r = rows[0], best_c = cols[0];
     float best = 1e30f;
-    for (int v = 0; v < nv; v++) {
+    for (int v = 0; v < nr; v++) {
         float ms;
         int rc = time_launches(ctx, s, 3, [&] {
-            return launch_grid_rows_reg(g, sc.row_start, sc.entries, n, variants[v], 0, cnt,
+            return launch_grid_rows_reg(g, sc.row_start, sc.entries, n, rows[v], 0, cnt,
                                         sc.inter, sc.isums, tw, s);
         }, &ms);
         if (rc) return rc;
-        if (ms < best) best = ms, best_r = variants[v];
+        if (ms < best) best = ms, best_r = rows[v];
     }
     HIP_TRY(launch_grid_rows_reg(g, sc.row_start, sc.entries, n, best_r, 0, cnt, sc.inter,
                                  sc.isums, tw, s));
     best = 1e30f;
-    for (int v = 0; v < nv; v++) {
+    for (int v = 0; v < nc; v++) {
         float ms;
         int rc = time_launches(ctx, s, 3, [&] {
-            return launch_cols_reg(sc.inter, st.images, n, variants[v], 0, cnt, st.n_ch,
+            return launch_cols_reg(sc.inter, st.images, n, cols[v], 0, cnt, st.n_ch,
                                    sc.row_start, sc.isums, tw, s);
         }, &ms);
         if (rc) return rc;
-        if (ms < best) best = ms, best_c = variants[v];
+        if (ms < best) best = ms, best_c = cols[v];
     }
     ctx->tiles[n] = {best_r, best_c};
     return BLINK_OK;
@@ -782,6 +789,8 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
         CHECK_ARG(!(st.corrections & BLINK_CORR_GEOMETRIC) || st.w_m, "set %d: w_m is null", i);
         CHECK_ARG(!(st.corrections & BLINK_CORR_CABLE) || st.cable_m, "set %d: cable_m is null", i);
         CHECK_ARG(st.pol >= 0 && st.pol <= 3, "set %d: pol must be 0..3", i);
+        CHECK_ARG(((uintptr_t)st.vis & 15) == 0 && ((uintptr_t)st.images & 15) == 0,
+                  "set %d: vis and images must be 16-byte aligned", i);
         CHECK_ARG(st.transform == BLINK_TRANSFORM_FFT || st.transform == BLINK_TRANSFORM_DFT_BF16X3,
                   "set %d: unknown transform %d", i, st.transform);
         if (st.transform == BLINK_TRANSFORM_DFT_BF16X3) {
@@ -887,6 +896,26 @@ BLINK_API int blink_profile_reset(blink_ctx* ctx) {
     if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
     prof_collect(ctx);
     ctx->prof_totals.clear();
+    return BLINK_OK;
+}
+
+BLINK_API int blink_set_batch_tiles(blink_ctx* ctx, int n_pixels, int row_tile, int col_tile) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(batch_reg_supported(n_pixels), "side %d uses the generic kernels (no tiles)",
+              n_pixels);
+    if (row_tile == 0 && col_tile == 0) {
+        ctx->tiles.erase(n_pixels);
+    } else {
+        int t[8];
+        const int nr = batch_row_variants(n_pixels, t, 8);
+        CHECK_ARG(std::find(t, t + nr, row_tile) != t + nr, "no row tile %d built for side %d",
+                  row_tile, n_pixels);
+        const int nc = batch_col_variants(n_pixels, t, 8);
+        CHECK_ARG(std::find(t, t + nc, col_tile) != t + nc, "no column block %d built for side %d",
+                  col_tile, n_pixels);
+        ctx->tiles[n_pixels] = {row_tile, col_tile};
+    }
+    drop_graphs(ctx);  // captured graphs hold the previous tiles
     return BLINK_OK;
 }
 

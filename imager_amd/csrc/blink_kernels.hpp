@@ -106,8 +106,9 @@ size_t dft_planes_bytes(int n);
 bool batch_reg_supported(int n);
 // Bytes of the row-pass buffer per image of side n.
 int64_t batch_inter_bytes(int n);
-// Tile sizes (rows per row tile / columns per column block) built for side n.
-int batch_reg_variants(int n, int* tiles, int max_tiles);
+// Built tile sizes for side n: rows per row tile, output columns per column block.
+int batch_row_variants(int n, int* tiles, int max_tiles);
+int batch_col_variants(int n, int* tiles, int max_tiles);
 hipError_t set_batch_kernel_attributes();
 hipError_t launch_grid_rows_reg(const GatherArgs& g, const int* row_start, const uint32_t* entries,
                                 int n, int tile, int64_t img0, int n_img, float* inter,

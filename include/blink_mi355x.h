@@ -177,7 +177,7 @@ BLINK_API int blink_image_averaging(blink_ctx* ctx, const float* images, int64_t
 /* One set of images that share an array, a grid size and a polarisation:
  * n_t x n_ch images from vis[t][ch][bl][4].  Every pointer is device memory. */
 typedef struct blink_image_set {
-    const float* vis;        /* [n_t][n_ch][n_bl][4] complex64, read only */
+    const float* vis;        /* [n_t][n_ch][n_bl][4] complex64, read only, 16-byte aligned */
     int n_ant, n_ch, n_t;
     const float* u_m;        /* [n_bl] metres */
     const float* v_m;        /* [n_bl] metres */
@@ -189,7 +189,11 @@ typedef struct blink_image_set {
     double delta_u, delta_v, min_uv;
     int n_pixels;
     int pol;                 /* blink_pol */
-    float* images;           /* [n_t][n_ch][N][N] complex64 output */
+    float* images;           /* [n_t][n_ch][N][N] complex64 output, 16-byte aligned.  The
+                                gridded plane is Hermitian, so the images are real: the FFT
+                                transform at N = 128 .. 2048 (powers of two) writes imaginary
+                                parts 0; other sides and transforms leave FP32 rounding
+                                noise there, as the reference's FFT does */
     double* count_sums;      /* optional [n_t*n_ch] output: sum of counts per image */
     int transform;           /* blink_transform: how the 2-D transform is computed */
 } blink_image_set;
@@ -232,6 +236,12 @@ BLINK_API int blink_profile_reset(blink_ctx* ctx);
  * negative for the defaults not yet tuned, 0 when the side uses the generic
  * kernels. */
 BLINK_API int blink_batch_tiles(blink_ctx* ctx, int n_pixels, int* row_tile, int* col_tile);
+/* Pins the tile sizes of the batched kernels for grid side n_pixels (row_tile
+ * rows per row tile, col_tile output columns per column block; both must be
+ * built variants) and turns the autotuner off for that side; 0, 0 returns the
+ * side to the autotuner.  No counterpart in the reference: for callers that
+ * need reproducible launch shapes, and for tests of every variant. */
+BLINK_API int blink_set_batch_tiles(blink_ctx* ctx, int n_pixels, int row_tile, int col_tile);
 
 #ifdef __cplusplus
 }

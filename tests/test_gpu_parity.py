@@ -342,6 +342,38 @@ def test_image_batch_matches_reference_pipeline(ctx, n, n_ant, pol):
     assert np.array_equal(_host(sums), c_ref.reshape(o.n_images, -1).sum(1).astype(np.float64))
 
 
+@pytest.mark.parametrize("n", [128, 256, 512, 1024, 2048])
+def test_image_batch_every_tile_variant(ctx, n):
+    """Every built row-tile and column-block variant (pinned through
+    blink_set_batch_tiles; unbuilt sizes are refused) against the oracle."""
+    from imager_amd import capi
+    o = synth.small(n_ant=48, n_ch=2, n_t=2, n_pixels=n, seed=3 * n, height_sigma_m=2.0)
+    ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, n)
+    d, imgs, sums, st = _batch(ctx, o)
+    cands = (2, 4, 8, 16, 32, 64, 128)
+    built = {"row": [], "col": []}
+    try:
+        for kind in ("row", "col"):
+            for t in cands:
+                rt, ct = (t, 16 if n < 2048 else 8) if kind == "row" else (16 if n < 2048 else 8, t)
+                try:
+                    ctx.set_batch_tiles(n, rt, ct)
+                except capi.BlinkInvalidArgument:
+                    continue
+                built[kind].append(t)
+                _say(f"batch N={n} tiles rows={rt} cols={ct}")
+                imgs.fill_(float("nan"))
+                ctx.image_batch([st])
+                got = _host(imgs)
+                assert oracle.peak_normalised_error(got, ref)[0] < IMAGE_TOL, (rt, ct)
+                # the gridded plane is Hermitian: the images are real
+                assert not np.any(got.imag), (rt, ct)
+    finally:
+        ctx.set_batch_tiles(n, 0, 0)
+    assert len(built["row"]) >= 2 and len(built["col"]) >= 3, built
+
+
 def test_image_batch_graph_replays_are_identical(ctx):
     import torch
     o = synth.small(n_ant=64, n_ch=4, n_t=2, n_pixels=256, seed=77)
