@@ -61,12 +61,10 @@ constexpr size_t batch_lds_bytes() {
     return sizeof(float2) * ((size_t)RB * tile_stride(N, RB) + N);
 }
 
-template <int N, int RB_>
+template <int N, int RB_, bool kMulti>
 __global__ void __launch_bounds__(RB_ * BatchLanes<N>::L)
-k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
-                const uint32_t* __restrict__ entries, int64_t img0, int n_img,
-                float2* __restrict__ inter, uint32_t* __restrict__ isums,
-                const float2* __restrict__ tw) {
+k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
+                int64_t gi0, float2* __restrict__ inter, const float2* __restrict__ tw) {
     constexpr int L = BatchLanes<N>::L, RB = RB_, TS = tile_stride(N, RB_);
     constexpr int NT = RB * L;
     using F = RegFft<N, L>;
@@ -82,12 +80,15 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
     if ((nwg & 7u) == 0) bid = (bid & 7u) * (nwg >> 3) + (bid >> 3);  // XCD-contiguous images
     const int tile_id = (int)(bid % n_rb);
     const int li = (int)(bid / n_rb);
-    const int64_t img = img0 + li;
-    const int ch = (int)img % g.n_ch;
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const GatherArgs& g = sd.g;
+    const int img = (int)(gi - sd.img_base);  // image within its set
+    const int ch = img % g.n_ch;
     const int r0 = tile_id * RB;
     const int nr = min(RB, H + 1 - r0);  // rows of this tile that are <= N/2
     const int tid = threadIdx.x;
-    const int* rsg = row_start + (int64_t)ch * (N + 1);
+    const int* rsg = sd.row_start + (int64_t)ch * (N + 1);
     if (tid <= nr) rs[tid] = rsg[r0 + tid];
     if (tid > nr && tid <= RB) rs[tid] = rsg[r0 + nr];
     __syncthreads();
@@ -97,7 +98,7 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
     for (int i = tid; i < N; i += NT) twl[i] = tw[i];
     for (int i = tid; i < RB * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    const uint32_t* ent = entries + (int64_t)ch * 2 * g.n_bl;
+    const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
     const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[RB], tid, NT,
                                      [&](int e, int u_idx, float2 v) {
         int r = 0;
@@ -109,7 +110,7 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
         return (row == 0 || row == H) ? 1 : 2;
     });
     const int tot = block_sum_int(n_ok, red);
-    if (tid == 0 && tot) atomicAdd(&isums[img], (uint32_t)tot);
+    if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();
 
     const int team = tid / L, lane = tid % L;
@@ -145,11 +146,10 @@ k_grid_rows_reg(GatherArgs g, const int* __restrict__ row_start,
 //   z[v] = c_x[v] + i c_{x+1}[v]  ->  Z[y] = img_x[y] + i img_{x+1}[y],
 // with both image columns real; the imaginary parts of the output are 0.
 // CB output columns per workgroup, CB / 2 teams of L lanes.
-template <int N, int CB_>
+template <int N, int CB_, bool kMulti>
 __global__ void __launch_bounds__(CB_ / 2 * BatchLanes<N>::L)
-k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_t img0,
-           int n_ch, const int* __restrict__ row_start, const uint32_t* __restrict__ isums,
-           const float2* __restrict__ tw) {
+k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
+           int64_t gi0, const float2* __restrict__ inter, const float2* __restrict__ tw) {
     constexpr int L = BatchLanes<N>::L, CB = CB_, TEAMS = CB_ / 2;
     constexpr int TS = tile_stride(N, TEAMS);
     constexpr int NT = TEAMS * L;
@@ -161,10 +161,12 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
     constexpr int n_cb = N / CB;
     const int cb = blockIdx.x % n_cb;
     const int li = blockIdx.x / n_cb;
-    const int64_t img = img0 + li;
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const int img = (int)(gi - sd.img_base);  // image within its set
     const int tid = threadIdx.x;
-    const uint32_t count = isums[img];
-    const int* rsg = row_start + (int64_t)((int)img % n_ch) * (N + 1);
+    const uint32_t count = sd.isums[img];
+    const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (N + 1);
     float2* twl = tile + TEAMS * TS;  // twiddle table copy [N]
     for (int i = tid; i < N; i += NT) {
         nonempty[i] = rsg[i + 1] != rsg[i];
@@ -200,7 +202,7 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
     __syncthreads();
     // shifted store: images[img][shift(r)][shift(c0 + 2 t) + {0, 1}] =
     // (Re Z_t[r], 0), (Im Z_t[r], 0) -- 16 bytes per thread, CB columns per row
-    float2* out = images + img * (int64_t)N * N;
+    float2* out = sd.images + (int64_t)img * N * N;
     constexpr int H = N / 2;
     const int oc0 = c0 < H ? c0 + H : c0 - H;  // shift of a block that does not straddle N/2
     for (int i = tid; i < TEAMS * N; i += NT) {
@@ -212,28 +214,40 @@ k_cols_reg(const float2* __restrict__ inter, float2* __restrict__ images, int64_
 }
 
 template <int N, int RB>
-static hipError_t launch_rows_reg_t(const GatherArgs& g, const int* row_start,
-                                    const uint32_t* entries, int64_t img0, int n_img, float* inter,
-                                    uint32_t* isums, const float* tw, hipStream_t stream) {
+static hipError_t launch_rows_reg_t(const SetDesc& first, const SetDesc* sets,
+                                    const int* img_set, int n_sets, int64_t gi0, int n_img,
+                                    float* inter, const float* tw, hipStream_t stream) {
     constexpr int NT = RB * BatchLanes<N>::L;
     constexpr size_t lds = batch_lds_bytes<N, RB>();
     if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
-    const dim3 grid((unsigned)((int64_t)((N / 2 + 1 + RB - 1) / RB) * n_img));
-    hipLaunchKernelGGL((k_grid_rows_reg<N, RB>), grid, dim3(NT), lds, stream, g, row_start,
-                       entries, img0, n_img, (float2*)inter, isums, (const float2*)tw);
+    const int64_t blocks = (int64_t)((N / 2 + 1 + RB - 1) / RB) * n_img;
+    if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    if (n_sets == 1)
+        hipLaunchKernelGGL((k_grid_rows_reg<N, RB, false>), dim3((unsigned)blocks), dim3(NT), lds,
+                           stream, first, sets, img_set, gi0, (float2*)inter, (const float2*)tw);
+    else
+        hipLaunchKernelGGL((k_grid_rows_reg<N, RB, true>), dim3((unsigned)blocks), dim3(NT), lds,
+                           stream, first, sets, img_set, gi0, (float2*)inter, (const float2*)tw);
     return hipGetLastError();
 }
 
 template <int N, int CB>
-static hipError_t launch_cols_reg_t(const float* inter, float* images, int64_t img0, int n_img,
-                                    int n_ch, const int* row_start, const uint32_t* isums,
-                                    const float* tw, hipStream_t stream) {
+static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
+                                    const int* img_set, int n_sets, int64_t gi0, int n_img,
+                                    const float* inter, const float* tw, hipStream_t stream) {
     constexpr int NT = CB / 2 * BatchLanes<N>::L;
     constexpr size_t lds = batch_lds_bytes<N, CB / 2>();
     if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
-    const dim3 grid((unsigned)((int64_t)(N / CB) * n_img));
-    hipLaunchKernelGGL((k_cols_reg<N, CB>), grid, dim3(NT), lds, stream, (const float2*)inter,
-                       (float2*)images, img0, n_ch, row_start, isums, (const float2*)tw);
+    const int64_t blocks = (int64_t)(N / CB) * n_img;
+    if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    if (n_sets == 1)
+        hipLaunchKernelGGL((k_cols_reg<N, CB, false>), dim3((unsigned)blocks), dim3(NT), lds,
+                           stream, first, sets, img_set, gi0, (const float2*)inter,
+                           (const float2*)tw);
+    else
+        hipLaunchKernelGGL((k_cols_reg<N, CB, true>), dim3((unsigned)blocks), dim3(NT), lds,
+                           stream, first, sets, img_set, gi0, (const float2*)inter,
+                           (const float2*)tw);
     return hipGetLastError();
 }
 
@@ -281,25 +295,27 @@ int batch_col_variants(int n, int* tiles, int max_tiles) {
     return k;
 }
 
-hipError_t launch_grid_rows_reg(const GatherArgs& g, const int* row_start, const uint32_t* entries,
-                                int n, int tile, int64_t img0, int n_img, float* inter,
-                                uint32_t* isums, const float* tw, hipStream_t stream) {
+hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                                int n_sets, int n, int tile, int64_t gi0, int n_img, float* inter,
+                                const float* tw, hipStream_t stream) {
     if (n_img == 0) return hipSuccess;
 #define X(NN, T) \
     if (n == NN && tile == T) \
-        return launch_rows_reg_t<NN, T>(g, row_start, entries, img0, n_img, inter, isums, tw, stream);
+        return launch_rows_reg_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, tw, \
+                                        stream);
     BLINK_ROW_VARIANTS(X)
 #undef X
     return hipErrorNotSupported;
 }
 
-hipError_t launch_cols_reg(const float* inter, float* images, int n, int tile, int64_t img0,
-                           int n_img, int n_ch, const int* row_start, const uint32_t* isums,
-                           const float* tw, hipStream_t stream) {
+hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                           int n_sets, int n, int tile, int64_t gi0, int n_img,
+                           const float* inter, const float* tw, hipStream_t stream) {
     if (n_img == 0) return hipSuccess;
 #define X(NN, T) \
     if (n == NN && tile == T) \
-        return launch_cols_reg_t<NN, T>(inter, images, img0, n_img, n_ch, row_start, isums, tw, stream);
+        return launch_cols_reg_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, tw, \
+                                        stream);
     BLINK_COL_VARIANTS(X)
 #undef X
     return hipErrorNotSupported;
@@ -312,10 +328,14 @@ hipError_t set_batch_kernel_attributes() {
         hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (r != hipSuccess && e == hipSuccess) e = r;
     };
-#define X(NN, T) set((const void*)k_grid_rows_reg<NN, T>);
+#define X(NN, T)                                       \
+    set((const void*)k_grid_rows_reg<NN, T, false>);   \
+    set((const void*)k_grid_rows_reg<NN, T, true>);
     BLINK_ROW_VARIANTS(X)
 #undef X
-#define X(NN, T) set((const void*)k_cols_reg<NN, T>);
+#define X(NN, T)                                  \
+    set((const void*)k_cols_reg<NN, T, false>);   \
+    set((const void*)k_cols_reg<NN, T, true>);
     BLINK_COL_VARIANTS(X)
 #undef X
     return e;

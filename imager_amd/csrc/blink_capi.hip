@@ -28,6 +28,7 @@ struct DeviceBuffer {
 struct GraphEntry {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    void* blob = nullptr;  // device copy of the call's descriptor tables
     uint64_t last_use = 0;
 };
 
@@ -153,9 +154,12 @@ int set_device(blink_ctx* ctx) {
 }
 
 void drop_graphs(blink_ctx* ctx) {
+    if (ctx->graphs.empty()) return;
+    (void)hipDeviceSynchronize();  // a queued replay may still read a descriptor blob
     for (auto& kv : ctx->graphs) {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+        if (kv.second.blob) (void)hipFree(kv.second.blob);
     }
     ctx->graphs.clear();
 }
@@ -539,20 +543,57 @@ BLINK_API int blink_image_averaging(blink_ctx* ctx, const float* images, int64_t
 }
 
 // ------------------------------------------------------------ batched
+//
+// A call is organised in groups: the sets that share a grid side and a
+// transform.  The FFT groups run one row launch and one column launch per
+// chunk of group images, whatever the number of sets in the group; the
+// kernels find an image's set through a table of set descriptors (SetDesc)
+// and an image-to-set table, uploaded once per captured graph.  Gridding
+// plans are built for every (set, channel) of the call in three launches.
 
-// Enqueues the fused pipeline of every set on `s`.  All scratch must have
-// been reserved beforehand (see reserve_batch) so that this can be captured.
+namespace {
+
 struct SetScratch {
     int* row_start;
     uint32_t* entries;
     uint32_t* isums;
-    float* inter;
-    int chunk;
+    int chunk;  // DFT sets: images per launch triple
 };
 
-static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
-                         std::vector<SetScratch>& out) {
-    out.resize(n_sets);
+struct BatchGroup {
+    int n = 0;
+    int transform = BLINK_TRANSFORM_FFT;
+    bool reg = false;            // register-FFT kernels (power-of-two sides 128..2048)
+    std::vector<int> sets;       // call indices of the sets, in descriptor order
+    int first_desc = 0;          // index of the group's first SetDesc in the table
+    int64_t n_img = 0;
+    int chunk = 1;               // FFT groups: images per row/column launch pair
+    size_t img_set_off = 0;      // blob offset of the image -> set table (2+ sets)
+};
+
+struct BatchPlan {
+    std::vector<BatchGroup> groups;
+    std::vector<SetScratch> sc;  // per call index
+    std::vector<unsigned char> blob;
+    size_t units_off = 0;
+    int n_units = 0, parts = 1, n_max = 0;
+    uint32_t* isums_all = nullptr;
+    int64_t isums_count = 0;
+    int* partial = nullptr;
+    float* inter = nullptr;
+    // host copy of a group's first set descriptor (a kernel argument)
+    const SetDesc& host_desc(const BatchGroup& g) const {
+        return ((const SetDesc*)blob.data())[g.first_desc];
+    }
+};
+
+size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+}  // namespace
+
+// Groups, scratch and descriptor tables of one call.  Every scratch buffer is
+// reserved here, before any capture.
+static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, BatchPlan& bp) {
     int rc;
     // Row-pass buffer: up to 4 GiB of row-transformed grids per chunk.  Fewer,
     // larger launches win over chunks small enough to stay in the 256 MiB
@@ -564,38 +605,86 @@ static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
         const long long mib = atoll(e);
         if (mib > 0) inter_budget = mib << 20;
     }
-    size_t inter_bytes = 0, rs_bytes = 0, en_bytes = 0, is_bytes = 0;
+    // groups in order of first appearance
+    std::map<std::pair<int, int>, int> gidx;
+    for (int i = 0; i < n_sets; i++) {
+        const std::pair<int, int> key(sets[i].n_pixels, sets[i].transform);
+        auto it = gidx.find(key);
+        if (it == gidx.end()) {
+            it = gidx.emplace(key, (int)bp.groups.size()).first;
+            BatchGroup g;
+            g.n = sets[i].n_pixels;
+            g.transform = sets[i].transform;
+            g.reg = g.transform == BLINK_TRANSFORM_FFT && batch_reg_supported(g.n);
+            bp.groups.push_back(g);
+        }
+        BatchGroup& g = bp.groups[it->second];
+        g.sets.push_back(i);
+        g.n_img += (int64_t)sets[i].n_ch * sets[i].n_t;
+    }
+    // scratch sizes
+    size_t rs_bytes = 0, en_bytes = 0, inter_bytes = 16;
     std::vector<size_t> rs_off(n_sets), en_off(n_sets), is_off(n_sets);
+    int64_t n_img_all = 0, n_units = 0, min_bl = INT64_MAX;
+    int n_max = 0;
     for (int i = 0; i < n_sets; i++) {
         const blink_image_set& st = sets[i];
         const int64_t n = st.n_pixels;
-        const int n_img = st.n_ch * st.n_t;
         const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
-        // the DFT path keeps the grid and the first product (two buffers);
-        // the register path keeps rows 0 .. N/2 of each row-transformed grid
-        int64_t per = 8 * n * n;
-        if (st.transform == BLINK_TRANSFORM_DFT_BF16X3) per = 16 * n * n;
-        else if (batch_reg_supported((int)n)) per = batch_inter_bytes((int)n);
-        int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
-        out[i].chunk = chunk;
-        inter_bytes = std::max(inter_bytes, (size_t)(per * chunk));
         rs_off[i] = rs_bytes;
-        rs_bytes += sizeof(int) * (size_t)st.n_ch * (n + 1);
+        rs_bytes += align16(sizeof(int) * (size_t)st.n_ch * (n + 1));
         en_off[i] = en_bytes;
-        en_bytes += sizeof(uint32_t) * 2 * (size_t)st.n_ch * std::max(n_bl, 1);
-        is_off[i] = is_bytes;
-        is_bytes += sizeof(uint32_t) * (size_t)std::max(n_img, 1);
+        en_bytes += align16(sizeof(uint32_t) * 2 * (size_t)st.n_ch * std::max(n_bl, 1));
+        is_off[i] = (size_t)n_img_all;
+        n_img_all += (int64_t)st.n_ch * st.n_t;
+        n_units += st.n_ch;
+        min_bl = std::min<int64_t>(min_bl, n_bl);
+        n_max = std::max(n_max, st.n_pixels);
     }
-    void *inter, *rs, *en, *is;
-    if ((rc = scratch(ctx, "batch_inter", std::max<size_t>(inter_bytes, 16), &inter))) return rc;
+    for (BatchGroup& g : bp.groups) {
+        const int64_t n = g.n;
+        if (g.transform == BLINK_TRANSFORM_DFT_BF16X3) {
+            for (int i : g.sets) {
+                const int n_img = sets[i].n_ch * sets[i].n_t;
+                const int64_t per = 16 * n * n;  // the grid and the first product
+                const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
+                inter_bytes = std::max(inter_bytes, (size_t)(per * chunk));
+            }
+            continue;
+        }
+        const int64_t per = g.reg ? batch_inter_bytes(g.n) : 8 * n * n;
+        g.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(g.n_img, inter_budget / per));
+        inter_bytes = std::max(inter_bytes, (size_t)(per * g.chunk));
+    }
+    // plan parts: about 2048 workgroups over the units, at least 256
+    // baselines per part, at most 64 parts
+    int parts = (int)std::max<int64_t>(1, (2048 + n_units - 1) / std::max<int64_t>(n_units, 1));
+    parts = (int)std::min<int64_t>({(int64_t)parts, 64, std::max<int64_t>(1, min_bl / 256)});
+    bp.parts = parts;
+    bp.n_units = (int)n_units;
+    bp.n_max = n_max;
+    bp.isums_count = n_img_all;
+    const size_t partial_bytes = sizeof(int) * (size_t)n_units * parts * n_max;
+
+    void *inter, *rs, *en, *is, *pa;
+    if ((rc = scratch(ctx, "batch_inter", inter_bytes, &inter))) return rc;
     if ((rc = scratch(ctx, "batch_rows", std::max<size_t>(rs_bytes, 16), &rs))) return rc;
     if ((rc = scratch(ctx, "batch_entries", std::max<size_t>(en_bytes, 16), &en))) return rc;
-    if ((rc = scratch(ctx, "batch_isums", std::max<size_t>(is_bytes, 16), &is))) return rc;
+    if ((rc = scratch(ctx, "batch_isums", std::max<size_t>(4 * (size_t)n_img_all, 16), &is)))
+        return rc;
+    if ((rc = scratch(ctx, "batch_plan_parts", std::max<size_t>(partial_bytes, 16), &pa)))
+        return rc;
+    bp.inter = (float*)inter;
+    bp.isums_all = (uint32_t*)is;
+    bp.partial = (int*)pa;
+    bp.sc.resize(n_sets);
     for (int i = 0; i < n_sets; i++) {
-        out[i].inter = (float*)inter;
-        out[i].row_start = (int*)((char*)rs + rs_off[i]);
-        out[i].entries = (uint32_t*)((char*)en + en_off[i]);
-        out[i].isums = (uint32_t*)((char*)is + is_off[i]);
+        bp.sc[i].row_start = (int*)((char*)rs + rs_off[i]);
+        bp.sc[i].entries = (uint32_t*)((char*)en + en_off[i]);
+        bp.sc[i].isums = (uint32_t*)is + is_off[i];
+        const int64_t per = 16 * (int64_t)sets[i].n_pixels * sets[i].n_pixels;
+        const int n_img = sets[i].n_ch * sets[i].n_t;
+        bp.sc[i].chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
     }
     for (int i = 0; i < n_sets; i++) {
         const float* tw;
@@ -605,7 +694,85 @@ static int reserve_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets
             if ((rc = dft_planes(ctx, sets[i].n_pixels, &pl))) return rc;
         }
     }
+
+    // descriptor blob: SetDesc table (group order), units, image -> set tables
+    std::vector<SetDesc> desc;
+    std::vector<int> desc_of(n_sets);
+    for (BatchGroup& g : bp.groups) {
+        g.first_desc = (int)desc.size();
+        int64_t base = 0;
+        for (int i : g.sets) {
+            const blink_image_set& st = sets[i];
+            SetDesc d{};
+            d.g.vis = (const float4*)st.vis;
+            d.g.w_m = st.w_m;
+            d.g.cable_m = st.cable_m;
+            d.g.freq_hz = st.freq_hz;
+            d.g.corrections = st.corrections & 3u;
+            d.g.pol = st.pol;
+            d.g.n_bl = st.n_ant * (st.n_ant + 1) / 2;
+            d.g.n_ch = st.n_ch;
+            d.u_m = st.u_m;
+            d.v_m = st.v_m;
+            d.flags = st.bl_flags;
+            d.du = st.delta_u;
+            d.dv = st.delta_v;
+            d.min_uv = st.min_uv;
+            d.n_pixels = st.n_pixels;
+            d.n_img = st.n_ch * st.n_t;
+            d.img_base = base;
+            d.row_start = bp.sc[i].row_start;
+            d.entries = bp.sc[i].entries;
+            d.isums = bp.sc[i].isums;
+            d.images = (float2*)st.images;
+            base += d.n_img;
+            desc_of[i] = (int)desc.size();
+            desc.push_back(d);
+        }
+    }
+    size_t off = align16(sizeof(SetDesc) * desc.size());
+    bp.units_off = off;
+    off = align16(off + sizeof(int2) * (size_t)n_units);
+    for (BatchGroup& g : bp.groups) {
+        if (g.sets.size() < 2 || g.transform != BLINK_TRANSFORM_FFT) continue;
+        g.img_set_off = off;
+        off = align16(off + sizeof(int) * (size_t)g.n_img);
+    }
+    bp.blob.assign(off, 0);
+    memcpy(bp.blob.data(), desc.data(), sizeof(SetDesc) * desc.size());
+    int2* units = (int2*)(bp.blob.data() + bp.units_off);
+    int64_t u = 0;
+    for (int i = 0; i < n_sets; i++)
+        for (int ch = 0; ch < sets[i].n_ch; ch++) units[u++] = make_int2(desc_of[i], ch);
+    for (BatchGroup& g : bp.groups) {
+        if (g.sets.size() < 2 || g.transform != BLINK_TRANSFORM_FFT) continue;
+        int* t = (int*)(bp.blob.data() + g.img_set_off);
+        int64_t k = 0;
+        for (size_t j = 0; j < g.sets.size(); j++) {
+            const int n_img = sets[g.sets[j]].n_ch * sets[g.sets[j]].n_t;
+            for (int q = 0; q < n_img; q++) t[k++] = (int)j;
+        }
+    }
     return BLINK_OK;
+}
+
+// Device addresses of the blob's tables.
+struct BatchTables {
+    const SetDesc* sets;
+    const int2* units;
+    const unsigned char* base;
+    const SetDesc* group_sets(const BatchGroup& g) const { return sets + g.first_desc; }
+    const int* img_set(const BatchGroup& g) const {
+        return g.sets.size() < 2 ? nullptr : (const int*)(base + g.img_set_off);
+    }
+};
+
+static BatchTables tables_at(const BatchPlan& bp, const void* dblob) {
+    BatchTables t;
+    t.base = (const unsigned char*)dblob;
+    t.sets = (const SetDesc*)dblob;
+    t.units = (const int2*)(t.base + bp.units_off);
+    return t;
 }
 
 // Default tiles when the autotuner is off: the largest built variant up to
@@ -644,48 +811,37 @@ static int time_launches(blink_ctx* ctx, hipStream_t s, int reps, F fn, float* m
 }
 extern "C" {
 
-// Chooses the row-tile and column-block sizes of the batched kernels for one
-// set's grid side by timing every built variant on the set's first chunk.
-static int autotune_set(blink_ctx* ctx, const blink_image_set& st, const SetScratch& sc,
-                        hipStream_t s) {
-    const int n = st.n_pixels;
+// Chooses the row-tile and column-block sizes of the register kernels for
+// one grid side by timing every built variant on the group's first chunk.
+// The plans must have been built (zero + plan launches) on s beforehand.
+static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup& g,
+                          const BatchTables& t, hipStream_t s) {
+    const int n = g.n;
     int rows[8], cols[8];
     const int nr = batch_row_variants(n, rows, 8);
     const int nc = batch_col_variants(n, cols, 8);
-    const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
-    const int n_img = st.n_ch * st.n_t;
-    const int cnt = std::min(sc.chunk, n_img);
+    const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img);
     const float* tw = ctx->twiddles[n];
-    HIP_TRY(launch_plan_rows(st.u_m, st.v_m, st.bl_flags, n_bl, st.freq_hz, st.n_ch, st.delta_u,
-                             st.delta_v, st.min_uv, n, sc.row_start, sc.entries, s));
-    GatherArgs g{};
-    g.vis = (const float4*)st.vis;
-    g.w_m = st.w_m;
-    g.cable_m = st.cable_m;
-    g.freq_hz = st.freq_hz;
-    g.corrections = st.corrections & 3u;
-    g.pol = st.pol;
-    g.n_bl = n_bl;
-    g.n_ch = st.n_ch;
+    const SetDesc* sd = t.group_sets(g);
+    const SetDesc& first = bp.host_desc(g);
+    const int* is = t.img_set(g);
+    const int ns = (int)g.sets.size();
     int best_r = rows[0], best_c = cols[0];
     float best = 1e30f;
     for (int v = 0; v < nr; v++) {
         float ms;
         int rc = time_launches(ctx, s, 3, [&] {
-            return launch_grid_rows_reg(g, sc.row_start, sc.entries, n, rows[v], 0, cnt,
-                                        sc.inter, sc.isums, tw, s);
+            return launch_grid_rows_reg(first, sd, is, ns, n, rows[v], 0, cnt, bp.inter, tw, s);
         }, &ms);
         if (rc) return rc;
         if (ms < best) best = ms, best_r = rows[v];
     }
-    HIP_TRY(launch_grid_rows_reg(g, sc.row_start, sc.entries, n, best_r, 0, cnt, sc.inter,
-                                 sc.isums, tw, s));
+    HIP_TRY(launch_grid_rows_reg(first, sd, is, ns, n, best_r, 0, cnt, bp.inter, tw, s));
     best = 1e30f;
     for (int v = 0; v < nc; v++) {
         float ms;
         int rc = time_launches(ctx, s, 3, [&] {
-            return launch_cols_reg(sc.inter, st.images, n, cols[v], 0, cnt, st.n_ch,
-                                   sc.row_start, sc.isums, tw, s);
+            return launch_cols_reg(first, sd, is, ns, n, cols[v], 0, cnt, bp.inter, tw, s);
         }, &ms);
         if (rc) return rc;
         if (ms < best) best = ms, best_c = cols[v];
@@ -694,85 +850,111 @@ static int autotune_set(blink_ctx* ctx, const blink_image_set& st, const SetScra
     return BLINK_OK;
 }
 
-static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
-                         const std::vector<SetScratch>& sc, hipStream_t s) {
-    for (int i = 0; i < n_sets; i++) {
-        const blink_image_set& st = sets[i];
-        const int n = st.n_pixels;
-        const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
-        const int n_img = st.n_ch * st.n_t;
-        FftPlan plan;
-        make_fft_plan(n, plan);
+// Zeroes the count sums and builds every plan of the call on s.
+static int enqueue_plans(blink_ctx* ctx, const BatchPlan& bp, const BatchTables& t,
+                         hipStream_t s) {
+    LAUNCH("zero", s, launch_zero(bp.isums_all, sizeof(uint32_t) * bp.isums_count, s));
+    LAUNCH("plan", s,
+           launch_plan_units(t.sets, t.units, bp.n_units, bp.parts, bp.n_max, bp.partial, s));
+    return BLINK_OK;
+}
+
+// Enqueues the whole call on s.  All scratch was reserved by build_batch, so
+// this can be captured.
+static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const BatchPlan& bp,
+                         const BatchTables& t, hipStream_t s) {
+    int rc = enqueue_plans(ctx, bp, t, s);
+    if (rc) return rc;
+    for (const BatchGroup& g : bp.groups) {
+        const int n = g.n;
         const float* tw = ctx->twiddles[n];
-        LAUNCH("zero", s, launch_zero(sc[i].isums, sizeof(uint32_t) * (int64_t)n_img, s));
-        LAUNCH("plan_rows", s,
-               launch_plan_rows(st.u_m, st.v_m, st.bl_flags, n_bl, st.freq_hz, st.n_ch,
-                                st.delta_u, st.delta_v, st.min_uv, n, sc[i].row_start,
-                                sc[i].entries, s));
-        GatherArgs g{};
-        g.vis = (const float4*)st.vis;
-        g.w_m = st.w_m;
-        g.cable_m = st.cable_m;
-        g.freq_hz = st.freq_hz;
-        g.corrections = st.corrections & 3u;
-        g.pol = st.pol;
-        g.n_bl = n_bl;
-        g.n_ch = st.n_ch;
         const int64_t gs = (int64_t)n * n;
-        if (st.transform == BLINK_TRANSFORM_DFT_BF16X3) {
-            // grid (overwrite) -> G D'^T -> (.) D'^T with scale, per chunk
+        if (g.transform == BLINK_TRANSFORM_DFT_BF16X3) {
+            // per set: grid (overwrite) -> G D'^T -> (.) D'^T with scale, per chunk
             const uint16_t* pl = ctx->dft_planes[n];
-            float* gbuf = sc[i].inter;
-            float* tbuf = sc[i].inter + 2 * gs * sc[i].chunk;
-            for (int img0 = 0; img0 < n_img; img0 += sc[i].chunk) {
-                const int cnt = std::min(sc[i].chunk, n_img - img0);
-                LAUNCH("grid_tiles", s,
-                       launch_grid_tiles(g, sc[i].row_start, sc[i].entries, cnt, n, gbuf, nullptr,
-                                         s, 1, sc[i].isums, img0));
-                LAUNCH("dft_gemm", s,
-                       launch_dft_gemm(gbuf, pl, tbuf, n, cnt, img0, nullptr, nullptr, s));
-                LAUNCH("dft_gemm", s,
-                       launch_dft_gemm(tbuf, pl, st.images + 2 * gs * img0, n, cnt, img0,
-                                       sc[i].isums, nullptr, s));
+            for (int i : g.sets) {
+                const blink_image_set& st = sets[i];
+                const SetScratch& sc = bp.sc[i];
+                const int n_img = st.n_ch * st.n_t;
+                GatherArgs ga{};
+                ga.vis = (const float4*)st.vis;
+                ga.w_m = st.w_m;
+                ga.cable_m = st.cable_m;
+                ga.freq_hz = st.freq_hz;
+                ga.corrections = st.corrections & 3u;
+                ga.pol = st.pol;
+                ga.n_bl = st.n_ant * (st.n_ant + 1) / 2;
+                ga.n_ch = st.n_ch;
+                float* gbuf = bp.inter;
+                float* tbuf = bp.inter + 2 * gs * sc.chunk;
+                for (int img0 = 0; img0 < n_img; img0 += sc.chunk) {
+                    const int cnt = std::min(sc.chunk, n_img - img0);
+                    LAUNCH("grid_tiles", s,
+                           launch_grid_tiles(ga, sc.row_start, sc.entries, cnt, n, gbuf, nullptr,
+                                             s, 1, sc.isums, img0));
+                    LAUNCH("dft_gemm", s,
+                           launch_dft_gemm(gbuf, pl, tbuf, n, cnt, img0, nullptr, nullptr, s));
+                    LAUNCH("dft_gemm", s,
+                           launch_dft_gemm(tbuf, pl, st.images + 2 * gs * img0, n, cnt, img0,
+                                           sc.isums, nullptr, s));
+                }
             }
-            if (st.count_sums)
-                LAUNCH("isums_to_double", s,
-                       launch_isums_to_double(sc[i].isums, st.count_sums, n_img, s));
             continue;
         }
-        const bool reg = batch_reg_supported(n);
-        for (int img0 = 0; img0 < n_img; img0 += sc[i].chunk) {
-            const int cnt = std::min(sc[i].chunk, n_img - img0);
-            if (reg) {
+        const SetDesc* sd = t.group_sets(g);
+        const SetDesc& first = bp.host_desc(g);
+        const int* is = t.img_set(g);
+        const int ns = (int)g.sets.size();
+        FftPlan plan;
+        make_fft_plan(n, plan);
+        for (int64_t gi0 = 0; gi0 < g.n_img; gi0 += g.chunk) {
+            const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img - gi0);
+            if (g.reg) {
                 auto it = ctx->tiles.find(n);
-                const std::pair<int, int> t = it != ctx->tiles.end() ? it->second : default_tiles(n);
+                const std::pair<int, int> tl = it != ctx->tiles.end() ? it->second : default_tiles(n);
                 LAUNCH("grid_rows_fft", s,
-                       launch_grid_rows_reg(g, sc[i].row_start, sc[i].entries, n, t.first, img0,
-                                            cnt, sc[i].inter, sc[i].isums, tw, s));
+                       launch_grid_rows_reg(first, sd, is, ns, n, tl.first, gi0, cnt, bp.inter,
+                                            tw, s));
                 LAUNCH("fft_cols", s,
-                       launch_cols_reg(sc[i].inter, st.images, n, t.second, img0, cnt, st.n_ch,
-                                       sc[i].row_start, sc[i].isums, tw, s));
+                       launch_cols_reg(first, sd, is, ns, n, tl.second, gi0, cnt, bp.inter, tw,
+                                       s));
                 continue;
             }
             LAUNCH("grid_rows_fft", s,
-                   launch_grid_rows_fft(g, sc[i].row_start, sc[i].entries, plan, img0, cnt,
-                                        sc[i].inter, sc[i].isums, tw, s));
+                   launch_grid_rows_fft(first, sd, is, ns, plan, gi0, cnt, bp.inter, tw, s));
             ColArgs a{};
-            a.in = (const float2*)sc[i].inter;
-            a.out = (float2*)(st.images + 2 * gs * img0);
-            a.in_img_stride = a.out_img_stride = gs;
-            a.out_img_offset = img0;
+            a.in = (const float2*)bp.inter;
+            a.in_img_stride = gs;
+            a.out_img_offset = gi0;
             a.paired = 0;   // out of place: no shift partner needed
             a.shift = 1;
-            a.isums = sc[i].isums;
-            a.row_start = sc[i].row_start;
-            a.n_ch = st.n_ch;
+            a.sets = sd;
+            a.img_set = is;
+            a.n_sets = ns;
+            a.first = first;
             LAUNCH("fft_cols", s, launch_fft_cols(a, plan, cnt, BLINK_FFT_BACKWARD, tw, s));
         }
-        if (st.count_sums)
-            LAUNCH("isums_to_double", s,
-                   launch_isums_to_double(sc[i].isums, st.count_sums, n_img, s));
     }
+    for (int i = 0; i < (int)bp.sc.size(); i++)
+        if (sets[i].count_sums)
+            LAUNCH("isums_to_double", s,
+                   launch_isums_to_double(bp.sc[i].isums, sets[i].count_sums,
+                                          sets[i].n_ch * sets[i].n_t, s));
+    return BLINK_OK;
+}
+
+// Autotunes the register kernels of every group whose side has no tiles yet.
+static int autotune_batch(blink_ctx* ctx, const BatchPlan& bp, const BatchTables& t,
+                          hipStream_t s) {
+    if (!ctx->autotune) return BLINK_OK;
+    bool need = false;
+    for (const BatchGroup& g : bp.groups) need |= g.reg && !ctx->tiles.count(g.n);
+    if (!need) return BLINK_OK;
+    int rc = enqueue_plans(ctx, bp, t, s);
+    if (rc) return rc;
+    for (const BatchGroup& g : bp.groups)
+        if (g.reg && !ctx->tiles.count(g.n) && (rc = autotune_group(ctx, bp, g, t, s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
     return BLINK_OK;
 }
 
@@ -806,39 +988,57 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
     }
     int rc = set_device(ctx);
     if (rc) return rc;
-    std::vector<SetScratch> sc;
-    if ((rc = reserve_batch(ctx, sets, n_sets, sc))) return rc;
+    BatchPlan bp;
+    if ((rc = build_batch(ctx, sets, n_sets, bp))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    if (ctx->autotune) {
-        for (int i = 0; i < n_sets; i++) {
-            const int n = sets[i].n_pixels;
-            if (sets[i].transform != BLINK_TRANSFORM_FFT || !batch_reg_supported(n) ||
-                ctx->tiles.count(n))
-                continue;
-            if ((rc = autotune_set(ctx, sets[i], sc[i], s))) return rc;
-        }
+
+    if (!ctx->use_graphs || ctx->prof_on) {
+        // eager: the descriptor blob goes to a context buffer that no queued
+        // work may still read
+        void* dblob;
+        if ((rc = scratch(ctx, "batch_desc", bp.blob.size(), &dblob))) return rc;
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemcpy(dblob, bp.blob.data(), bp.blob.size(), hipMemcpyHostToDevice));
+        const BatchTables t = tables_at(bp, dblob);
+        if ((rc = autotune_batch(ctx, bp, t, s))) return rc;
+        return enqueue_batch(ctx, sets, bp, t, s);
     }
-    if (!ctx->use_graphs || ctx->prof_on) return enqueue_batch(ctx, sets, n_sets, sc, s);
 
     std::string key((const char*)sets, sizeof(blink_image_set) * (size_t)n_sets);
     auto it = ctx->graphs.find(key);
     if (it == ctx->graphs.end()) {
         GraphEntry ge;
-        hipStream_t cs = ctx->capture_stream;
-        HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_batch(ctx, sets, n_sets, sc, cs);
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamEndCapture(cs, &graph);
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
+        HIP_TRY(hipMalloc(&ge.blob, bp.blob.size()));
+        hipError_t e = hipMemcpy(ge.blob, bp.blob.data(), bp.blob.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(ge.blob);
+            return ctx->fail(BLINK_ERR_HIP, "descriptor upload failed: %s", hipGetErrorString(e));
+        }
+        const BatchTables t = tables_at(bp, ge.blob);
+        if ((rc = autotune_batch(ctx, bp, t, s))) {
+            (void)hipFree(ge.blob);
             return rc;
         }
-        if (e != hipSuccess)
+        hipStream_t cs = ctx->capture_stream;
+        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            (void)hipFree(ge.blob);
             return ctx->fail(BLINK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        }
+        rc = enqueue_batch(ctx, sets, bp, t, cs);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(cs, &graph);
+        if (rc || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipFree(ge.blob);
+            if (rc) return rc;
+            return ctx->fail(BLINK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        }
         ge.graph = graph;
         e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
         if (e != hipSuccess) {
             (void)hipGraphDestroy(graph);
+            (void)hipFree(ge.blob);
             return ctx->fail(BLINK_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         }
         if (ctx->graphs.size() >= kMaxGraphs) {
@@ -849,6 +1049,7 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
             HIP_TRY(hipDeviceSynchronize());
             (void)hipGraphExecDestroy(lru->second.exec);
             (void)hipGraphDestroy(lru->second.graph);
+            if (lru->second.blob) (void)hipFree(lru->second.blob);
             ctx->graphs.erase(lru);
         }
         it = ctx->graphs.emplace(key, ge).first;

@@ -257,6 +257,141 @@ hipError_t launch_plan_rows(const float* u_m, const float* v_m, const int* flags
     return hipGetLastError();
 }
 
+// ------------------------------------------ (b) gridding plans of a batched call
+//
+// One plan unit per (set, channel): the counting sort of the channel's 2 n_bl
+// cells by grid row, the same result k_plan_rows gives.  `parts` workgroups
+// per unit share its baselines: each counts its own slice into a private
+// histogram (count), a per-unit scan turns the histograms into row starts and
+// per-part write offsets (scan), and each part scatters its slice (scatter).
+// Entry order inside a row is not fixed; the sums it feeds are FP32 sums in
+// any order either way.
+
+constexpr int kUnitThreads = 256;
+
+__device__ __forceinline__ void part_range(int n_bl, int parts, int part, int& b0, int& b1) {
+    const int per = (n_bl + parts - 1) / parts;
+    b0 = min(n_bl, part * per);
+    b1 = min(n_bl, b0 + per);
+}
+
+__global__ void __launch_bounds__(kUnitThreads)
+k_plan_count(const SetDesc* __restrict__ sets, const int2* __restrict__ units, int parts,
+             int n_max, int* __restrict__ partial) {
+    extern __shared__ int unit_lds[];
+    int* hist = unit_lds;  // [n]
+    const int unit = blockIdx.x / parts, part = blockIdx.x % parts;
+    const int2 us = units[unit];
+    const SetDesc sd = sets[us.x];
+    const int n = sd.n_pixels, n_bl = sd.g.n_bl;
+    const double wl = wavelength_of(sd.g.freq_hz[us.y]);
+    for (int i = threadIdx.x; i < n; i += kUnitThreads) hist[i] = 0;
+    __syncthreads();
+    int b0, b1;
+    part_range(n_bl, parts, part, b0, b1);
+    for (int bl = b0 + (int)threadIdx.x; bl < b1; bl += kUnitThreads) {
+        const int f = sd.flags[bl];
+        const float u = sd.u_m[bl], v = sd.v_m[bl];
+        int up, vp;
+        if (f || !cell_of(u, v, wl, sd.du, sd.dv, sd.min_uv, n, up, vp)) continue;
+        atomicAdd(&hist[wrap_index(vp, n)], 1);
+        atomicAdd(&hist[wrap_index(-vp, n)], 1);
+    }
+    __syncthreads();
+    int* out = partial + ((int64_t)unit * parts + part) * n_max;
+    for (int i = threadIdx.x; i < n; i += kUnitThreads) out[i] = hist[i];
+}
+
+__global__ void __launch_bounds__(kUnitThreads)
+k_plan_scan(const SetDesc* __restrict__ sets, const int2* __restrict__ units, int parts,
+            int n_max, int* __restrict__ partial) {
+    __shared__ int warp_tot[kUnitThreads / 64];
+    const int unit = blockIdx.x;
+    const int2 us = units[unit];
+    const SetDesc sd = sets[us.x];
+    const int n = sd.n_pixels;
+    int* rs = sd.row_start + (int64_t)us.y * (n + 1);
+    int* base = partial + (int64_t)unit * parts * n_max;
+    const int tid = threadIdx.x;
+    // each thread owns a contiguous chunk of rows: sum, block scan, rewrite
+    const int chunk = (n + kUnitThreads - 1) / kUnitThreads;
+    const int c0 = min(tid * chunk, n), c1 = min(c0 + chunk, n);
+    int local = 0;
+    for (int i = c0; i < c1; i++)
+        for (int p = 0; p < parts; p++) local += base[(int64_t)p * n_max + i];
+    const int lane = tid & 63, wid = tid >> 6;
+    int x = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) warp_tot[wid] = x;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < kUnitThreads / 64; w++) {
+        if (w < wid) before += warp_tot[w];
+        total += warp_tot[w];
+    }
+    int run = before + x - local;
+    for (int i = c0; i < c1; i++) {
+        rs[i] = run;
+        for (int p = 0; p < parts; p++) {
+            int* q = base + (int64_t)p * n_max + i;
+            const int h = *q;
+            *q = run;  // this part's first slot in row i
+            run += h;
+        }
+    }
+    if (tid == 0) rs[n] = total;
+}
+
+__global__ void __launch_bounds__(kUnitThreads)
+k_plan_scatter(const SetDesc* __restrict__ sets, const int2* __restrict__ units, int parts,
+               int n_max, const int* __restrict__ partial) {
+    extern __shared__ int unit_lds[];
+    int* cursor = unit_lds;  // [n]
+    const int unit = blockIdx.x / parts, part = blockIdx.x % parts;
+    const int2 us = units[unit];
+    const SetDesc sd = sets[us.x];
+    const int n = sd.n_pixels, n_bl = sd.g.n_bl;
+    const double wl = wavelength_of(sd.g.freq_hz[us.y]);
+    const int* off = partial + ((int64_t)unit * parts + part) * n_max;
+    for (int i = threadIdx.x; i < n; i += kUnitThreads) cursor[i] = off[i];
+    __syncthreads();
+    uint32_t* ent = sd.entries + (int64_t)us.y * 2 * n_bl;
+    int b0, b1;
+    part_range(n_bl, parts, part, b0, b1);
+    for (int bl = b0 + (int)threadIdx.x; bl < b1; bl += kUnitThreads) {
+        const int f = sd.flags[bl];
+        const float u = sd.u_m[bl], v = sd.v_m[bl];
+        int up, vp;
+        if (f || !cell_of(u, v, wl, sd.du, sd.dv, sd.min_uv, n, up, vp)) continue;
+        const int p = atomicAdd(&cursor[wrap_index(vp, n)], 1);
+        ent[p] = (uint32_t)wrap_index(up, n) | ((uint32_t)bl << 14);
+        const int q = atomicAdd(&cursor[wrap_index(-vp, n)], 1);
+        ent[q] = (uint32_t)wrap_index(-up, n) | (1u << 13) | ((uint32_t)bl << 14);
+    }
+}
+
+hipError_t launch_plan_units(const SetDesc* sets, const int2* units, int n_units, int parts,
+                             int n_max, int* partial, hipStream_t stream) {
+    if (n_units == 0) return hipSuccess;
+    if (parts < 1 || n_max < 1 || n_max > kMaxPixels) return hipErrorInvalidValue;
+    const int64_t blocks = (int64_t)n_units * parts;
+    if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    const size_t lds = sizeof(int) * (size_t)n_max;
+    hipLaunchKernelGGL(k_plan_count, dim3((unsigned)blocks), dim3(kUnitThreads), lds, stream,
+                       sets, units, parts, n_max, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_plan_scan, dim3((unsigned)n_units), dim3(kUnitThreads), 0, stream, sets,
+                       units, parts, n_max, partial);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_plan_scatter, dim3((unsigned)blocks), dim3(kUnitThreads), lds, stream,
+                       sets, units, parts, n_max, (const int*)partial);
+    return hipGetLastError();
+}
+
 // Locates the tile row of entry e given the tile's row starts rs[0..nr].
 __device__ __forceinline__ int row_of_entry(const int* rs, int nr, int e) {
     int r = 0;
@@ -427,9 +562,18 @@ k_fft_cols(ColArgs a, FftPlan plan, int team, const float2* __restrict__ tw) {
     const int c0 = cb * a.width;
     const int w_here = min(a.width, span - c0);
     const float2* src = a.in + img * a.in_img_stride;
-    const int64_t oimg = a.out_img_offset + img;
+    int64_t oimg = a.out_img_offset + img;
     float2* dst = a.out + img * a.out_img_stride;
     const int* rs = a.row_start ? a.row_start + (oimg % a.n_ch) * (int64_t)(n + 1) : nullptr;
+    const uint32_t* isums = a.isums;
+    if (a.sets) {  // group image oimg of a batched call
+        const SetDesc sd = a.n_sets == 1 ? a.first : a.sets[a.img_set[oimg]];
+        const int li = (int)(oimg - sd.img_base);
+        oimg = li;
+        dst = sd.images + (int64_t)li * n * n;
+        rs = sd.row_start + (int64_t)(li % sd.g.n_ch) * (n + 1);
+        isums = sd.isums;
+    }
 
     // load: element (r, w) of the set; columns c0+w and c0+N/2+w
     const int total = n * wt;
@@ -460,8 +604,8 @@ k_fft_cols(ColArgs a, FftPlan plan, int team, const float2* __restrict__ tw) {
     __syncthreads();
     float scale = 1.0f;
     if (a.sums) scale = (float)(1.0 / a.sums[oimg]);
-    else if (a.isums) scale = (float)(1.0 / (double)a.isums[oimg]);
-    const bool do_scale = a.sums || a.isums;
+    else if (isums) scale = (float)(1.0 / (double)isums[oimg]);
+    const bool do_scale = a.sums || isums;
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const int r = i / wt;
         const int wl = i % wt;
@@ -570,11 +714,10 @@ hipError_t launch_average(const float* images, int64_t image_size, int n_images,
 // row-pass buffer `inter` (image index relative to the chunk).  Empty rows are
 // not written: the column pass reads the plan and treats them as zero.  The
 // number of gridded cells per image goes to isums (integer atomics).
-template <int S, bool kBlockTeam>
+template <int S, bool kBlockTeam, bool kMulti>
 __global__ void __launch_bounds__(kFftThreads)
-k_grid_rows_fft(GatherArgs g, const int* __restrict__ row_start,
-                const uint32_t* __restrict__ entries, FftPlan plan, int team, int rb,
-                int64_t img0, int n_img, float2* __restrict__ inter, uint32_t* __restrict__ isums,
+k_grid_rows_fft(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
+                FftPlan plan, int team, int rb, int64_t gi0, float2* __restrict__ inter,
                 const float2* __restrict__ tw) {
     extern __shared__ float4 grf_lds4[];
     const int n = plan.n;
@@ -595,29 +738,29 @@ k_grid_rows_fft(GatherArgs g, const int* __restrict__ row_start,
     if ((nwg & 7u) == 0) bid = (bid & 7u) * (nwg >> 3) + (bid >> 3);
     const int tile_id = (int)(bid % n_rb);
     const int li = (int)(bid / n_rb);
-    const int64_t img = img0 + li;
-    const int ch = (int)(img % g.n_ch);
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const GatherArgs& g = sd.g;
+    const int img = (int)(gi - sd.img_base);  // image within its set
+    const int ch = img % g.n_ch;
     const int r0 = tile_id * rb;
     const int nr = min(rb, n - r0);
-    const int* rsg = row_start + (int64_t)ch * (n + 1);
-    const uint32_t* ent = entries + (int64_t)ch * 2 * g.n_bl;
+    const int* rsg = sd.row_start + (int64_t)ch * (n + 1);
+    const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
     if ((int)threadIdx.x <= nr) rs[threadIdx.x] = rsg[r0 + threadIdx.x];
     __syncthreads();
     if (rs[nr] == rs[0]) return;  // uniform: no entries in this tile
     for (int i = threadIdx.x; i < nr * n; i += blockDim.x) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    int n_ok = 0;
-    for (int e = rs[0] + threadIdx.x; e < rs[nr]; e += blockDim.x) {
-        int u_idx;
-        float2 v;
-        if (!gather_entry(g, img, ch, ent[e], u_idx, v)) continue;
+    const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[nr], (int)threadIdx.x,
+                                     (int)blockDim.x, [&](int e, int u_idx, float2 v) {
         const int r = row_of_entry(rs, nr, e);
         atomicAdd(&tile[r * n + u_idx].x, v.x);
         atomicAdd(&tile[r * n + u_idx].y, v.y);
-        n_ok++;
-    }
+        return 1;
+    });
     const int tot = block_sum_int(n_ok, red);
-    if (threadIdx.x == 0 && tot) atomicAdd(&isums[img], (uint32_t)tot);
+    if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();
     const int rounds = kBlockTeam ? nr : (nr + teams - 1) / teams;
     for (int k = 0; k < rounds; k++) {
@@ -754,27 +897,28 @@ int grid_rows_tile_rows(int n) {
     return std::max(1, std::min(rb, n));
 }
 
-hipError_t launch_grid_rows_fft(const GatherArgs& g, const int* row_start, const uint32_t* entries,
-                                const FftPlan& plan, int64_t img0, int n_img, float* inter,
-                                uint32_t* isums, const float* tw, hipStream_t stream) {
+hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                                int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
+                                float* inter, const float* tw, hipStream_t stream) {
     if (n_img == 0) return hipSuccess;
     const int n = plan.n;
     const int team = fft_team_for(n);
     const bool block_team = team == kFftThreads;
     const int teams = block_team ? 1 : kFftThreads / team;
-    const int rb = grid_rows_tile_rows(n);
+    // rows per tile: as many as fit, but at least ~2048 workgroups in the
+    // launch, so that a few images still spread over the whole chip
+    const int64_t want = (int64_t)n * n_img / 2048;
+    const int rb = (int)std::max<int64_t>(1, std::min<int64_t>(grid_rows_tile_rows(n), want));
     const int n_rb = (n + rb - 1) / rb;
     const size_t lds = sizeof(float2) * ((size_t)rb * n + (size_t)teams * n);
     if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)((int64_t)n_rb * n_img));
-    if (block_team)
-        hipLaunchKernelGGL((k_grid_rows_fft<1, true>), grid, dim3(kFftThreads), lds, stream, g,
-                           row_start, entries, plan, team, rb, img0, n_img, (float2*)inter, isums,
-                           (const float2*)tw);
-    else
-        hipLaunchKernelGGL((k_grid_rows_fft<1, false>), grid, dim3(kFftThreads), lds, stream, g,
-                           row_start, entries, plan, team, rb, img0, n_img, (float2*)inter, isums,
-                           (const float2*)tw);
+    const bool multi = n_sets > 1;
+    auto k = block_team
+                 ? (multi ? k_grid_rows_fft<1, true, true> : k_grid_rows_fft<1, true, false>)
+                 : (multi ? k_grid_rows_fft<1, false, true> : k_grid_rows_fft<1, false, false>);
+    hipLaunchKernelGGL(k, grid, dim3(kFftThreads), lds, stream, first, sets, img_set, plan, team,
+                       rb, gi0, (float2*)inter, (const float2*)tw);
     return hipGetLastError();
 }
 
@@ -807,8 +951,10 @@ hipError_t set_kernel_attributes() {
     set((const void*)k_fft_cols<1, false>);
     set((const void*)k_fft_cols<-1, true>);
     set((const void*)k_fft_cols<-1, false>);
-    set((const void*)k_grid_rows_fft<1, true>);
-    set((const void*)k_grid_rows_fft<1, false>);
+    set((const void*)k_grid_rows_fft<1, true, false>);
+    set((const void*)k_grid_rows_fft<1, false, false>);
+    set((const void*)k_grid_rows_fft<1, true, true>);
+    set((const void*)k_grid_rows_fft<1, false, true>);
     set((const void*)k_grid_tiles);
     set((const void*)k_plan_rows);
     return e;

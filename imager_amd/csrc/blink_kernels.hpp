@@ -34,6 +34,38 @@ struct GatherArgs {
     int n_bl, n_ch;
 };
 
+// One image set of a batched call as the kernels see it (a table of these
+// lives in device memory for the duration of the call or captured graph).
+// Images of all sets that share a grid side and path form a group; a launch
+// covers a contiguous range of group image indices gi, and a set owns
+// gi = img_base .. img_base + n_img - 1.
+struct SetDesc {
+    GatherArgs g;             // visibilities, corrections, pol, n_bl, n_ch
+    const float* u_m;
+    const float* v_m;
+    const int* flags;
+    double du, dv, min_uv;
+    int n_pixels;
+    int n_img;                // n_ch * n_t
+    int64_t img_base;         // first group image index of the set
+    int* row_start;           // plan: [n_ch][N + 1]
+    uint32_t* entries;        // plan: [n_ch][2 n_bl]
+    uint32_t* isums;          // [n_img] gridded visibilities per image
+    float2* images;           // [n_img][N][N]
+};
+
+// The set of group image gi, by value.  Kernels are built twice: for a
+// group of one set (kMulti false) the descriptor is `first`, a kernel
+// argument, so its fields arrive with the other arguments and add no
+// dependent load at workgroup start; otherwise it is the table entry named
+// by the image-to-set table.  Uniform per workgroup.
+template <bool kMulti>
+__device__ __forceinline__ SetDesc set_of(const SetDesc& first, const SetDesc* __restrict__ sets,
+                                          const int* __restrict__ img_set, int64_t gi) {
+    if constexpr (!kMulti) return first;
+    else return sets[img_set[gi]];
+}
+
 struct ColArgs {
     const float2* in;          // [img][N][N], images of this launch
     float2* out;               // [img][N][N], images of this launch
@@ -47,6 +79,13 @@ struct ColArgs {
     const uint32_t* isums;     // integer sums alternative, or null
     const int* row_start;      // plan rows: skip empty rows on load, or null
     int n_ch;
+    // batched groups: when sets is set, image out_img_offset + i of the
+    // launch is a group image; its output, plan and count sum come from its
+    // set (out, isums, row_start, n_ch above are then unused)
+    const SetDesc* sets;
+    const int* img_set;
+    int n_sets;
+    SetDesc first;             // sets[0], by value
 };
 
 bool make_fft_plan(int n, FftPlan& plan);
@@ -88,9 +127,14 @@ hipError_t launch_shift_norm(float* images, int nx, int ny, int n_images, const 
                              int n_sums, hipStream_t stream);
 hipError_t launch_average(const float* images, int64_t image_size, int n_images, int real_only,
                           float* avg, hipStream_t stream);
-hipError_t launch_grid_rows_fft(const GatherArgs& g, const int* row_start, const uint32_t* entries,
-                                const FftPlan& plan, int64_t img0, int n_img, float* inter,
-                                uint32_t* isums, const float* tw, hipStream_t stream);
+hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                                int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
+                                float* inter, const float* tw, hipStream_t stream);
+// Gridding plans of a batched call: one unit per (set, channel), units[u] =
+// (set, channel); `parts` workgroups per unit split its baselines.  partial
+// is [n_units][parts][n_max] ints of scratch.
+hipError_t launch_plan_units(const SetDesc* sets, const int2* units, int n_units, int parts,
+                             int n_max, int* partial, hipStream_t stream);
 hipError_t launch_isums_to_double(const uint32_t* isums, double* sums, int n, hipStream_t stream);
 
 // Split-precision DFT-as-GEMM (blink_dft_mfma.hip): out = (D' applied to the
@@ -110,11 +154,11 @@ int64_t batch_inter_bytes(int n);
 int batch_row_variants(int n, int* tiles, int max_tiles);
 int batch_col_variants(int n, int* tiles, int max_tiles);
 hipError_t set_batch_kernel_attributes();
-hipError_t launch_grid_rows_reg(const GatherArgs& g, const int* row_start, const uint32_t* entries,
-                                int n, int tile, int64_t img0, int n_img, float* inter,
-                                uint32_t* isums, const float* tw, hipStream_t stream);
-hipError_t launch_cols_reg(const float* inter, float* images, int n, int tile, int64_t img0,
-                           int n_img, int n_ch, const int* row_start, const uint32_t* isums,
-                           const float* tw, hipStream_t stream);
+hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                                int n_sets, int n, int tile, int64_t gi0, int n_img, float* inter,
+                                const float* tw, hipStream_t stream);
+hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                           int n_sets, int n, int tile, int64_t gi0, int n_img,
+                           const float* inter, const float* tw, hipStream_t stream);
 
 }  // namespace blink

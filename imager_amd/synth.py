@@ -134,6 +134,25 @@ def sky_visibilities(u, v, freq, l, m, s):
     return out
 
 
+def diffuse_visibilities(rng, u, v, freq, amplitude: float, index: float = 2.0,
+                         uv_ref: float = 10.0):
+    """XX visibilities [n_ch][n_bl] complex128 of a diffuse Gaussian random
+    field with angular power spectrum P(q) ~ (q / uv_ref)^-index, q = |uv| in
+    wavelengths: each (baseline, channel) draws a complex normal of variance
+    amplitude^2 P(q), Hermitian by construction because only u, v of the
+    baseline itself are used (the conjugate cell gets the conjugate value)."""
+    u64 = u.astype(np.float64)
+    v64 = v.astype(np.float64)
+    out = np.empty((len(freq), len(u)), np.complex128)
+    for k, f in enumerate(freq):
+        lam = C_LIGHT / f
+        q = np.hypot(u64, v64) / lam
+        p = (np.maximum(q, 0.5) / uv_ref) ** (-index)
+        z = rng.standard_normal((len(u), 2))
+        out[k] = amplitude * np.sqrt(p / 2) * (z[:, 0] + 1j * z[:, 1])
+    return out
+
+
 def make_vis(rng, model_xx, n_t: int, sigma: float) -> np.ndarray:
     n_ch, n_bl = model_xx.shape
     vis = np.empty((n_t, n_ch, n_bl, 4), np.complex64)
@@ -148,7 +167,7 @@ def make_vis(rng, model_xx, n_t: int, sigma: float) -> np.ndarray:
 def observation(name: str, n_ant: int, layout: str, n_ch: int, n_t: int, freq, n_pixels: int,
                 n_src: int, sigma: float, seed: int, flagged_antennas=(), height_sigma_m=0.5,
                 cable_range=(50.0, 300.0), diameter_m=35.0, gauss_sigma_m=300.0,
-                clip_m=1500.0, bl_flag_fraction=0.0) -> Observation:
+                clip_m=1500.0, bl_flag_fraction=0.0, diffuse=0.0) -> Observation:
     rng = np.random.Generator(np.random.PCG64(seed))
     if layout == "disk":
         x, y = disk_layout(rng, n_ant, diameter_m)
@@ -165,6 +184,8 @@ def observation(name: str, n_ant: int, layout: str, n_ch: int, n_t: int, freq, n
     assert len(freq) == n_ch
     l, m, s = point_sources(rng, n_src)
     model = sky_visibilities(u, v, freq, l, m, s)
+    if diffuse > 0:
+        model += diffuse_visibilities(rng, u, v, freq, diffuse)
     vis = make_vis(rng, model, n_t, sigma)
     u_max, v_max, du, dv = grid_scalars(u, v, flags, n_pixels)
     return Observation(name=name, n_ant=n_ant, x=x, y=y, z=z, u=u, v=v, w=w, flags=flags,
@@ -179,12 +200,14 @@ def eda2_station(n_ch=1, n_t=1, n_pixels=180, seed=1001, freq=None, **kw) -> Obs
     return observation("eda2_256ant", 256, "disk", n_ch, n_t, f, n_pixels, 5, 0.1, seed, **kw)
 
 
-def mwa_array(n_ch=64, n_t=10, n_pixels=1024, seed=1002, freq=None, n_ant=128, **kw) -> Observation:
+def mwa_array(n_ch=64, n_t=10, n_pixels=1024, seed=1002, freq=None, n_ant=128, n_src=20,
+              **kw) -> Observation:
     """128 tiles, Gaussian layout (sigma 300 m, clipped at 1.5 km), channels
     170-200 MHz evenly spaced, 20 sources, sigma 0.5."""
     if freq is None:
         freq = (170.0 + 30.0 * np.arange(n_ch) / max(n_ch - 1, 1)) * 1e6
-    return observation("mwa_128t", n_ant, "gauss", n_ch, n_t, freq, n_pixels, 20, 0.5, seed, **kw)
+    return observation("mwa_128t", n_ant, "gauss", n_ch, n_t, freq, n_pixels, n_src, 0.5, seed,
+                       **kw)
 
 
 def mwa_fine_channels(n_ch=768, first_mhz=169.60, step_mhz=0.04):
@@ -200,3 +223,40 @@ def small(n_ant=16, n_ch=3, n_t=2, n_pixels=64, seed=7, layout="disk", diameter_
         freq = np.linspace(150e6, 200e6, n_ch)
     return observation(f"small_{n_ant}ant", n_ant, layout, n_ch, n_t, freq, n_pixels, n_src, sigma,
                        seed, diameter_m=diameter_m, **kw)
+
+
+def mixed_batch(n_images=512, seed=1003, sizes=(128, 180, 256, 512, 1024, 2048),
+                max_flag_fraction=0.3):
+    """Uneven batch: every image from either the 128-tile Gaussian array or the
+    256-antenna 35 m station, a grid side drawn from `sizes`, one fine channel
+    at random, and its own random baseline flags (0 .. max_flag_fraction of the
+    baselines on top of the autos), so the gridded visibility counts differ
+    from image to image.  Returns one single-image Observation per image."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    arrays = []
+    for n_ant, layout, n_src, sigma in ((128, "gauss", 20, 0.5), (256, "disk", 5, 0.1)):
+        if layout == "disk":
+            x, y = disk_layout(rng, n_ant, 35.0)
+        else:
+            x, y = gaussian_layout(rng, n_ant, 300.0, 1500.0)
+        z = rng.normal(0, 0.5, n_ant)
+        u, v, w = zenith_uvw(x, y, z)
+        cable = rng.uniform(50.0, 300.0, n_ant)
+        src = point_sources(rng, n_src)
+        arrays.append((n_ant, layout, x, y, z, u, v, w, cable, src, sigma))
+    chans = mwa_fine_channels(768)
+    out = []
+    for i in range(n_images):
+        n_ant, layout, x, y, z, u, v, w, cable, (l, m, s), sigma = arrays[int(rng.integers(2))]
+        n = int(sizes[int(rng.integers(len(sizes)))])
+        frac = float(rng.uniform(0.0, max_flag_fraction))
+        flags = (baseline_flags(n_ant) | (rng.random(len(u)) < frac)).astype(np.int32)
+        freq = np.array([chans[int(rng.integers(len(chans)))]])
+        model = sky_visibilities(u, v, freq, l, m, s)
+        vis = make_vis(rng, model, 1, sigma)
+        u_max, v_max, du, dv = grid_scalars(u, v, flags, n)
+        out.append(Observation(name=f"mixed_{i}", n_ant=n_ant, x=x, y=y, z=z, u=u, v=v, w=w,
+                               flags=flags, cable=cable, freq=freq, vis=vis, n_pixels=n,
+                               u_max=u_max, v_max=v_max, delta_u=du, delta_v=dv,
+                               meta=dict(layout=layout, flag_fraction=frac, seed=seed)))
+    return out

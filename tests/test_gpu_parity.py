@@ -402,6 +402,47 @@ def test_image_batch_two_sets(ctx):
         assert oracle.peak_normalised_error(_host(im), ref)[0] < IMAGE_TOL
 
 
+def test_image_batch_grouped_sets(ctx):
+    """Several sets per grid side in one call (one launch pair per side):
+    different arrays, channel and timestep counts, flags and polarisations,
+    register (256, 128) and generic (180) sides, plus a DFT set."""
+    import torch
+    from imager_amd import capi
+    specs = [(256, 40, 2, 1, "XX", 0), (256, 64, 1, 3, "I", 0), (180, 48, 3, 1, "XX", 0),
+             (256, 24, 2, 2, "YY", 0), (128, 32, 1, 1, "V", 0), (180, 40, 1, 2, "XX", 0),
+             (64, 32, 2, 1, "XX", 1)]
+    obs, keep, sets = [], [], []
+    for k, (n, n_ant, n_ch, n_t, pol, dft) in enumerate(specs):
+        o = synth.small(n_ant=n_ant, n_ch=n_ch, n_t=n_t, n_pixels=n, seed=500 + k,
+                        height_sigma_m=1.0, flagged_antennas=(k % 5, 7), bl_flag_fraction=0.1)
+        d = _obs_dev(o)
+        imgs = torch.empty((n_t, n_ch, n, n), dtype=torch.complex64, device="cuda")
+        sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
+        st = capi.make_set(d["vis"], n_ant, n_ch, n_t, d["u"], d["v"], d["w"], d["flags"],
+                           d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, n, pol, imgs,
+                           sums, transform=capi.TRANSFORM_DFT_BF16X3 if dft else
+                           capi.TRANSFORM_FFT)
+        obs.append((o, pol))
+        keep.append((d, imgs, sums))
+        sets.append(st)
+    for rep in range(2):   # capture, then replay
+        for _, imgs, _ in keep:
+            imgs.fill_(float("nan"))
+        ctx.image_batch(sets)
+        for (o, pol), (_, imgs, sums) in zip(obs, keep):
+            ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True,
+                                    True, o.delta_u, o.delta_v, o.min_uv, o.n_pixels, pol)
+            err = oracle.peak_normalised_error(_host(imgs), ref)[0]
+            _say(f"  grouped N={o.n_pixels} {o.n_ant} ant {o.n_ch}x{o.n_t} {pol}: {err:.2e}")
+            assert err < IMAGE_TOL, (o.n_pixels, pol, rep)
+            _, c_ref, _ = oracle.gridding(oracle.cable_corrections(
+                oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq), o.n_ant, o.cable,
+                o.freq), o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v, o.min_uv,
+                o.n_pixels, pol)
+            assert np.array_equal(_host(sums),
+                                  c_ref.reshape(o.n_images, -1).sum(1).astype(np.float64))
+
+
 # ------------------------------------------------------------ error paths
 
 def test_error_paths(ctx):
