@@ -119,7 +119,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         float2 v[F::V];
         F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
         team_sync<false>();
-        F::template run<1>(v, row, lane, twl);
+        F::template run<1>(v, row, lane, TwTable{twl});
         F::store(v, lane, [&](int i, float2 x) { row[pad_index(i)] = x; });
     }
     __syncthreads();
@@ -193,7 +193,7 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
                            __int_as_float(__float_as_int(im) & m));
     });
     float2* row = tile + team * TS;
-    F::template run<1>(v, row, lane, twl);
+    F::template run<1>(v, row, lane, TwTable{twl});
     const float s = (float)(1.0 / (double)count);
 #pragma unroll
     for (int k = 0; k < F::V; k++) v[k] = cscale(v[k], s);
@@ -211,6 +211,194 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
         const float2 z = tile[t * TS + pad_index(r)];
         *(float4*)&out[(int64_t)orow * N + oc0 + 2 * t] = make_float4(z.x, 0.f, z.y, 0.f);
     }
+}
+
+// ------------------------------------------------------------- N = 4096
+//
+// At N = 4096 one row needs 4 wavefronts (256 lanes x 16 values) and 34 KiB
+// of LDS, so a workgroup holds one to four rows or column pairs.  Too few
+// rows per tile for 64-byte segments of a transposed store, so the row-pass
+// buffer is row-major here: inter[img][v][x], v = 0 .. N/2.  The row kernel
+// stores each transformed row with unit-stride lanes; the column kernel stages
+// its CB columns of rows 0 .. N/2 through LDS (16-byte loads, two columns per
+// thread), forming z = c_x + i c_{x+1} for rows v and N - v on the way.
+// Twiddles come from two 64-entry tables (w(m) = lo[m % 64] hi[m / 64]) kept
+// in LDS, instead of a 32 KiB table.
+
+constexpr int kWideLanes = 256;
+
+__host__ __device__ constexpr int wide_tile_stride(int n, int rows) { return tile_stride(n, rows); }
+
+template <int N, int RB>
+constexpr size_t wide_rows_lds_bytes() {
+    return sizeof(float2) * ((size_t)RB * padded_len(N) + 128);
+}
+template <int N, int CB>
+constexpr size_t wide_cols_lds_bytes() {
+    return sizeof(float2) * ((size_t)(CB / 2) * wide_tile_stride(N, CB / 2) + 128) + N;
+}
+
+template <int N, int RB, bool kMulti>
+__global__ void __launch_bounds__(RB * kWideLanes)
+k_grid_rows_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
+                 int64_t gi0, float2* __restrict__ inter, const float2* __restrict__ tws) {
+    constexpr int L = kWideLanes, TS = padded_len(N), NT = RB * L;
+    using F = RegFft<N, L>;
+    extern __shared__ float4 rows_wide_lds4[];
+    float2* tile = (float2*)rows_wide_lds4;  // [RB][TS]
+    float2* twl = tile + RB * TS;             // lo[64], hi[64]
+    __shared__ int rs[RB + 1];
+    __shared__ int red[NT / 64 + 1];
+
+    constexpr int H = N / 2;
+    constexpr int n_rb = (H + 1 + RB - 1) / RB;
+    const unsigned nwg = gridDim.x;
+    unsigned bid = blockIdx.x;
+    if ((nwg & 7u) == 0) bid = (bid & 7u) * (nwg >> 3) + (bid >> 3);
+    const int tile_id = (int)(bid % n_rb);
+    const int li = (int)(bid / n_rb);
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const GatherArgs& g = sd.g;
+    const int img = (int)(gi - sd.img_base);
+    const int ch = img % g.n_ch;
+    const int r0 = tile_id * RB;
+    const int nr = min(RB, H + 1 - r0);
+    const int tid = threadIdx.x;
+    const int* rsg = sd.row_start + (int64_t)ch * (N + 1);
+    if (tid <= nr) rs[tid] = rsg[r0 + tid];
+    if (tid > nr && tid <= RB) rs[tid] = rsg[r0 + nr];
+    __syncthreads();
+    if (rs[RB] == rs[0]) return;  // uniform: nothing lands in this tile
+
+    if (tid < 128) twl[tid] = tws[tid];
+    for (int i = tid; i < RB * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
+    const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[RB], tid, NT,
+                                     [&](int e, int u_idx, float2 v) {
+        int r = 0;
+        while (r + 1 < RB && rs[r + 1] <= e) r++;
+        float2* c = &tile[r * TS + pad_index(u_idx)];
+        atomicAdd(&c->x, v.x);
+        atomicAdd(&c->y, v.y);
+        const int row = r0 + r;
+        return (row == 0 || row == H) ? 1 : 2;
+    });
+    const int tot = block_sum_int(n_ok, red);
+    if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
+    __syncthreads();
+
+    // every team transforms its row (the exchanges synchronise the workgroup);
+    // rows with entries are stored
+    const int team = tid / L, lane = tid % L;
+    float2* row = tile + team * TS;
+    float2 v[F::V];
+    F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
+    __syncthreads();
+    F::template run<1>(v, row, lane, TwSplit64{twl, twl + 64});
+    if (team < nr && rs[team + 1] != rs[team]) {
+        float2* dst = inter + ((int64_t)li * (H + 1) + r0 + team) * N;
+        F::store(v, lane, [&](int i, float2 x) { dst[i] = x; });
+    }
+}
+
+template <int N, int CB, bool kMulti>
+__global__ void __launch_bounds__(CB / 2 * kWideLanes)
+k_cols_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
+            int64_t gi0, const float2* __restrict__ inter, const float2* __restrict__ tws) {
+    constexpr int L = kWideLanes, TEAMS = CB / 2, NT = TEAMS * L;
+    constexpr int TS = wide_tile_stride(N, TEAMS);
+    constexpr int H = N / 2;
+    using F = RegFft<N, L>;
+    extern __shared__ float4 cols_wide_lds4[];
+    float2* tile = (float2*)cols_wide_lds4;   // [TEAMS][TS]
+    float2* twl = tile + TEAMS * TS;           // lo[64], hi[64]
+    unsigned char* nonempty = (unsigned char*)(twl + 128);  // [N / 2 + 1]
+
+    constexpr int n_cb = N / CB;
+    // XCD-contiguous order: neighbouring column blocks share the cache lines
+    // of every output row
+    const unsigned nwg = gridDim.x;
+    unsigned bid = blockIdx.x;
+    if ((nwg & 7u) == 0) bid = (bid & 7u) * (nwg >> 3) + (bid >> 3);
+    const int cb = (int)(bid % n_cb);
+    const int li = (int)(bid / n_cb);
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const int img = (int)(gi - sd.img_base);
+    const int tid = threadIdx.x;
+    const uint32_t count = sd.isums[img];
+    const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (N + 1);
+    for (int i = tid; i <= H; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
+    if (tid < 128) twl[tid] = tws[tid];
+    __syncthreads();
+
+    // stage: z_t[v] and z_t[N - v] from inter[li][v][c0 + 2t .. c0 + 2t + 1]
+    const int c0 = cb * CB;
+    const float2* src = inter + (int64_t)li * (H + 1) * N + c0;
+    for (int i = tid; i < (H + 1) * TEAMS; i += NT) {
+        const int vr = i / TEAMS, t = i % TEAMS;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (nonempty[vr]) p = *(const float4*)&src[(int64_t)vr * N + 2 * t];
+        // a = c_x[v] = (p.x, p.y), b = c_{x+1}[v] = (p.z, p.w); row N - v holds conjugates
+        float2* zr = tile + t * TS;
+        zr[pad_index(vr)] = make_float2(p.x - p.w, p.y + p.z);
+        if (vr > 0 && vr < H) zr[pad_index(N - vr)] = make_float2(p.x + p.w, p.z - p.y);
+    }
+    __syncthreads();
+
+    const int team = tid / L, lane = tid % L;
+    float2* row = tile + team * TS;
+    float2 v[F::V];
+    F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
+    __syncthreads();
+    F::template run<1>(v, row, lane, TwSplit64{twl, twl + 64});
+    const float s = (float)(1.0 / (double)count);
+#pragma unroll
+    for (int k = 0; k < F::V; k++) v[k] = cscale(v[k], s);
+    __syncthreads();
+    F::store(v, lane, [&](int i, float2 x) { row[pad_index(i)] = x; });
+    __syncthreads();
+    // shifted store, as in k_cols_reg: (Re Z_t[r], 0), (Im Z_t[r], 0)
+    float2* out = sd.images + (int64_t)img * N * N;
+    const int oc0 = c0 < H ? c0 + H : c0 - H;
+    for (int i = tid; i < TEAMS * N; i += NT) {
+        const int r = i / TEAMS, t = i % TEAMS;
+        const int orow = r < H ? r + H : r - H;
+        const float2 z = tile[t * TS + pad_index(r)];
+        *(float4*)&out[(int64_t)orow * N + oc0 + 2 * t] = make_float4(z.x, 0.f, z.y, 0.f);
+    }
+}
+
+template <int N, int RB>
+static hipError_t launch_rows_wide_t(const SetDesc& first, const SetDesc* sets,
+                                     const int* img_set, int n_sets, int64_t gi0, int n_img,
+                                     float* inter, const float* tws, hipStream_t stream) {
+    constexpr int NT = RB * kWideLanes;
+    constexpr size_t lds = wide_rows_lds_bytes<N, RB>();
+    if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
+    const int64_t blocks = (int64_t)((N / 2 + 1 + RB - 1) / RB) * n_img;
+    if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    auto k = n_sets == 1 ? k_grid_rows_wide<N, RB, false> : k_grid_rows_wide<N, RB, true>;
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(NT), lds, stream, first, sets, img_set,
+                       gi0, (float2*)inter, (const float2*)tws);
+    return hipGetLastError();
+}
+
+template <int N, int CB>
+static hipError_t launch_cols_wide_t(const SetDesc& first, const SetDesc* sets,
+                                     const int* img_set, int n_sets, int64_t gi0, int n_img,
+                                     const float* inter, const float* tws, hipStream_t stream) {
+    constexpr int NT = CB / 2 * kWideLanes;
+    constexpr size_t lds = wide_cols_lds_bytes<N, CB>();
+    if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
+    const int64_t blocks = (int64_t)(N / CB) * n_img;
+    if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    auto k = n_sets == 1 ? k_cols_wide<N, CB, false> : k_cols_wide<N, CB, true>;
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(NT), lds, stream, first, sets, img_set,
+                       gi0, (const float2*)inter, (const float2*)tws);
+    return hipGetLastError();
 }
 
 template <int N, int RB>
@@ -259,6 +447,8 @@ static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
     X(512, 8) X(512, 16) \
     X(1024, 4) X(1024, 8) X(1024, 16) \
     X(2048, 4) X(2048, 8)
+#define BLINK_ROW_WIDE_VARIANTS(X) X(4096, 1) X(4096, 2) X(4096, 4)
+#define BLINK_COL_WIDE_VARIANTS(X) X(4096, 2) X(4096, 4) X(4096, 8)
 #define BLINK_COL_VARIANTS(X) \
     X(128, 16) X(128, 32) X(128, 64) \
     X(256, 16) X(256, 32) X(256, 64) \
@@ -268,20 +458,24 @@ static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
 
 bool batch_reg_supported(int n) {
     switch (n) {
-        case 128: case 256: case 512: case 1024: case 2048:
+        case 128: case 256: case 512: case 1024: case 2048: case 4096:
             return true;
         default:
             return false;
     }
 }
 
-int64_t batch_inter_bytes(int n) { return (int64_t)sizeof(float2) * n * half_stride(n); }
+int64_t batch_inter_bytes(int n) {
+    if (n >= 4096) return (int64_t)sizeof(float2) * (n / 2 + 1) * n;  // row-major
+    return (int64_t)sizeof(float2) * n * half_stride(n);                // transposed
+}
 
 int batch_row_variants(int n, int* tiles, int max_tiles) {
     int k = 0;
 #define X(NN, T) \
     if (n == NN && k < max_tiles) tiles[k++] = T;
     BLINK_ROW_VARIANTS(X)
+    BLINK_ROW_WIDE_VARIANTS(X)
 #undef X
     return k;
 }
@@ -291,6 +485,7 @@ int batch_col_variants(int n, int* tiles, int max_tiles) {
 #define X(NN, T) \
     if (n == NN && k < max_tiles) tiles[k++] = T;
     BLINK_COL_VARIANTS(X)
+    BLINK_COL_WIDE_VARIANTS(X)
 #undef X
     return k;
 }
@@ -305,6 +500,12 @@ hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const
                                         stream);
     BLINK_ROW_VARIANTS(X)
 #undef X
+#define X(NN, T) \
+    if (n == NN && tile == T) \
+        return launch_rows_wide_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, \
+                                         tw + 2 * NN, stream);
+    BLINK_ROW_WIDE_VARIANTS(X)
+#undef X
     return hipErrorNotSupported;
 }
 
@@ -317,6 +518,12 @@ hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int*
         return launch_cols_reg_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, tw, \
                                         stream);
     BLINK_COL_VARIANTS(X)
+#undef X
+#define X(NN, T) \
+    if (n == NN && tile == T) \
+        return launch_cols_wide_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, \
+                                         tw + 2 * NN, stream);
+    BLINK_COL_WIDE_VARIANTS(X)
 #undef X
     return hipErrorNotSupported;
 }
@@ -337,6 +544,16 @@ hipError_t set_batch_kernel_attributes() {
     set((const void*)k_cols_reg<NN, T, false>);   \
     set((const void*)k_cols_reg<NN, T, true>);
     BLINK_COL_VARIANTS(X)
+#undef X
+#define X(NN, T)                                        \
+    set((const void*)k_grid_rows_wide<NN, T, false>);   \
+    set((const void*)k_grid_rows_wide<NN, T, true>);
+    BLINK_ROW_WIDE_VARIANTS(X)
+#undef X
+#define X(NN, T)                                   \
+    set((const void*)k_cols_wide<NN, T, false>);   \
+    set((const void*)k_cols_wide<NN, T, true>);
+    BLINK_COL_WIDE_VARIANTS(X)
 #undef X
     return e;
 }

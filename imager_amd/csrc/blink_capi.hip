@@ -183,18 +183,25 @@ int scratch(blink_ctx* ctx, const char* name, size_t bytes, void** out) {
     return BLINK_OK;
 }
 
-// tw[m] = exp(+2 pi i m / N), computed in double, rounded to FP32.
+// tw[m] = exp(+2 pi i m / N), m < N, computed in double, rounded to FP32;
+// followed by lo[j] = tw[j] and hi[j] = exp(+2 pi i 64 j / N), j < 64 (the
+// two-level tables of the N = 4096 kernels).
 int twiddles(blink_ctx* ctx, int n, const float** out) {
     auto it = ctx->twiddles.find(n);
     if (it != ctx->twiddles.end()) {
         *out = it->second;
         return BLINK_OK;
     }
-    std::vector<float> h(2 * (size_t)n);
-    for (int m = 0; m < n; m++) {
-        const double a = 2.0 * M_PI * (double)m / (double)n;
-        h[2 * m] = (float)std::cos(a);
-        h[2 * m + 1] = (float)std::sin(a);
+    std::vector<float> h(2 * ((size_t)n + 128));
+    auto put = [&](size_t at, double turns) {
+        const double a = 2.0 * M_PI * turns;
+        h[2 * at] = (float)std::cos(a);
+        h[2 * at + 1] = (float)std::sin(a);
+    };
+    for (int m = 0; m < n; m++) put(m, (double)m / (double)n);
+    for (int j = 0; j < 64; j++) {
+        put((size_t)n + j, (double)j / (double)n);
+        put((size_t)n + 64 + j, (double)(64 * j % n) / (double)n);
     }
     float* d = nullptr;
     HIP_TRY(hipMalloc(&d, sizeof(float) * h.size()));

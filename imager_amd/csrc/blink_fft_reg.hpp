@@ -1,7 +1,7 @@
 // Register-resident Stockham FFT for power-of-two rows (gfx950).
 //
-// A team of L lanes (L <= 64, inside one wavefront) transforms one row of N
-// complex values; each lane keeps V = N / L values in VGPRs for the whole
+// A team of L lanes (inside one wavefront, or several wavefronts with
+// workgroup barriers) transforms one row of N complex values; each lane keeps V = N / L values in VGPRs for the whole
 // transform.  Pass p (radix R, span NS) works on butterflies
 // j = lane + L*b, b < N/(R*L): inputs x[j + r*N/R], twiddle w_N^(r*(j mod NS)
 // * N/(NS*R)), radix-R DFT, outputs y[(j - j mod NS)*R + j mod NS + r*NS]
@@ -56,7 +56,23 @@ __host__ __device__ constexpr int reg_span(RegRadices rr, int p) {
     return s;
 }
 
-template <int N, int L>
+// Twiddle sources for RegFft::run: w(m) = exp(+2 pi i m / N), 0 <= m < N.
+struct TwTable {  // a full table of N entries (LDS or global)
+    const float2* t;
+    __device__ __forceinline__ float2 operator()(int m) const { return t[m]; }
+};
+struct TwSplit64 {  // two 64-entry tables: w(m) = lo[m % 64] * hi[m / 64], N <= 4096
+    const float2* lo;
+    const float2* hi;
+    __device__ __forceinline__ float2 operator()(int m) const {
+        return cmul(lo[m & 63], hi[m >> 6]);
+    }
+};
+
+// kBlock: the team spans several wavefronts (L > 64), so the LDS exchanges
+// synchronise the whole workgroup, and every team of the workgroup must run
+// the same passes.
+template <int N, int L, bool kBlock = (L > 64)>
 struct RegFft {
     static constexpr int V = N / L;
     static constexpr RegRadices RAD = reg_radices(N, L);
@@ -84,9 +100,8 @@ struct RegFft {
             for (int r = 0; r < R; r++) st(lane + L * b + r * NB, v[b * R + r]);
     }
 
-    template <int S, int P>
-    __device__ __forceinline__ static void pass(float2 (&v)[V], int lane,
-                                                const float2* __restrict__ tw) {
+    template <int S, int P, class TW>
+    __device__ __forceinline__ static void pass(float2 (&v)[V], int lane, const TW& tw) {
         constexpr int R = RAD.r[P];
         constexpr int NS = reg_span(RAD, P);
         constexpr int NB = N / R, B = NB / L;
@@ -98,7 +113,7 @@ struct RegFft {
                 constexpr int step = N / (NS * R);
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    float2 w = tw[r * k * step];
+                    float2 w = tw(r * k * step);
                     if (S < 0) w.y = -w.y;
                     v[b * R + r] = cmul(v[b * R + r], w);
                 }
@@ -108,8 +123,9 @@ struct RegFft {
     }
 
     // Stockham output order of pass P -> input order of pass P+1, through the
-    // team's LDS row (padded).  The team is one wavefront or part of one, whose
-    // LDS operations execute in order; the fences stop compiler reordering.
+    // team's LDS row (padded).  A team inside one wavefront executes its LDS
+    // operations in order (the fences stop compiler reordering); a wider team
+    // synchronises the workgroup.
     template <int P>
     __device__ __forceinline__ static void exchange(float2 (&v)[V], float2* row, int lane) {
         constexpr int R1 = RAD.r[P];
@@ -125,23 +141,24 @@ struct RegFft {
 #pragma unroll
             for (int r = 0; r < R1; r++) row[pad_index(base + r * NS1)] = v[b * R1 + r];
         }
-        team_sync<false>();
+        team_sync<kBlock>();
 #pragma unroll
         for (int b = 0; b < B2; b++)
 #pragma unroll
             for (int r = 0; r < R2; r++) v[b * R2 + r] = row[pad_index(lane + L * b + r * NB2)];
-        team_sync<false>();
+        team_sync<kBlock>();
     }
 
-    template <int S, int P = 0>
+    template <int S, int P = 0, class TW>
     __device__ __forceinline__ static void run(float2 (&v)[V], float2* row, int lane,
-                                               const float2* __restrict__ tw) {
+                                               const TW& tw) {
         pass<S, P>(v, lane, tw);
         if constexpr (P + 1 < RAD.n_pass) {
             exchange<P>(v, row, lane);
             run<S, P + 1>(v, row, lane, tw);
         }
     }
+
 };
 
 }  // namespace blink

@@ -342,7 +342,7 @@ def test_image_batch_matches_reference_pipeline(ctx, n, n_ant, pol):
     assert np.array_equal(_host(sums), c_ref.reshape(o.n_images, -1).sum(1).astype(np.float64))
 
 
-@pytest.mark.parametrize("n", [128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("n", [128, 256, 512, 1024, 2048, 4096])
 def test_image_batch_every_tile_variant(ctx, n):
     """Every built row-tile and column-block variant (pinned through
     blink_set_batch_tiles; unbuilt sizes are refused) against the oracle."""
@@ -351,12 +351,13 @@ def test_image_batch_every_tile_variant(ctx, n):
     ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
                             o.delta_u, o.delta_v, o.min_uv, n)
     d, imgs, sums, st = _batch(ctx, o)
-    cands = (2, 4, 8, 16, 32, 64, 128)
+    cands = (1, 2, 4, 8, 16, 32, 64, 128)
     built = {"row": [], "col": []}
+    r_def, c_def = (abs(t) for t in ctx.batch_tiles(n))
     try:
         for kind in ("row", "col"):
             for t in cands:
-                rt, ct = (t, 16 if n < 2048 else 8) if kind == "row" else (16 if n < 2048 else 8, t)
+                rt, ct = (t, c_def) if kind == "row" else (r_def, t)
                 try:
                     ctx.set_batch_tiles(n, rt, ct)
                 except capi.BlinkInvalidArgument:
