@@ -89,18 +89,33 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     const int nr = min(RB, H + 1 - r0);  // rows of this tile that are <= N/2
     const int tid = threadIdx.x;
     const int* rsg = sd.row_start + (int64_t)ch * (N + 1);
-    if (tid <= nr) rs[tid] = rsg[r0 + tid];
-    if (tid > nr && tid <= RB) rs[tid] = rsg[r0 + nr];
-    __syncthreads();
-    if (rs[RB] == rs[0]) return;  // uniform: nothing lands in this tile
+    const int e0 = rsg[r0], e1 = rsg[r0 + nr];  // uniform
+    if (e0 == e1) return;                       // nothing lands in this tile
 
+    // The first batch of gathers, the twiddle table and the row starts are
+    // requested before the LDS set-up, so their latency overlaps the zeroing
+    // of the tile; only LDS work waits at the barrier.
+    constexpr int U = 4;
+    const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
+    const float4* vis = g.vis + 2 * ((int64_t)img * g.n_bl);
+    GatherBatch<U> gb;
+    gb.load_entries(ent, e0 + tid, e0, e1, NT);
+    constexpr int TWK = (N + NT - 1) / NT;
+    float2 twv[TWK];
+#pragma unroll
+    for (int k = 0; k < TWK; k++)
+        if (tid + k * NT < N) twv[k] = tw[tid + k * NT];
+    const int rsv = tid <= RB ? rsg[r0 + min(tid, nr)] : 0;
+    const double nu = g.freq_hz[ch];
+    gb.load_records(g, vis);
     float2* twl = tile + RB * TS;  // twiddle table copy [N]
-    for (int i = tid; i < N; i += NT) twl[i] = tw[i];
+#pragma unroll
+    for (int k = 0; k < TWK; k++)
+        if (tid + k * NT < N) twl[tid + k * NT] = twv[k];
+    if (tid <= RB) rs[tid] = rsv;
     for (int i = tid; i < RB * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
-    const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[RB], tid, NT,
-                                     [&](int e, int u_idx, float2 v) {
+    auto sink = [&](int e, int u_idx, float2 v) {
         int r = 0;
         while (r + 1 < RB && rs[r + 1] <= e) r++;
         float2* c = &tile[r * TS + pad_index(u_idx)];
@@ -108,7 +123,14 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         atomicAdd(&c->y, v.y);
         const int row = r0 + r;
         return (row == 0 || row == H) ? 1 : 2;
-    });
+    };
+    int n_ok = gb.consume(g, nu, sink);
+    for (int base = e0 + tid + NT * U; base < e1; base += NT * U) {
+        GatherBatch<U> gm;
+        gm.load_entries(ent, base, e0, e1, NT);
+        gm.load_records(g, vis);
+        n_ok += gm.consume(g, nu, sink);
+    }
     const int tot = block_sum_int(n_ok, red);
     if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();

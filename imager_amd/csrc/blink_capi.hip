@@ -587,6 +587,7 @@ struct BatchPlan {
     uint32_t* isums_all = nullptr;
     int64_t isums_count = 0;
     int* partial = nullptr;
+    uint32_t* cells = nullptr;
     float* inter = nullptr;
     // host copy of a group's first set descriptor (a kernel argument)
     const SetDesc& host_desc(const BatchGroup& g) const {
@@ -632,7 +633,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     // scratch sizes
     size_t rs_bytes = 0, en_bytes = 0, inter_bytes = 16;
     std::vector<size_t> rs_off(n_sets), en_off(n_sets), is_off(n_sets);
-    int64_t n_img_all = 0, n_units = 0, min_bl = INT64_MAX;
+    int64_t n_img_all = 0, n_units = 0, min_bl = INT64_MAX, max_bl = 0, n_cells = 0;
     int n_max = 0;
     for (int i = 0; i < n_sets; i++) {
         const blink_image_set& st = sets[i];
@@ -645,7 +646,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         is_off[i] = (size_t)n_img_all;
         n_img_all += (int64_t)st.n_ch * st.n_t;
         n_units += st.n_ch;
+        n_cells += (int64_t)st.n_ch * n_bl;
         min_bl = std::min<int64_t>(min_bl, n_bl);
+        max_bl = std::max<int64_t>(max_bl, n_bl);
         n_max = std::max(n_max, st.n_pixels);
     }
     for (BatchGroup& g : bp.groups) {
@@ -663,17 +666,18 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         g.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(g.n_img, inter_budget / per));
         inter_bytes = std::max(inter_bytes, (size_t)(per * g.chunk));
     }
-    // plan parts: about 2048 workgroups over the units, at least 256
-    // baselines per part, at most 64 parts
-    int parts = (int)std::max<int64_t>(1, (2048 + n_units - 1) / std::max<int64_t>(n_units, 1));
-    parts = (int)std::min<int64_t>({(int64_t)parts, 64, std::max<int64_t>(1, min_bl / 256)});
-    bp.parts = parts;
+    // plan parts: about 2048 workgroups over the units with at least 256
+    // baselines per part, and never more than kPlanPartMax baselines per part
+    int64_t parts = std::max<int64_t>(1, (2048 + n_units - 1) / std::max<int64_t>(n_units, 1));
+    parts = std::min<int64_t>({parts, 64, std::max<int64_t>(1, min_bl / 256)});
+    parts = std::max<int64_t>(parts, (max_bl + kPlanPartMax - 1) / kPlanPartMax);
+    bp.parts = (int)parts;
     bp.n_units = (int)n_units;
     bp.n_max = n_max;
     bp.isums_count = n_img_all;
     const size_t partial_bytes = sizeof(int) * (size_t)n_units * parts * n_max;
 
-    void *inter, *rs, *en, *is, *pa;
+    void *inter, *rs, *en, *is, *pa, *cl;
     if ((rc = scratch(ctx, "batch_inter", inter_bytes, &inter))) return rc;
     if ((rc = scratch(ctx, "batch_rows", std::max<size_t>(rs_bytes, 16), &rs))) return rc;
     if ((rc = scratch(ctx, "batch_entries", std::max<size_t>(en_bytes, 16), &en))) return rc;
@@ -681,6 +685,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         return rc;
     if ((rc = scratch(ctx, "batch_plan_parts", std::max<size_t>(partial_bytes, 16), &pa)))
         return rc;
+    if ((rc = scratch(ctx, "batch_plan_cells", std::max<size_t>(4 * (size_t)n_cells, 16), &cl)))
+        return rc;
+    bp.cells = (uint32_t*)cl;
     bp.inter = (float*)inter;
     bp.isums_all = (uint32_t*)is;
     bp.partial = (int*)pa;
@@ -739,7 +746,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     }
     size_t off = align16(sizeof(SetDesc) * desc.size());
     bp.units_off = off;
-    off = align16(off + sizeof(int2) * (size_t)n_units);
+    off = align16(off + sizeof(PlanUnit) * (size_t)n_units);
     for (BatchGroup& g : bp.groups) {
         if (g.sets.size() < 2 || g.transform != BLINK_TRANSFORM_FFT) continue;
         g.img_set_off = off;
@@ -747,10 +754,15 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     }
     bp.blob.assign(off, 0);
     memcpy(bp.blob.data(), desc.data(), sizeof(SetDesc) * desc.size());
-    int2* units = (int2*)(bp.blob.data() + bp.units_off);
-    int64_t u = 0;
-    for (int i = 0; i < n_sets; i++)
-        for (int ch = 0; ch < sets[i].n_ch; ch++) units[u++] = make_int2(desc_of[i], ch);
+    PlanUnit* units = (PlanUnit*)(bp.blob.data() + bp.units_off);
+    int64_t u = 0, cell_off = 0;
+    for (int i = 0; i < n_sets; i++) {
+        const int n_bl = sets[i].n_ant * (sets[i].n_ant + 1) / 2;
+        for (int ch = 0; ch < sets[i].n_ch; ch++) {
+            units[u++] = PlanUnit{desc_of[i], ch, cell_off};
+            cell_off += n_bl;
+        }
+    }
     for (BatchGroup& g : bp.groups) {
         if (g.sets.size() < 2 || g.transform != BLINK_TRANSFORM_FFT) continue;
         int* t = (int*)(bp.blob.data() + g.img_set_off);
@@ -766,7 +778,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
 // Device addresses of the blob's tables.
 struct BatchTables {
     const SetDesc* sets;
-    const int2* units;
+    const PlanUnit* units;
     const unsigned char* base;
     const SetDesc* group_sets(const BatchGroup& g) const { return sets + g.first_desc; }
     const int* img_set(const BatchGroup& g) const {
@@ -778,7 +790,7 @@ static BatchTables tables_at(const BatchPlan& bp, const void* dblob) {
     BatchTables t;
     t.base = (const unsigned char*)dblob;
     t.sets = (const SetDesc*)dblob;
-    t.units = (const int2*)(t.base + bp.units_off);
+    t.units = (const PlanUnit*)(t.base + bp.units_off);
     return t;
 }
 
@@ -862,7 +874,8 @@ static int enqueue_plans(blink_ctx* ctx, const BatchPlan& bp, const BatchTables&
                          hipStream_t s) {
     LAUNCH("zero", s, launch_zero(bp.isums_all, sizeof(uint32_t) * bp.isums_count, s));
     LAUNCH("plan", s,
-           launch_plan_units(t.sets, t.units, bp.n_units, bp.parts, bp.n_max, bp.partial, s));
+           launch_plan_units(t.sets, t.units, bp.n_units, bp.parts, bp.n_max, bp.partial,
+                             bp.cells, s));
     return BLINK_OK;
 }
 

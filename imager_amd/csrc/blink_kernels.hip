@@ -261,13 +261,17 @@ hipError_t launch_plan_rows(const float* u_m, const float* v_m, const int* flags
 //
 // One plan unit per (set, channel): the counting sort of the channel's 2 n_bl
 // cells by grid row, the same result k_plan_rows gives.  `parts` workgroups
-// per unit share its baselines: each counts its own slice into a private
-// histogram (count), a per-unit scan turns the histograms into row starts and
-// per-part write offsets (scan), and each part scatters its slice (scatter).
+// per unit share its baselines.
+//   count    the cell of every baseline of the part (FP64, bit-exact), kept
+//            packed for the scatter, and the part's histogram by row;
+//   scan     per unit: row starts, and each part's first slot in every row;
+//   scatter  each part sorts its entries by row in LDS and writes them out
+//            as contiguous runs (no FP64 work, no scattered 4-byte stores).
 // Entry order inside a row is not fixed; the sums it feeds are FP32 sums in
 // any order either way.
 
 constexpr int kUnitThreads = 256;
+constexpr uint32_t kNoCell = 0xffffffffu;
 
 __device__ __forceinline__ void part_range(int n_bl, int parts, int part, int& b0, int& b1) {
     const int per = (n_bl + parts - 1) / parts;
@@ -275,27 +279,72 @@ __device__ __forceinline__ void part_range(int n_bl, int parts, int part, int& b
     b1 = min(n_bl, b0 + per);
 }
 
+// cell_of with the min_uv test reduced to a NaN test when min_uv is -inf
+// (sqrt(u^2 + v^2) > -inf holds for every non-NaN u, v): same results.
+__device__ __forceinline__ bool plan_cell(float u_m, float v_m, double wl, const SetDesc& sd,
+                                          int& up, int& vp) {
+#pragma clang fp contract(off)
+    if (sd.min_uv == -INFINITY) {
+        const double u = (double)u_m / wl;
+        const double v = (double)v_m / wl;
+        if (isnan(u) || isnan(v)) return false;
+        const double ru = round(u / sd.du);
+        const double rv = round(v / sd.dv);
+        if (!(fabs(ru) < 1073741824.0) || !(fabs(rv) < 1073741824.0)) return false;
+        up = (int)ru;
+        vp = (int)rv;
+        return true;
+    }
+    return cell_of(u_m, v_m, wl, sd.du, sd.dv, sd.min_uv, sd.n_pixels, up, vp);
+}
+
+// Exclusive scan of x over the workgroup (kUnitThreads threads); returns the
+// thread's prefix and the total in *total.
+__device__ __forceinline__ int unit_scan(int x, int* warp_tot, int* total) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int y = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(y, o, 64);
+        if (lane >= o) y += t;
+    }
+    if (lane == 63) warp_tot[wid] = y;
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int w = 0; w < kUnitThreads / 64; w++) {
+        if (w < wid) before += warp_tot[w];
+        tot += warp_tot[w];
+    }
+    *total = tot;
+    return before + y - x;
+}
+
 __global__ void __launch_bounds__(kUnitThreads)
-k_plan_count(const SetDesc* __restrict__ sets, const int2* __restrict__ units, int parts,
-             int n_max, int* __restrict__ partial) {
+k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ units, int parts,
+             int n_max, int* __restrict__ partial, uint32_t* __restrict__ cells) {
     extern __shared__ int unit_lds[];
     int* hist = unit_lds;  // [n]
     const int unit = blockIdx.x / parts, part = blockIdx.x % parts;
-    const int2 us = units[unit];
-    const SetDesc sd = sets[us.x];
+    const PlanUnit pu = units[unit];
+    const SetDesc sd = sets[pu.set];
     const int n = sd.n_pixels, n_bl = sd.g.n_bl;
-    const double wl = wavelength_of(sd.g.freq_hz[us.y]);
+    const double wl = wavelength_of(sd.g.freq_hz[pu.ch]);
     for (int i = threadIdx.x; i < n; i += kUnitThreads) hist[i] = 0;
     __syncthreads();
     int b0, b1;
     part_range(n_bl, parts, part, b0, b1);
+    uint32_t* cl = cells + pu.cell_off;
     for (int bl = b0 + (int)threadIdx.x; bl < b1; bl += kUnitThreads) {
         const int f = sd.flags[bl];
         const float u = sd.u_m[bl], v = sd.v_m[bl];
         int up, vp;
-        if (f || !cell_of(u, v, wl, sd.du, sd.dv, sd.min_uv, n, up, vp)) continue;
-        atomicAdd(&hist[wrap_index(vp, n)], 1);
-        atomicAdd(&hist[wrap_index(-vp, n)], 1);
+        uint32_t packed = kNoCell;
+        if (!f && plan_cell(u, v, wl, sd, up, vp)) {
+            const int r = wrap_index(vp, n);
+            packed = ((uint32_t)r << 16) | (uint32_t)wrap_index(up, n);
+            atomicAdd(&hist[r], 1);
+            atomicAdd(&hist[r == 0 ? 0 : n - r], 1);
+        }
+        cl[bl] = packed;
     }
     __syncthreads();
     int* out = partial + ((int64_t)unit * parts + part) * n_max;
@@ -303,14 +352,14 @@ k_plan_count(const SetDesc* __restrict__ sets, const int2* __restrict__ units, i
 }
 
 __global__ void __launch_bounds__(kUnitThreads)
-k_plan_scan(const SetDesc* __restrict__ sets, const int2* __restrict__ units, int parts,
+k_plan_scan(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ units, int parts,
             int n_max, int* __restrict__ partial) {
     __shared__ int warp_tot[kUnitThreads / 64];
     const int unit = blockIdx.x;
-    const int2 us = units[unit];
-    const SetDesc sd = sets[us.x];
+    const PlanUnit pu = units[unit];
+    const SetDesc sd = sets[pu.set];
     const int n = sd.n_pixels;
-    int* rs = sd.row_start + (int64_t)us.y * (n + 1);
+    int* rs = sd.row_start + (int64_t)pu.ch * (n + 1);
     int* base = partial + (int64_t)unit * parts * n_max;
     const int tid = threadIdx.x;
     // each thread owns a contiguous chunk of rows: sum, block scan, rewrite
@@ -319,20 +368,8 @@ k_plan_scan(const SetDesc* __restrict__ sets, const int2* __restrict__ units, in
     int local = 0;
     for (int i = c0; i < c1; i++)
         for (int p = 0; p < parts; p++) local += base[(int64_t)p * n_max + i];
-    const int lane = tid & 63, wid = tid >> 6;
-    int x = local;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) warp_tot[wid] = x;
-    __syncthreads();
-    int before = 0, total = 0;
-    for (int w = 0; w < kUnitThreads / 64; w++) {
-        if (w < wid) before += warp_tot[w];
-        total += warp_tot[w];
-    }
-    int run = before + x - local;
+    int total;
+    int run = unit_scan(local, warp_tot, &total);
     for (int i = c0; i < c1; i++) {
         rs[i] = run;
         for (int p = 0; p < parts; p++) {
@@ -346,49 +383,86 @@ k_plan_scan(const SetDesc* __restrict__ sets, const int2* __restrict__ units, in
 }
 
 __global__ void __launch_bounds__(kUnitThreads)
-k_plan_scatter(const SetDesc* __restrict__ sets, const int2* __restrict__ units, int parts,
-               int n_max, const int* __restrict__ partial) {
+k_plan_scatter(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ units, int parts,
+               int n_max, const int* __restrict__ partial, const uint32_t* __restrict__ cells) {
     extern __shared__ int unit_lds[];
-    int* cursor = unit_lds;  // [n]
+    __shared__ int warp_tot[kUnitThreads / 64];
     const int unit = blockIdx.x / parts, part = blockIdx.x % parts;
-    const int2 us = units[unit];
-    const SetDesc sd = sets[us.x];
+    const PlanUnit pu = units[unit];
+    const SetDesc sd = sets[pu.set];
     const int n = sd.n_pixels, n_bl = sd.g.n_bl;
-    const double wl = wavelength_of(sd.g.freq_hz[us.y]);
+    int* first = unit_lds;            // [n] the part's first global slot per row
+    int* lstart = first + n_max;      // [n] the row's first local slot
+    int* lfill = lstart + n_max;      // [n]
+    uint32_t* lent = (uint32_t*)(lfill + n_max);              // [2 kPlanPartMax]
+    unsigned short* lrow = (unsigned short*)(lent + 2 * kPlanPartMax);  // [2 kPlanPartMax]
     const int* off = partial + ((int64_t)unit * parts + part) * n_max;
-    for (int i = threadIdx.x; i < n; i += kUnitThreads) cursor[i] = off[i];
+    const int* rs = sd.row_start + (int64_t)pu.ch * (n + 1);
+    const int tid = threadIdx.x;
+    // the part's count per row is the next part's first slot (or the row
+    // end) minus its own; local starts are their exclusive scan
+    const int chunk = (n + kUnitThreads - 1) / kUnitThreads;
+    const int c0 = min(tid * chunk, n), c1 = min(c0 + chunk, n);
+    int local = 0;
+    for (int i = c0; i < c1; i++) {
+        const int o = off[i];
+        const int e = part + 1 < parts ? off[n_max + i] : rs[i + 1];
+        first[i] = o;
+        lfill[i] = e - o;  // count, for now
+        local += e - o;
+    }
+    int n_local;
+    int run = unit_scan(local, warp_tot, &n_local);
+    for (int i = c0; i < c1; i++) {
+        const int c = lfill[i];
+        lstart[i] = run;
+        lfill[i] = 0;
+        run += c;
+    }
     __syncthreads();
-    uint32_t* ent = sd.entries + (int64_t)us.y * 2 * n_bl;
     int b0, b1;
     part_range(n_bl, parts, part, b0, b1);
-    for (int bl = b0 + (int)threadIdx.x; bl < b1; bl += kUnitThreads) {
-        const int f = sd.flags[bl];
-        const float u = sd.u_m[bl], v = sd.v_m[bl];
-        int up, vp;
-        if (f || !cell_of(u, v, wl, sd.du, sd.dv, sd.min_uv, n, up, vp)) continue;
-        const int p = atomicAdd(&cursor[wrap_index(vp, n)], 1);
-        ent[p] = (uint32_t)wrap_index(up, n) | ((uint32_t)bl << 14);
-        const int q = atomicAdd(&cursor[wrap_index(-vp, n)], 1);
-        ent[q] = (uint32_t)wrap_index(-up, n) | (1u << 13) | ((uint32_t)bl << 14);
+    const uint32_t* cl = cells + pu.cell_off;
+    for (int bl = b0 + tid; bl < b1; bl += kUnitThreads) {
+        const uint32_t packed = cl[bl];
+        if (packed == kNoCell) continue;
+        const int r = (int)(packed >> 16), u = (int)(packed & 0xffffu);
+        const int p = lstart[r] + atomicAdd(&lfill[r], 1);
+        lent[p] = (uint32_t)u | ((uint32_t)bl << 14);
+        lrow[p] = (unsigned short)r;
+        const int r2 = r == 0 ? 0 : n - r, u2 = u == 0 ? 0 : n - u;
+        const int q = lstart[r2] + atomicAdd(&lfill[r2], 1);
+        lent[q] = (uint32_t)u2 | (1u << 13) | ((uint32_t)bl << 14);
+        lrow[q] = (unsigned short)r2;
+    }
+    __syncthreads();
+    uint32_t* ent = sd.entries + (int64_t)pu.ch * 2 * n_bl;
+    for (int i = tid; i < n_local; i += kUnitThreads) {
+        const int r = lrow[i];
+        ent[first[r] + (i - lstart[r])] = lent[i];
     }
 }
 
-hipError_t launch_plan_units(const SetDesc* sets, const int2* units, int n_units, int parts,
-                             int n_max, int* partial, hipStream_t stream) {
+hipError_t launch_plan_units(const SetDesc* sets, const PlanUnit* units, int n_units, int parts,
+                             int n_max, int* partial, uint32_t* cells, hipStream_t stream) {
     if (n_units == 0) return hipSuccess;
     if (parts < 1 || n_max < 1 || n_max > kMaxPixels) return hipErrorInvalidValue;
     const int64_t blocks = (int64_t)n_units * parts;
     if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
-    const size_t lds = sizeof(int) * (size_t)n_max;
-    hipLaunchKernelGGL(k_plan_count, dim3((unsigned)blocks), dim3(kUnitThreads), lds, stream,
-                       sets, units, parts, n_max, partial);
+    const size_t lds_count = sizeof(int) * (size_t)n_max;
+    const size_t lds_scatter = sizeof(int) * 3 * (size_t)n_max +
+                               (sizeof(uint32_t) + sizeof(unsigned short)) * 2 * kPlanPartMax;
+    if (!lds_fits(lds_scatter)) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k_plan_count, dim3((unsigned)blocks), dim3(kUnitThreads), lds_count,
+                       stream, sets, units, parts, n_max, partial, cells);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_plan_scan, dim3((unsigned)n_units), dim3(kUnitThreads), 0, stream, sets,
                        units, parts, n_max, partial);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_plan_scatter, dim3((unsigned)blocks), dim3(kUnitThreads), lds, stream,
-                       sets, units, parts, n_max, (const int*)partial);
+    hipLaunchKernelGGL(k_plan_scatter, dim3((unsigned)blocks), dim3(kUnitThreads), lds_scatter,
+                       stream, sets, units, parts, n_max, (const int*)partial,
+                       (const uint32_t*)cells);
     return hipGetLastError();
 }
 
@@ -957,6 +1031,8 @@ hipError_t set_kernel_attributes() {
     set((const void*)k_grid_rows_fft<1, false, true>);
     set((const void*)k_grid_tiles);
     set((const void*)k_plan_rows);
+    set((const void*)k_plan_count);
+    set((const void*)k_plan_scatter);
     return e;
 }
 

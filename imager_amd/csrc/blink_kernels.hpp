@@ -130,11 +130,17 @@ hipError_t launch_average(const float* images, int64_t image_size, int n_images,
 hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                 int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
                                 float* inter, const float* tw, hipStream_t stream);
-// Gridding plans of a batched call: one unit per (set, channel), units[u] =
-// (set, channel); `parts` workgroups per unit split its baselines.  partial
-// is [n_units][parts][n_max] ints of scratch.
-hipError_t launch_plan_units(const SetDesc* sets, const int2* units, int n_units, int parts,
-                             int n_max, int* partial, hipStream_t stream);
+// Gridding plans of a batched call: one unit per (set, channel); `parts`
+// workgroups per unit split its baselines (at most kPlanPartMax each).
+// Scratch: partial is [n_units][parts][n_max] ints, cells holds one packed
+// cell per (unit, baseline) from cell_off on.
+constexpr int kPlanPartMax = 2048;
+struct PlanUnit {
+    int set, ch;
+    int64_t cell_off;
+};
+hipError_t launch_plan_units(const SetDesc* sets, const PlanUnit* units, int n_units, int parts,
+                             int n_max, int* partial, uint32_t* cells, hipStream_t stream);
 hipError_t launch_isums_to_double(const uint32_t* isums, double* sums, int n, hipStream_t stream);
 
 // Split-precision DFT-as-GEMM (blink_dft_mfma.hip): out = (D' applied to the
