@@ -31,6 +31,7 @@ WEIGHTING_NATURAL, WEIGHTING_UNIFORM = 0, 1
 CORR_GEOMETRIC, CORR_CABLE = 1, 2
 FFT_FORWARD, FFT_BACKWARD = -1, 1
 GRID_TILED, GRID_ATOMIC = 0, 1
+KERNEL_KAISER_BESSEL, KERNEL_KAISER_BESSEL_NO_SINC = 1, 2
 TRANSFORM_FFT, TRANSFORM_DFT_BF16X3 = 0, 1
 MAX_PIXELS = 4096
 
@@ -38,6 +39,7 @@ MAX_PIXELS = 4096
 EXPORTED_SYMBOLS = (
     "blink_create", "blink_destroy", "blink_last_error", "blink_version", "blink_fft_supported",
     "blink_apply_geometric_corrections", "blink_apply_cable_corrections", "blink_gridding",
+    "blink_gridding_kernel",
     "blink_uv_cells", "blink_fft2d_c2c", "blink_vector_sum", "blink_fft_shift_and_norm",
     "blink_image", "blink_image_averaging", "blink_image_batch", "blink_release_cache",
     "blink_profile_enable", "blink_profile_get", "blink_profile_kernels", "blink_profile_reset",
@@ -97,6 +99,8 @@ def load():
         "blink_apply_geometric_corrections": (i, [vp, vp, i, i, i, vp, vp, vp]),
         "blink_apply_cable_corrections": (i, [vp, vp, i, i, i, vp, vp, vp]),
         "blink_gridding": (i, [vp, vp, i, i, i, vp, vp, vp, vp, d, d, d, i, i, i, i, vp, vp, vp]),
+        "blink_gridding_kernel": (i, [vp, vp, i, i, i, vp, vp, vp, vp, d, d, d, i, i, i, i, i,
+                                      vp, vp, vp]),
         "blink_uv_cells": (i, [vp, vp, vp, vp, i, vp, i, d, d, d, i, vp, vp]),
         "blink_fft2d_c2c": (i, [vp, vp, vp, i, i, i, vp]),
         "blink_vector_sum": (i, [vp, vp, i64, i, vp, vp]),
@@ -211,6 +215,16 @@ class Context:
             _ptr(freq_hz), float(delta_u), float(delta_v), float(min_uv), int(n_pixels),
             POL[pol] if isinstance(pol, str) else int(pol), int(weighting), int(algo),
             _ptr(grids), _ptr(counts), _stream(stream)))
+
+    def gridding_kernel(self, vis, n_ant, n_ch, n_t, u_m, v_m, bl_flags, freq_hz, delta_u,
+                        delta_v, min_uv, n_pixels, pol, weighting, kernel_size, grids, counts,
+                        kernel_mode=KERNEL_KAISER_BESSEL, stream=None):
+        """Convolutional (Kaiser-Bessel) gridding of kernel_size^2 cells per visibility."""
+        self._check(load().blink_gridding_kernel(
+            self._h, _ptr(vis), n_ant, n_ch, n_t, _ptr(u_m), _ptr(v_m), _ptr(bl_flags),
+            _ptr(freq_hz), float(delta_u), float(delta_v), float(min_uv), int(n_pixels),
+            POL[pol] if isinstance(pol, str) else int(pol), int(weighting), int(kernel_size),
+            int(kernel_mode), _ptr(grids), _ptr(counts), _stream(stream)))
 
     def uv_cells(self, u_m, v_m, bl_flags, n_bl, freq_hz, n_ch, delta_u, delta_v, min_uv,
                  n_pixels, cells, stream=None):

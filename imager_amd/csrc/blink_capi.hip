@@ -44,6 +44,7 @@ struct blink_ctx {
     std::string error;
     std::map<int, float*> twiddles;               // N -> device table of N float2
     std::map<int, uint16_t*> dft_planes;          // N -> split DFT matrix planes
+    std::map<std::pair<int, int>, double*> kb_tables;  // (size, mode) -> gridding kernels
     std::map<std::string, DeviceBuffer> scratch;  // named grow-only buffers
     std::map<std::string, GraphEntry> graphs;     // descriptor bytes -> graph
     uint64_t graph_clock = 0;
@@ -337,6 +338,7 @@ BLINK_API void blink_destroy(blink_ctx* ctx) {
     blink_release_cache(ctx);
     for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
     for (auto& kv : ctx->dft_planes) (void)hipFree(kv.second);
+    for (auto& kv : ctx->kb_tables) (void)hipFree(kv.second);
     prof_collect(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
@@ -465,6 +467,90 @@ BLINK_API int blink_gridding(blink_ctx* ctx, const float* vis, int n_ant, int n_
     if (weighting == BLINK_WEIGHTING_UNIFORM)
         LAUNCH("uniform_weight", s,
                launch_uniform_weight(grids, counts, (int64_t)n_img * n_pixels * n_pixels, s));
+    return BLINK_OK;
+}
+
+// Modified Bessel function I0 by its power series sum (x/2)^2m / (m!)^2,
+// summed until a term falls to `rel` of the sum (the stopping rule of the
+// reference's bessel0, src/pacer_imager.cpp:566-578).
+static double bessel_i0(double x, double rel) {
+    double sum = 1.0, term = 1.0, d = 0.0;
+    do {
+        d += 2.0;
+        term *= x * x / (d * d);
+        sum += term;
+    } while (term > sum * rel);
+    return sum;
+}
+
+// Gridding kernel table: row r (oversampled fractional position), column c
+// (footprint cell) = the reference's _griddingKernels[r][c], built by
+// makeKaiserBesselKernel (alpha 8.6, oversampling 1023, optional sinc) and
+// makeKernels (src/pacer_imager.cpp:580-651).  The 1-D kernel of K * 1023
+// samples is filled from its middle outwards and mirrored; for even K the
+// reference also writes one value past its end, which is not kept here.
+static int kb_table(blink_ctx* ctx, int K, int mode, const double** out) {
+    auto key = std::make_pair(K, mode);
+    auto it = ctx->kb_tables.find(key);
+    if (it != ctx->kb_tables.end()) {
+        *out = it->second;
+        return BLINK_OK;
+    }
+    const int os = 1023;
+    const double alpha = 8.6;
+    const size_t n = (size_t)K * os, mid = n / 2;
+    std::vector<double> k1(n, 0.0);
+    const double ratio = 1.0 / (double)os;
+    const double norm = (double)os / bessel_i0(alpha, 1e-8);
+    for (size_t i = 0; i <= mid; i++) {
+        const double x = (double)i;
+        const double sinc = i == 0 ? ratio
+                            : (mode == BLINK_KERNEL_KAISER_BESSEL ? std::sin(M_PI * ratio * x) / (M_PI * x)
+                                                                  : ratio);
+        const double t = (double)i / (double)mid;
+        if (mid + i < n) k1[mid + i] = sinc * bessel_i0(alpha * std::sqrt(1.0 - t * t), 1e-8) * norm;
+    }
+    for (size_t i = 0; i < mid; i++) k1[i] = k1[n - 1 - i];
+    std::vector<double> table((size_t)os * K);
+    for (int i = 0; i < os; i++)
+        for (int c = 0; c < K; c++) table[(size_t)(os - 1 - i) * K + c] = k1[(size_t)c * os + i];
+    double* d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(double) * table.size()));
+    HIP_TRY(hipMemcpy(d, table.data(), sizeof(double) * table.size(), hipMemcpyHostToDevice));
+    ctx->kb_tables[key] = d;
+    *out = d;
+    return BLINK_OK;
+}
+
+BLINK_API int blink_gridding_kernel(blink_ctx* ctx, const float* vis, int n_ant, int n_ch,
+                                    int n_t, const float* u_m, const float* v_m,
+                                    const int* bl_flags, const double* freq_hz, double delta_u,
+                                    double delta_v, double min_uv, int n_pixels, int pol,
+                                    int weighting, int kernel_size, int kernel_mode,
+                                    float* grids, float* counts, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    int rc = check_vis_shape(ctx, n_ant, n_ch, n_t);
+    if (rc) return rc;
+    CHECK_ARG(vis && u_m && v_m && bl_flags && freq_hz && grids && counts, "null pointer argument");
+    CHECK_ARG(pol >= 0 && pol <= 3, "pol must be 0..3 (XX, YY, I, V), got %d", pol);
+    CHECK_ARG(weighting == BLINK_WEIGHTING_NATURAL || weighting == BLINK_WEIGHTING_UNIFORM,
+              "weighting must be NATURAL or UNIFORM");
+    CHECK_ARG(kernel_size >= 1 && kernel_size <= BLINK_MAX_KERNEL_SIZE,
+              "kernel_size must be 1..%d, got %d", BLINK_MAX_KERNEL_SIZE, kernel_size);
+    CHECK_ARG(kernel_mode == BLINK_KERNEL_KAISER_BESSEL ||
+                  kernel_mode == BLINK_KERNEL_KAISER_BESSEL_NO_SINC,
+              "unknown kernel mode %d", kernel_mode);
+    CHECK_ARG(n_pixels >= 1 && n_pixels <= 65535, "n_pixels=%d out of range", n_pixels);
+    if ((rc = set_device(ctx))) return rc;
+    const double* table;
+    if ((rc = kb_table(ctx, kernel_size, kernel_mode, &table))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH("grid_kernel", s,
+           launch_grid_kb(vis, n_ant, n_ch, n_t, u_m, v_m, bl_flags, freq_hz, delta_u, delta_v,
+                          min_uv, n_pixels, pol, kernel_size, table, grids, counts, s));
+    if (weighting == BLINK_WEIGHTING_UNIFORM)
+        LAUNCH("uniform_weight", s,
+               launch_uniform_weight(grids, counts, (int64_t)n_ch * n_t * n_pixels * n_pixels, s));
     return BLINK_OK;
 }
 

@@ -173,6 +173,89 @@ hipError_t launch_grid_atomic(const float* vis, int n_ant, int n_ch, int n_t, co
     return hipGetLastError();
 }
 
+// --------------------------------------- (b) convolutional (Kaiser-Bessel) gridding
+
+// Column (or row) of footprint coordinate p for sign s, as calc_xy_grid2
+// computes it (src/pacer_imager.cpp:716-750): index s*p + N/2, then the
+// half swap; coordinates that land outside the grid are dropped.
+__device__ __forceinline__ bool kb_cell(int p, int s, int n, int& x) {
+    const int c = n / 2;
+    const int idx = s * p + c;
+    x = idx < c ? c + idx : idx - c;
+    return x >= 0 && x < n;
+}
+
+// One thread per (t, ch, bl).  The visibility and its conjugate are spread
+// over K x K cells with weights kt[ky][j] * kt[kx][i] (double), the kernel
+// row chosen from the 1023-times oversampled fractional cell position as
+// convolve_with_kernel does (src/pacer_imager.cpp:654-714); counts go to the
+// nearest cells (:387,408).  FP32 global atomics.
+__global__ void k_grid_kb(const float4* __restrict__ vis, int n_bl, int n_ch, int64_t n_rec,
+                          const float* __restrict__ u_m, const float* __restrict__ v_m,
+                          const int* __restrict__ flags, const double* __restrict__ freq_hz,
+                          double du, double dv, double min_uv, int n_pixels, int pol, int K,
+                          const double* __restrict__ kt, float* __restrict__ grids,
+                          float* __restrict__ counts) {
+#pragma clang fp contract(off)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t gs = (int64_t)n_pixels * n_pixels;
+    const int mid = K / 2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_rec; i += stride) {
+        const int bl = (int)(i % n_bl);
+        const int64_t img = i / n_bl;
+        const int ch = (int)(img % n_ch);
+        if (flags[bl]) continue;
+        const float2 val = select_pol(vis[2 * i], vis[2 * i + 1], pol);
+        if (!vis_ok(val)) continue;
+        const double wl = wavelength_of(freq_hz[ch]);
+        int up, vp;
+        if (!cell_of(u_m[bl], v_m[bl], wl, du, dv, min_uv, n_pixels, up, vp)) continue;
+        const double ue = (double)u_m[bl] / wl / du;
+        const double ve = (double)v_m[bl] / wl / dv;
+        int kx = (int)round((ue - (double)up) * 1023.0);
+        int ky = (int)round((ve - (double)vp) * 1023.0);
+        kx = (kx + 1534) % 1023;
+        ky = (ky + 1534) % 1023;
+        const double* xk = kt + (int64_t)kx * K;
+        const double* yk = kt + (int64_t)ky * K;
+        float* g = grids + 2 * img * gs;
+        for (int s = 1; s >= -1; s -= 2) {
+            for (int j = 0; j < K; j++) {
+                int y;
+                if (!kb_cell(vp - mid + j, s, n_pixels, y)) continue;
+                const double wy = yk[j];
+                for (int c = 0; c < K; c++) {
+                    int x;
+                    if (!kb_cell(up - mid + c, s, n_pixels, x)) continue;
+                    const double w = wy * xk[c];
+                    const double re = (double)val.x * w;
+                    const double im = (double)s * (double)val.y * w;
+                    const int64_t p = (int64_t)y * n_pixels + x;
+                    unsafeAtomicAdd(&g[2 * p], (float)re);
+                    unsafeAtomicAdd(&g[2 * p + 1], (float)im);
+                }
+            }
+        }
+        unsafeAtomicAdd(&counts[img * gs + (int64_t)wrap_index(vp, n_pixels) * n_pixels +
+                                wrap_index(up, n_pixels)], 1.0f);
+        unsafeAtomicAdd(&counts[img * gs + (int64_t)wrap_index(-vp, n_pixels) * n_pixels +
+                                wrap_index(-up, n_pixels)], 1.0f);
+    }
+}
+
+hipError_t launch_grid_kb(const float* vis, int n_ant, int n_ch, int n_t, const float* u_m,
+                          const float* v_m, const int* flags, const double* freq_hz, double du,
+                          double dv, double min_uv, int n_pixels, int pol, int kernel_size,
+                          const double* table, float* grids, float* counts, hipStream_t stream) {
+    const int n_bl = n_ant * (n_ant + 1) / 2;
+    const int64_t n_rec = (int64_t)n_bl * n_ch * n_t;
+    if (n_rec == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grid_kb, dim3(grid_for(n_rec, 256, 65536)), dim3(256), 0, stream,
+                       (const float4*)vis, n_bl, n_ch, n_rec, u_m, v_m, flags, freq_hz, du, dv,
+                       min_uv, n_pixels, pol, kernel_size, table, grids, counts);
+    return hipGetLastError();
+}
+
 // --------------------------------------------- (b) gridding plan (row CSR)
 
 // Per channel: every gridded (baseline, conjugate) pair becomes two entries,

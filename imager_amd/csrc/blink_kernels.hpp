@@ -105,6 +105,12 @@ hipError_t launch_grid_atomic(const float* vis, int n_ant, int n_ch, int n_t, co
                               const float* v_m, const int* flags, const double* freq_hz, double du,
                               double dv, double min_uv, int n_pixels, int pol, float* grids,
                               float* counts, hipStream_t stream);
+// Convolutional gridding with a kernel table of 1023 rows of kernel_size
+// doubles (row = oversampled fractional cell position).
+hipError_t launch_grid_kb(const float* vis, int n_ant, int n_ch, int n_t, const float* u_m,
+                          const float* v_m, const int* flags, const double* freq_hz, double du,
+                          double dv, double min_uv, int n_pixels, int pol, int kernel_size,
+                          const double* table, float* grids, float* counts, hipStream_t stream);
 hipError_t launch_plan_rows(const float* u_m, const float* v_m, const int* flags, int n_bl,
                             const double* freq_hz, int n_ch, double du, double dv, double min_uv,
                             int n_pixels, int* row_start, uint32_t* entries, hipStream_t stream);

@@ -308,9 +308,16 @@ Images CPacerImagerMI355X::run(Visibilities& xcorr) {
 }
 
 void CPacerImagerMI355X::gridding(Visibilities& xcorr) {
-    if (m_nConvolvingKernelSize > 0)
-        throw std::runtime_error("convolutional (Kaiser-Bessel) gridding is not provided by the "
-                                 "MI355X imager; set m_nConvolvingKernelSize = 0");
+    int kernel_mode = 0;
+    if (m_nConvolvingKernelSize > 0) {
+        if (_gridMode == GridMode::KaiserBesselWithoutSinc)
+            kernel_mode = BLINK_KERNEL_KAISER_BESSEL_NO_SINC;
+        else if (_gridMode == GridMode::KaiserBesselKernel ||
+                 _gridMode == GridMode::NearestNeighbourGridding)
+            kernel_mode = BLINK_KERNEL_KAISER_BESSEL;
+        else
+            throw std::runtime_error("only the Kaiser-Bessel gridding kernels are implemented");
+    }
     upload_inputs(xcorr);
     xcorr.to_gpu();
     const size_t n_img = n_gridded_channels * n_gridded_intervals;
@@ -323,11 +330,21 @@ void CPacerImagerMI355X::gridding(Visibilities& xcorr) {
     }
     const int uniform = std::strcmp(weighting, "U") == 0 ? BLINK_WEIGHTING_UNIFORM
                                                          : BLINK_WEIGHTING_NATURAL;
-    check(blink_gridding(ctx_, (const float*)xcorr.data(), (int)xcorr.obsInfo.nAntennas,
-                         (int)n_gridded_channels, (int)n_gridded_intervals, u_gpu.data(),
-                         v_gpu.data(), baseline_flags_gpu.data(), frequencies_gpu.data(), delta_u,
-                         delta_v, min_uv, n_pixels, (int)pol_to_image, uniform, BLINK_GRID_TILED,
-                         (float*)grids.data(), grids_counters.data(), nullptr));
+    if (kernel_mode)
+        check(blink_gridding_kernel(ctx_, (const float*)xcorr.data(),
+                                    (int)xcorr.obsInfo.nAntennas, (int)n_gridded_channels,
+                                    (int)n_gridded_intervals, u_gpu.data(), v_gpu.data(),
+                                    baseline_flags_gpu.data(), frequencies_gpu.data(), delta_u,
+                                    delta_v, min_uv, n_pixels, (int)pol_to_image, uniform,
+                                    m_nConvolvingKernelSize, kernel_mode, (float*)grids.data(),
+                                    grids_counters.data(), nullptr));
+    else
+        check(blink_gridding(ctx_, (const float*)xcorr.data(), (int)xcorr.obsInfo.nAntennas,
+                             (int)n_gridded_channels, (int)n_gridded_intervals, u_gpu.data(),
+                             v_gpu.data(), baseline_flags_gpu.data(), frequencies_gpu.data(),
+                             delta_u, delta_v, min_uv, n_pixels, (int)pol_to_image, uniform,
+                             BLINK_GRID_TILED, (float*)grids.data(), grids_counters.data(),
+                             nullptr));
 }
 
 Images CPacerImagerMI355X::image(ObservationInfo& obs_info) {
@@ -386,6 +403,11 @@ void CPacerImagerMI355X::ApplyCableCorrections(Visibilities& xcorr, MemoryBuffer
 }
 
 Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
+    if (std::strcmp(weighting, "U") == 0)
+        throw std::invalid_argument("run_batched images with natural weighting; use run() for \"U\"");
+    if (m_nConvolvingKernelSize > 0)
+        throw std::invalid_argument("run_batched grids to the nearest cell; use run() for "
+                                    "convolutional gridding");
     UpdateFlags();
     CalculateUVW();
     if (frequencies.size() != xcorr.nFrequencies) frequencies.allocate(xcorr.nFrequencies);
@@ -437,6 +459,16 @@ Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
     check(blink_image_batch(ctx_, &st, 1, nullptr));
     const double pixscale_ra = (1.00 / (oversampling_factor * u_max)) * (180.00 / kPi);
     const double pixscale_dec = (1.00 / (oversampling_factor * v_max)) * (180.00 / kPi);
+    if (average_images) {
+        // the average of every image, as image() returns it (complex, the
+        // GPU path's averaging, src/gpu/gpu_utils.cpp:143-161)
+        MemoryBuffer<std::complex<float>> avg((size_t)n_pixels * n_pixels, MemoryType::DEVICE);
+        check(blink_image_averaging(ctx_, (const float*)images_buffer.data(),
+                                    (int64_t)n_pixels * n_pixels, (int)n_img, 0,
+                                    (float*)avg.data(), nullptr));
+        return Images{std::move(avg), xcorr.obsInfo, 1, 1, (unsigned)n_pixels,
+                      m_MetaData.raHrs * 15.0, m_MetaData.decDegs, pixscale_ra, pixscale_dec};
+    }
     return Images{std::move(images_buffer), xcorr.obsInfo, (unsigned)n_gridded_intervals,
                   (unsigned)n_gridded_channels, (unsigned)n_pixels, m_MetaData.raHrs * 15.0,
                   m_MetaData.decDegs, pixscale_ra, pixscale_dec};

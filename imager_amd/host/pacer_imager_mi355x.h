@@ -4,7 +4,8 @@
 // Same constructor, public members and methods as CPacerImager; the four
 // virtuals that the reference's GPU class CPacerImagerHip overrides
 // (src/gpu/pacer_imager_hip.h:46-54) run on the GPU through the C ABI:
-//   gridding()                  -> blink_gridding (LDS-tiled, per-image counts)
+//   gridding()                  -> blink_gridding (LDS-tiled, per-image counts), or
+//                                  blink_gridding_kernel when m_nConvolvingKernelSize > 0
 //   image()                     -> blink_image (+ blink_image_averaging)
 //   ApplyGeometricCorrections() -> blink_apply_geometric_corrections (in place)
 //   ApplyCableCorrections()     -> blink_apply_cable_corrections (in place)
@@ -23,8 +24,9 @@
 //   * counts are kept per image, as the reference's CPU path does
 //     (src/pacer_imager.cpp:387,408; its GPU path counts timestep 0 only);
 //   * the geometric correction uses w[bl] (src/gpu/corrections_gpu.cpp:61);
-//   * Kaiser-Bessel convolutional gridding (m_nConvolvingKernelSize > 0) is
-//     not provided: run()/gridding() throw std::runtime_error for it.
+//   * run_batched() images with natural weighting and nearest-cell gridding
+//     only; it throws std::invalid_argument for weighting "U" or
+//     m_nConvolvingKernelSize > 0 (run() handles both).
 #ifndef PACER_IMAGER_MI355X_H
 #define PACER_IMAGER_MI355X_H
 
@@ -126,6 +128,9 @@ class CPacerImagerMI355X {
 
     Polarization pol_to_image{Polarization::XX};
     double m_fFrequencyMHz{-1};
+    // convolutional gridding: kernel size in cells (> 0 enables it) and kernel
+    // (src/pacer_imager.h:168-169)
+    GridMode _gridMode{GridMode::KaiserBesselKernel};
     int m_nConvolvingKernelSize{0};
 
     CPacerImagerMI355X(const std::string metadata_file, int n_pixels,

@@ -54,6 +54,14 @@ enum blink_pol { BLINK_POL_XX = 0, BLINK_POL_YY = 1, BLINK_POL_I = 2, BLINK_POL_
 /* Weighting: natural, or the reference's "U" (src/pacer_imager.cpp:421-428). */
 enum blink_weighting { BLINK_WEIGHTING_NATURAL = 0, BLINK_WEIGHTING_UNIFORM = 1 };
 
+/* Convolution kernels of blink_gridding_kernel (the implemented GridMode
+ * values, src/pacer_imager_defs.h:99-137, src/pacer_imager.cpp:617-623). */
+enum blink_grid_kernel {
+    BLINK_KERNEL_KAISER_BESSEL = 1,          /* Kaiser-Bessel (alpha 8.6) times a sinc */
+    BLINK_KERNEL_KAISER_BESSEL_NO_SINC = 2   /* Kaiser-Bessel alone */
+};
+#define BLINK_MAX_KERNEL_SIZE 31
+
 /* Phase corrections fused into blink_image_batch. */
 enum blink_corrections { BLINK_CORR_GEOMETRIC = 1u, BLINK_CORR_CABLE = 2u };
 
@@ -123,6 +131,23 @@ BLINK_API int blink_gridding(blink_ctx* ctx, const float* vis, int n_ant, int n_
                              const double* freq_hz, double delta_u, double delta_v,
                              double min_uv, int n_pixels, int pol, int weighting, int algo,
                              float* grids, float* counts, void* stream);
+
+/* Convolutional gridding: CPacerImager::gridding with m_nConvolvingKernelSize
+ * = kernel_size > 0 (src/pacer_imager.cpp:378-381,398-401).  Each gridded
+ * visibility and its conjugate are spread over kernel_size x kernel_size
+ * cells with the 1023-times oversampled kernels of makeKernels /
+ * makeKaiserBesselKernel (:566-651), placed as convolve_with_kernel and
+ * calc_xy_grid2 place them (:654-750; footprint cells outside the grid are
+ * dropped, not wrapped); counts stay at the nearest cell, as the reference
+ * keeps them.  Same inputs, accumulation and weighting as blink_gridding.
+ * kernel_size 1..BLINK_MAX_KERNEL_SIZE (even sizes follow the reference's
+ * kernel table except its one write past the table end). */
+BLINK_API int blink_gridding_kernel(blink_ctx* ctx, const float* vis, int n_ant, int n_ch,
+                                    int n_t, const float* u_m, const float* v_m,
+                                    const int* bl_flags, const double* freq_hz, double delta_u,
+                                    double delta_v, double min_uv, int n_pixels, int pol,
+                                    int weighting, int kernel_size, int kernel_mode,
+                                    float* grids, float* counts, void* stream);
 
 /* Cell indices of every (channel, baseline): cells[(ch*n_bl + bl)*4 + k] =
  * u, v, u_conj, v_conj, or -1 for flagged / min_uv-cut baselines.  The

@@ -79,6 +79,12 @@ def lib():
         fn.restype = _c_i64
         fn.argtypes = [_f32p, _c_int, _c_int, _c_int, _f32p, _f32p, _i32p, _f64p, _c_dbl,
                        _c_dbl, _c_dbl, _c_int, _c_int, _c_int, gp, gp]
+    L.oracle_kb_kernels.argtypes = [_c_int, _c_int, _f64p]
+    for name, gp in (("oracle_gridding_kb_f32", _f32p), ("oracle_gridding_kb_f64", _f64p)):
+        fn = getattr(L, name)
+        fn.restype = _c_i64
+        fn.argtypes = [_f32p, _c_int, _c_int, _c_int, _f32p, _f32p, _i32p, _f64p, _c_dbl,
+                       _c_dbl, _c_dbl, _c_int, _c_int, _c_int, _c_int, _c_int, gp, gp]
     L.oracle_fft2d_f32.restype = _c_int
     L.oracle_fft2d_f32.argtypes = [_f32p, _f32p, _c_int, _c_int, _c_int]
     L.oracle_fft2d_f64.restype = _c_int
@@ -189,24 +195,36 @@ def cells(u_m, v_m, flags, freq_hz, delta_u, delta_v, min_uv, n_pixels) -> np.nd
 
 
 def gridding(vis, n_ant, u_m, v_m, flags, freq_hz, delta_u, delta_v, min_uv, n_pixels,
-             pol="XX", uniform=False, precision="f32"):
+             pol="XX", uniform=False, precision="f32", kernel_size=0, with_sinc=True):
     """Grid complex64 vis [t][ch][bl][4].  Returns (grids, counts, n_gridded) with
-    grids complex [t][ch][N][N] and counts [t][ch][N][N]."""
+    grids complex [t][ch][N][N] and counts [t][ch][N][N].  kernel_size > 0
+    selects the convolutional (Kaiser-Bessel) mode of the reference."""
     vis = np.ascontiguousarray(vis, dtype=np.complex64)
     n_t, n_ch = vis.shape[0], vis.shape[1]
     shape = (n_t, n_ch, n_pixels, n_pixels)
+    kb = kernel_size > 0
     if precision == "f32":
         g = np.zeros(shape, np.complex64)
         c = np.zeros(shape, np.float32)
-        fn = lib().oracle_gridding_f32
+        fn = lib().oracle_gridding_kb_f32 if kb else lib().oracle_gridding_f32
     else:
         g = np.zeros(shape, np.complex128)
         c = np.zeros(shape, np.float64)
-        fn = lib().oracle_gridding_f64
-    n = fn(_as_float_view(vis), n_ant, n_ch, n_t, _f32(u_m), _f32(v_m), _i32(flags),
-           _f64(freq_hz), delta_u, delta_v, min_uv, n_pixels, POL[pol] if isinstance(pol, str)
-           else int(pol), 1 if uniform else 0, _as_float_view(g), c)
+        fn = lib().oracle_gridding_kb_f64 if kb else lib().oracle_gridding_f64
+    args = [_as_float_view(vis), n_ant, n_ch, n_t, _f32(u_m), _f32(v_m), _i32(flags),
+            _f64(freq_hz), delta_u, delta_v, min_uv, n_pixels,
+            POL[pol] if isinstance(pol, str) else int(pol), 1 if uniform else 0]
+    if kb:
+        args += [int(kernel_size), 1 if with_sinc else 0]
+    n = fn(*args, _as_float_view(g), c)
     return g, c, n
+
+
+def kb_kernels(kernel_size: int, with_sinc: bool = True) -> np.ndarray:
+    """The convolutional mode's kernel table [1023][K] (double)."""
+    t = np.zeros((1023, kernel_size), np.float64)
+    lib().oracle_kb_kernels(int(kernel_size), 1 if with_sinc else 0, t)
+    return t
 
 
 def fft2d(grids: np.ndarray, sign: int = +1) -> np.ndarray:

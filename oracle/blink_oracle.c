@@ -299,6 +299,152 @@ int64_t oracle_gridding_f64(const float* vis, int n_ant, int n_ch, int n_t,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Convolutional (Kaiser-Bessel) gridding.                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Power series of I0, stopped when a term is within `precision` of the sum
+ * (bessel0, src/pacer_imager.cpp:566-578). */
+static double kb_bessel0(double x, double precision) {
+    double d = 0.0, ds = 1.0, sum = 1.0;
+    do {
+        d += 2.0;
+        ds *= x * x / (d * d);
+        sum += ds;
+    } while (ds > sum * precision);
+    return sum;
+}
+
+/* makeKaiserBesselKernel (:580-599) on K * 1023 samples, then makeKernels'
+ * re-ordering into 1023 rows of K (:601-651).  For even K the reference
+ * writes kernel[n] (one past the end); that value is dropped here. */
+void oracle_kb_kernels(int K, int with_sinc, double* table) {
+    const int os = 1023;
+    const size_t n = (size_t)K * os, mid = n / 2;
+    double* k1 = (double*)calloc(n, sizeof(double));
+    double* sinc = (double*)malloc(sizeof(double) * (mid + 1));
+    const double filter_ratio = 1.0 / (double)os;
+    sinc[0] = filter_ratio;
+    for (size_t i = 1; i != mid + 1; i++) {
+        const double x = (double)i;
+        sinc[i] = with_sinc ? (sin(M_PI * filter_ratio * x) / (M_PI * x)) : filter_ratio;
+    }
+    const double norm = (double)os / kb_bessel0(8.6, 1e-8);
+    for (size_t i = 0; i != mid + 1; i++) {
+        const double term = (double)i / (double)mid;
+        if (mid + i < n) k1[mid + i] = sinc[i] * kb_bessel0(8.6 * sqrt(1.0 - term * term), 1e-8) * norm;
+    }
+    for (size_t i = 0; i != mid; i++) k1[i] = k1[n - 1 - i];
+    for (int i = 0; i < os; i++)
+        for (int x = 0; x < K; x++) table[(size_t)(os - i - 1) * K + x] = k1[(size_t)x * os + i];
+    free(sinc);
+    free(k1);
+}
+
+/* calc_xy_grid2 with is_odd = 0: grid column of footprint coordinate p for
+ * im_sign s; returns 0 when it falls outside the grid. */
+static int kb_xy(int p, int s, int n_pixels, int* x) {
+    const int center = n_pixels / 2;
+    const int idx = s * p + n_pixels / 2;
+    *x = idx < center ? center + idx : idx - center;
+    return *x >= 0 && *x < n_pixels;
+}
+
+#define KB_CONVOLVE(ACC_T)                                                                     \
+    {                                                                                          \
+        const double u_exact = u / delta_u, v_exact = v / delta_v;                             \
+        const double u_pix = round(u / delta_u), v_pix = round(v / delta_v);                   \
+        int xk = (int)round((u_exact - u_pix) * 1023);                                         \
+        int yk = (int)round((v_exact - v_pix) * 1023);                                         \
+        xk = (xk + (1023 * 3) / 2) % 1023;                                                     \
+        yk = (yk + (1023 * 3) / 2) % 1023;                                                     \
+        const double* xker = table + (size_t)xk * K;                                           \
+        const double* yker = table + (size_t)yk * K;                                           \
+        const int mid = K / 2;                                                                 \
+        int vv = (int)(v_pix - mid);                                                           \
+        for (int j = 0; j != K; ++j, ++vv) {                                                   \
+            const double ykv = yker[j];                                                        \
+            int uu = (int)(u_pix - mid);                                                       \
+            for (int i = 0; i != K; ++i, ++uu) {                                               \
+                const double kv = ykv * xker[i];                                               \
+                const double re_conv = re * kv;                                                \
+                const double im_conv = im_sign * im * kv;                                      \
+                int xg, yg;                                                                    \
+                if (!kb_xy(uu, (int)im_sign, n_pixels, &xg) ||                                 \
+                    !kb_xy(vv, (int)im_sign, n_pixels, &yg))                                   \
+                    continue;                                                                  \
+                const int64_t c = (int64_t)yg * n_pixels + xg;                                 \
+                g[2 * c] = (ACC_T)((double)g[2 * c] + re_conv);                                \
+                g[2 * c + 1] = (ACC_T)((double)g[2 * c + 1] + im_conv);                        \
+            }                                                                                  \
+        }                                                                                      \
+    }
+
+#define GRIDDING_KB_BODY(ACC_T)                                                                \
+    const int64_t n_bl = oracle_n_baselines(n_ant);                                            \
+    const int64_t grid_size = (int64_t)n_pixels * n_pixels;                                    \
+    double* table = (double*)malloc(sizeof(double) * 1023 * (size_t)K);                        \
+    oracle_kb_kernels(K, with_sinc, table);                                                    \
+    int64_t n_gridded = 0;                                                                     \
+    _Pragma("omp parallel for collapse(2) schedule(static) reduction(+:n_gridded)")            \
+    for (int t = 0; t < n_t; t++) {                                                            \
+        for (int ch = 0; ch < n_ch; ch++) {                                                    \
+            ACC_T* g = grids + ((int64_t)t * n_ch + ch) * grid_size * 2;                       \
+            ACC_T* cnt = counts + ((int64_t)t * n_ch + ch) * grid_size;                        \
+            const double wavelength = ORACLE_VEL_LIGHT / freq_hz[ch];                          \
+            for (int64_t bl = 0; bl < n_bl; bl++) {                                            \
+                if (bl_flags[bl]) continue;                                                    \
+                const float* rec = vis + (((int64_t)t * n_ch + ch) * n_bl + bl) * 8;           \
+                float ref, imf;                                                                \
+                select_pol(rec, pol, &ref, &imf);                                              \
+                if (isnan(ref) || isnan(imf)) continue;                                        \
+                if (!(fabs(ref) < ORACLE_MAX_VIS && fabs(imf) < ORACLE_MAX_VIS)) continue;     \
+                int ui, vi, uc, vc;                                                            \
+                if (!oracle_cell(u_m[bl], v_m[bl], freq_hz[ch], delta_u, delta_v, min_uv,      \
+                                 n_pixels, &ui, &vi, &uc, &vc))                                \
+                    continue;                                                                  \
+                const double u = (double)u_m[bl] / wavelength;                                 \
+                const double v = (double)v_m[bl] / wavelength;                                 \
+                const float re = ref, im = imf;                                                \
+                {                                                                              \
+                    const double im_sign = 1.00;                                               \
+                    KB_CONVOLVE(ACC_T)                                                         \
+                }                                                                              \
+                cnt[(int64_t)vi * n_pixels + ui] += 1;                                         \
+                {                                                                              \
+                    const double im_sign = -1.00;                                              \
+                    KB_CONVOLVE(ACC_T)                                                         \
+                }                                                                              \
+                cnt[(int64_t)vc * n_pixels + uc] += 1;                                         \
+                n_gridded++;                                                                   \
+            }                                                                                  \
+            if (uniform) {                                                                     \
+                for (int64_t i = 0; i < grid_size; i++) {                                      \
+                    g[2 * i] = g[2 * i] / cnt[i];                                              \
+                    g[2 * i + 1] = g[2 * i + 1] / cnt[i];                                      \
+                }                                                                              \
+            }                                                                                  \
+        }                                                                                      \
+    }                                                                                          \
+    free(table);                                                                               \
+    return n_gridded;
+
+int64_t oracle_gridding_kb_f32(const float* vis, int n_ant, int n_ch, int n_t,
+                               const float* u_m, const float* v_m, const int* bl_flags,
+                               const double* freq_hz, double delta_u, double delta_v,
+                               double min_uv, int n_pixels, int pol, int uniform, int K,
+                               int with_sinc, float* grids, float* counts) {
+    GRIDDING_KB_BODY(float)
+}
+
+int64_t oracle_gridding_kb_f64(const float* vis, int n_ant, int n_ch, int n_t,
+                               const float* u_m, const float* v_m, const int* bl_flags,
+                               const double* freq_hz, double delta_u, double delta_v,
+                               double min_uv, int n_pixels, int pol, int uniform, int K,
+                               int with_sinc, double* grids, double* counts) {
+    GRIDDING_KB_BODY(double)
+}
+
+/* ------------------------------------------------------------------------ */
 /* FFT: recursive mixed-radix decimation in time (Cooley-Tukey), out of     */
 /* place, with the sub-transform combination done as a direct DFT of the    */
 /* radix.  Stands in for FFTW's fftwf_plan_many_dft(..., FFTW_BACKWARD)     */

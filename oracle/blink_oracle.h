@@ -110,6 +110,27 @@ int64_t oracle_gridding_f64(const float* vis, int n_ant, int n_ch, int n_t,
                             double min_uv, int n_pixels, int pol, int uniform,
                             double* grids, double* counts);
 
+/* Gridding kernels of the convolutional mode (src/pacer_imager.cpp:566-651):
+ * table[r * K + c] = _griddingKernels[r][c], r < 1023 (oversampled
+ * fractional cell position), c < K.  with_sinc selects KaiserBesselKernel
+ * (1) or KaiserBesselWithoutSinc (0). */
+void oracle_kb_kernels(int K, int with_sinc, double* table);
+
+/* CPacerImager::gridding with m_nConvolvingKernelSize = K > 0
+ * (src/pacer_imager.cpp:378-381,398-401, convolve_with_kernel :654-714,
+ * calc_xy_grid2 :716-750).  Counts stay nearest-cell.  Same arguments as
+ * oracle_gridding_*; returns the number of gridded visibilities. */
+int64_t oracle_gridding_kb_f32(const float* vis, int n_ant, int n_ch, int n_t,
+                               const float* u_m, const float* v_m, const int* bl_flags,
+                               const double* freq_hz, double delta_u, double delta_v,
+                               double min_uv, int n_pixels, int pol, int uniform, int K,
+                               int with_sinc, float* grids, float* counts);
+int64_t oracle_gridding_kb_f64(const float* vis, int n_ant, int n_ch, int n_t,
+                               const float* u_m, const float* v_m, const int* bl_flags,
+                               const double* freq_hz, double delta_u, double delta_v,
+                               double min_uv, int n_pixels, int pol, int uniform, int K,
+                               int with_sinc, double* grids, double* counts);
+
 /* Batched 2-D complex FFT, unnormalised, out of place (in != out).
  * sign = +1 is FFTW_BACKWARD (exp(+2 pi i jk/N)), the direction the imager
  * uses (src/pacer_imager.cpp:150-152); sign = -1 is forward.  Mixed radix

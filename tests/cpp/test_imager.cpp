@@ -180,11 +180,75 @@ void test_error_paths() {
     imager.m_nConvolvingKernelSize = 7;
     thrown = false;
     try {
+        imager.run_batched(xcorr);
+    } catch (const std::invalid_argument&) {
+        thrown = true;
+    }
+    if (!thrown) throw TestFailed("run_batched must refuse convolutional gridding");
+    imager._gridMode = GridMode::GaussianKernel;
+    thrown = false;
+    try {
         imager.run(xcorr);
     } catch (const std::runtime_error&) {
         thrown = true;
     }
-    if (!thrown) throw TestFailed("convolutional gridding must be refused");
+    if (!thrown) throw TestFailed("kernels other than Kaiser-Bessel must be refused");
+}
+
+// tests/conv_test.cpp:22-36 runs the CPU imager with m_nConvolvingKernelSize =
+// 10 on MWA data and only writes FITS files; here run() with the Kaiser-Bessel
+// kernel (sizes 7 and 10, with and without the sinc, image averaging on) is
+// compared with the oracle's restatement of the convolutional gridding.
+void test_imager_kaiser_bessel() {
+    const int n_ch = 2, n_t = 2, N = 128;
+    Array a = make_array(40, 17.5, 31);
+    std::vector<double> freq{150.625e6, 151.25e6};  // exact in binary, as get_frequency_hz gives them
+    ObservationInfo info;
+    Visibilities xcorr0 = make_vis(a, n_ch, n_t, freq, 3, info);
+    std::vector<float> vis0((float*)xcorr0.data(), (float*)xcorr0.data() + 2 * xcorr0.size());
+    const int n_bl = a.n_ant * (a.n_ant + 1) / 2;
+    const size_t gs = (size_t)N * N, n_img = (size_t)n_ch * n_t;
+    const int sizes[] = {7, 10};
+    const GridMode modes[] = {GridMode::KaiserBesselKernel, GridMode::KaiserBesselWithoutSinc};
+    for (int K : sizes)
+        for (GridMode mode : modes) {
+            CPacerImagerMI355X imager("", N, {5}, true, Polarization::XX, 2.0f);
+            imager.SetAntennaPositions(a.x, a.y, a.z, a.cable);
+            imager.m_fFrequencyMHz = -1;
+            imager.m_nConvolvingKernelSize = K;
+            imager._gridMode = mode;
+            Visibilities xcorr(MemoryBuffer<std::complex<float>>(xcorr0.size()), info, 1, 1);
+            std::memcpy((void*)xcorr.data(), vis0.data(), sizeof(float) * vis0.size());
+            xcorr.obsInfo.frequencyResolution = 0.625;
+            xcorr.obsInfo.coarseChannelBandwidth = 1.25;
+            xcorr.obsInfo.coarseChannel = 121;  // 150.625 MHz + 0.625 ch
+            Images avg = imager.run(xcorr);
+            if (avg.size() != 1) throw TestFailed("average_images must return one image");
+            // oracle: corrections, convolutional gridding, image, complex average
+            std::vector<float> u(n_bl), v(n_bl), w(n_bl);
+            oracle_zenith_uvw(a.n_ant, a.x.data(), a.y.data(), a.z.data(), u.data(), v.data(),
+                              w.data());
+            std::vector<int> flags(n_bl);
+            oracle_baseline_flags(a.n_ant, imager.m_FlaggedAntennas.data(),
+                                  (int)imager.m_FlaggedAntennas.size(), flags.data());
+            std::vector<float> vis(vis0), grids(2 * gs * n_img), counts(gs * n_img),
+                images(2 * gs * n_img), ref(2 * gs);
+            oracle_geometric_corrections(vis.data(), a.n_ant, n_ch, n_t, w.data(), freq.data());
+            oracle_cable_corrections(vis.data(), a.n_ant, n_ch, n_t, a.cable.data(), freq.data());
+            oracle_gridding_kb_f32(vis.data(), a.n_ant, n_ch, n_t, u.data(), v.data(),
+                                   flags.data(), freq.data(), imager.delta_u, imager.delta_v,
+                                   imager.min_uv, N, ORACLE_POL_XX, 0, K,
+                                   mode == GridMode::KaiserBesselKernel, grids.data(),
+                                   counts.data());
+            oracle_image_f32(grids.data(), counts.data(), N, (int)n_img, images.data(), 0);
+            oracle_image_averaging_complex(images.data(), gs, n_img, ref.data());
+            avg.to_cpu();
+            const double err = peak_normalised_error(avg.data(),
+                                                     (const std::complex<float>*)ref.data(), gs);
+            std::cout << "Kaiser-Bessel K=" << K << (mode == GridMode::KaiserBesselKernel ? " sinc" : "")
+                      << ": peak-normalised error of the average vs oracle " << err << std::endl;
+            if (!(err < 1e-5)) throw TestFailed("Kaiser-Bessel run differs from the oracle");
+        }
 }
 
 int main() {
@@ -194,6 +258,7 @@ int main() {
         RUN_TEST(test_imager_run);
         RUN_TEST(test_imager_batched_and_pols);
         RUN_TEST(test_error_paths);
+        RUN_TEST(test_imager_kaiser_bessel);
     } catch (std::exception& ex) {
         std::cerr << ex.what() << std::endl;
         return 1;

@@ -364,3 +364,127 @@ def test_frequency_cotter():
     assert abs((f1 - f0) - 40e3) < 1e-3
     g0 = oracle.frequency_hz(0.04, 1, 1.28, 133, 0, False)
     assert abs((g0 - f0) - 20e3) < 1e-3
+
+
+# ------------------------------------------ convolutional (Kaiser-Bessel) mode
+
+def _kb_1d(table):
+    """The 1-D kernel of K * 1023 samples the table rows were cut from
+    (row r, column c = sample c*1023 + 1022 - r)."""
+    K = table.shape[1]
+    k1 = np.empty(K * 1023)
+    for r in range(1023):
+        k1[np.arange(K) * 1023 + (1022 - r)] = table[r]
+    return k1
+
+
+@pytest.mark.parametrize("K", [1, 5, 7, 11])
+@pytest.mark.parametrize("with_sinc", [True, False])
+def test_kb_kernel_is_kaiser_window_times_sinc(K, with_sinc):
+    """Against scipy's Kaiser window (beta 8.6, an independent I0): sample m
+    of the 1-D kernel is 1023 * kaiser[m] * sinc(m - mid) (odd K)."""
+    from scipy.signal.windows import kaiser
+    t = oracle.kb_kernels(K, with_sinc)
+    k1 = _kb_1d(t)
+    n = K * 1023
+    mid = n // 2
+    d = np.abs(np.arange(n) - mid).astype(np.float64)
+    ratio = 1.0 / 1023
+    sinc = np.where(d == 0, ratio, np.sin(np.pi * ratio * d) / (np.pi * np.maximum(d, 1)))
+    if not with_sinc:
+        sinc = np.full(n, ratio)
+    expect = 1023 * kaiser(n, 8.6, sym=True) * sinc
+    assert np.allclose(k1, expect, rtol=1e-7, atol=1e-12)
+
+
+@pytest.mark.parametrize("K", [5, 7, 10])
+def test_kb_kernel_rows(K):
+    t = oracle.kb_kernels(K, True)
+    # mirror: row r read backwards is row 1022 - r
+    assert np.array_equal(t[::-1, ::-1], t)
+    if K % 2:
+        # zero fractional offset (row 511): the windowed sinc interpolates,
+        # weight 1 at the centre cell and 0 at the others
+        c = t[511]
+        assert abs(c[K // 2] - 1.0) < 1e-12
+        assert np.max(np.abs(np.delete(c, K // 2))) < 1e-15
+
+
+def _kb_obs(u_cells, v_cells, vis_xx, n_pixels=64):
+    """Single-channel observation at lambda = 1 m with delta_u = delta_v = 1,
+    so u_m / lambda / delta_u is the given (possibly fractional) cell."""
+    n_bl = len(u_cells)
+    n_ant = int((np.sqrt(8 * n_bl + 1) - 1) / 2)
+    assert n_ant * (n_ant + 1) // 2 == n_bl
+    vis = np.zeros((1, 1, n_bl, 4), np.complex64)
+    vis[0, 0, :, 0] = vis_xx
+    freq = np.array([oracle.VEL_LIGHT if hasattr(oracle, "VEL_LIGHT") else 299792458.0])
+    u = np.asarray(u_cells, np.float32)
+    v = np.asarray(v_cells, np.float32)
+    flags = np.zeros(n_bl, np.int32)
+    return vis, n_ant, u, v, flags, freq, n_pixels
+
+
+def test_kb_at_cell_centres_equals_nearest_neighbour():
+    rng = np.random.default_rng(3)
+    n_bl = 10
+    u = rng.integers(-20, 20, n_bl)
+    v = rng.integers(-20, 20, n_bl)
+    val = (rng.normal(size=n_bl) + 1j * rng.normal(size=n_bl)).astype(np.complex64)
+    vis, n_ant, u, v, flags, freq, N = _kb_obs(u, v, val)
+    g_nn, c_nn, _ = oracle.gridding(vis, n_ant, u, v, flags, freq, 1.0, 1.0, -np.inf, N,
+                                    precision="f64")
+    g_kb, c_kb, _ = oracle.gridding(vis, n_ant, u, v, flags, freq, 1.0, 1.0, -np.inf, N,
+                                    precision="f64", kernel_size=7)
+    assert np.array_equal(c_nn, c_kb)
+    assert np.max(np.abs(g_kb - g_nn)) < 1e-12 * np.max(np.abs(g_nn))
+
+
+def test_kb_grid_hermitian_and_shift_equivariant():
+    """One visibility at a fractional cell: the grid is Hermitian (the
+    conjugate footprint mirrors the visibility's) and moving the visibility
+    by whole cells moves its footprint by as many cells."""
+    N = 64
+    u0, v0 = 5.3, -7.42
+    val = np.complex64(1.5 - 0.25j)
+    vis, n_ant, u, v, flags, freq, _ = _kb_obs([u0], [v0], [val], N)
+    g, c, n = oracle.gridding(vis, n_ant, u, v, flags, freq, 1.0, 1.0, -np.inf, N,
+                              precision="f64", kernel_size=7)
+    assert n == 1
+    g = g[0, 0]
+    mirrored = np.conj(np.roll(np.roll(g[::-1, ::-1], 1, axis=0), 1, axis=1))
+    assert np.max(np.abs(g - mirrored)) < 1e-12
+    # footprint: 7 x 7 cells around (5, -7) and their mirror
+    assert np.count_nonzero(np.abs(g) > 0) == 2 * 49
+    vis2, n_ant, u2, v2, flags, freq, _ = _kb_obs([u0 + 3], [v0 - 2], [val], N)
+    g2, _, _ = oracle.gridding(vis2, n_ant, u2, v2, flags, freq, 1.0, 1.0, -np.inf, N,
+                               precision="f64", kernel_size=7)
+    # the visibility's own footprint moves by (+3, -2); compare away from the mirror
+    sl = np.zeros((N, N), bool)
+    for j in range(-3, 4):
+        for i in range(-3, 4):
+            sl[(-7 + j) % N, (5 + i) % N] = True
+    moved = np.roll(np.roll(np.where(sl, g, 0), -2, axis=0), 3, axis=1)
+    assert np.max(np.abs(np.where(np.roll(np.roll(sl, -2, 0), 3, 1), g2[0, 0], 0) - moved)) < 1e-6
+
+
+def test_kb_counts_and_weight_sum():
+    """Counts stay nearest-cell; every footprint carries V times the product
+    of the two kernel rows' sums (the table row chosen by the fractional
+    position, 1023 rows)."""
+    N = 64
+    u0, v0 = 2.25, 3.6
+    val = np.complex64(2.0 + 1.0j)
+    vis, n_ant, u, v, flags, freq, _ = _kb_obs([u0], [v0], [val], N)
+    for K in (5, 7):
+        g, c, _ = oracle.gridding(vis, n_ant, u, v, flags, freq, 1.0, 1.0, -np.inf, N,
+                                  precision="f64", kernel_size=K)
+        _, c_nn, _ = oracle.gridding(vis, n_ant, u, v, flags, freq, 1.0, 1.0, -np.inf, N,
+                                     precision="f64")
+        assert np.array_equal(c, c_nn)
+        t = oracle.kb_kernels(K, True)
+        rx = (int(np.round((u0 - round(u0)) * 1023)) + 1534) % 1023
+        ry = (int(np.round((v0 - round(v0)) * 1023)) + 1534) % 1023
+        s = t[rx].sum() * t[ry].sum()
+        # total = V s + conj(V) s
+        assert abs(g[0, 0].sum() - 2 * val.real * s) < 1e-9
