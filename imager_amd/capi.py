@@ -39,7 +39,7 @@ MAX_PIXELS = 4096
 EXPORTED_SYMBOLS = (
     "blink_create", "blink_destroy", "blink_last_error", "blink_version", "blink_fft_supported",
     "blink_apply_geometric_corrections", "blink_apply_cable_corrections", "blink_gridding",
-    "blink_gridding_kernel",
+    "blink_gridding_kernel", "blink_zenith_uvw", "blink_baseline_flags", "blink_uv_max",
     "blink_uv_cells", "blink_fft2d_c2c", "blink_vector_sum", "blink_fft_shift_and_norm",
     "blink_image", "blink_image_averaging", "blink_image_batch", "blink_release_cache",
     "blink_profile_enable", "blink_profile_get", "blink_profile_kernels", "blink_profile_reset",
@@ -101,6 +101,9 @@ def load():
         "blink_gridding": (i, [vp, vp, i, i, i, vp, vp, vp, vp, d, d, d, i, i, i, i, vp, vp, vp]),
         "blink_gridding_kernel": (i, [vp, vp, i, i, i, vp, vp, vp, vp, d, d, d, i, i, i, i, i,
                                       vp, vp, vp]),
+        "blink_zenith_uvw": (i, [vp, vp, vp, vp, i, vp, vp, vp, vp]),
+        "blink_baseline_flags": (i, [vp, vp, i, vp, vp]),
+        "blink_uv_max": (i, [vp, vp, vp, vp, i, vp, vp]),
         "blink_uv_cells": (i, [vp, vp, vp, vp, i, vp, i, d, d, d, i, vp, vp]),
         "blink_fft2d_c2c": (i, [vp, vp, vp, i, i, i, vp]),
         "blink_vector_sum": (i, [vp, vp, i64, i, vp, vp]),
@@ -225,6 +228,19 @@ class Context:
             _ptr(freq_hz), float(delta_u), float(delta_v), float(min_uv), int(n_pixels),
             POL[pol] if isinstance(pol, str) else int(pol), int(weighting), int(kernel_size),
             int(kernel_mode), _ptr(grids), _ptr(counts), _stream(stream)))
+
+    # array geometry
+    def zenith_uvw(self, x, y, z, n_ant, u_m, v_m, w_m, stream=None):
+        self._check(load().blink_zenith_uvw(self._h, _ptr(x), _ptr(y), _ptr(z), int(n_ant),
+                                            _ptr(u_m), _ptr(v_m), _ptr(w_m), _stream(stream)))
+
+    def baseline_flags(self, ant_flags, n_ant, bl_flags, stream=None):
+        self._check(load().blink_baseline_flags(self._h, _ptr(ant_flags), int(n_ant),
+                                                _ptr(bl_flags), _stream(stream)))
+
+    def uv_max(self, u_m, v_m, bl_flags, n_bl, out, stream=None):
+        self._check(load().blink_uv_max(self._h, _ptr(u_m), _ptr(v_m), _ptr(bl_flags), int(n_bl),
+                                        _ptr(out), _stream(stream)))
 
     def uv_cells(self, u_m, v_m, bl_flags, n_bl, freq_hz, n_ch, delta_u, delta_v, min_uv,
                  n_pixels, cells, stream=None):

@@ -470,6 +470,47 @@ BLINK_API int blink_gridding(blink_ctx* ctx, const float* vis, int n_ant, int n_
     return BLINK_OK;
 }
 
+// ------------------------------------------------------------ array geometry
+
+BLINK_API int blink_zenith_uvw(blink_ctx* ctx, const double* x, const double* y,
+                               const double* z, int n_ant, float* u_m, float* v_m, float* w_m,
+                               void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(x && y && z && u_m && v_m && w_m, "null pointer argument");
+    CHECK_ARG(n_ant >= 1 && (int64_t)n_ant * (n_ant + 1) / 2 <= kMaxBaselines,
+              "n_ant=%d out of range", n_ant);
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH("zenith_uvw", s, launch_zenith_uvw(x, y, z, n_ant, u_m, v_m, w_m, s));
+    return BLINK_OK;
+}
+
+BLINK_API int blink_baseline_flags(blink_ctx* ctx, const int* ant_flags, int n_ant,
+                                   int* bl_flags, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(ant_flags && bl_flags, "null pointer argument");
+    CHECK_ARG(n_ant >= 1 && (int64_t)n_ant * (n_ant + 1) / 2 <= kMaxBaselines,
+              "n_ant=%d out of range", n_ant);
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH("baseline_flags", s, launch_baseline_flags(ant_flags, n_ant, bl_flags, s));
+    return BLINK_OK;
+}
+
+BLINK_API int blink_uv_max(blink_ctx* ctx, const float* u_m, const float* v_m,
+                           const int* bl_flags, int n_bl, double* uv_max, void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    CHECK_ARG(u_m && v_m && bl_flags && uv_max, "null pointer argument");
+    CHECK_ARG(n_bl >= 0 && n_bl <= kMaxBaselines, "n_bl=%d out of range", n_bl);
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH("uv_max", s, launch_uv_max(u_m, v_m, bl_flags, n_bl, uv_max, s));
+    return BLINK_OK;
+}
+
 // Modified Bessel function I0 by its power series sum (x/2)^2m / (m!)^2,
 // summed until a term falls to `rel` of the sum (the stopping rule of the
 // reference's bessel0, src/pacer_imager.cpp:566-578).

@@ -127,6 +127,84 @@ hipError_t launch_uv_cells(const float* u_m, const float* v_m, const int* flags,
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ array geometry
+
+__global__ void k_zenith_uvw(const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, int n_bl, float* __restrict__ u_m,
+                             float* __restrict__ v_m, float* __restrict__ w_m) {
+    const int bl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bl >= n_bl) return;
+    unsigned a1, a2;
+    baseline_antennas((unsigned)bl, a1, a2);
+    u_m[bl] = (float)(-(x[a1] - x[a2]));
+    v_m[bl] = (float)(-(y[a1] - y[a2]));
+    w_m[bl] = (float)(-(z[a1] - z[a2]));
+}
+
+__global__ void k_baseline_flags(const int* __restrict__ ant_flags, int n_bl,
+                                 int* __restrict__ bl_flags) {
+    const int bl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bl >= n_bl) return;
+    unsigned a1, a2;
+    baseline_antennas((unsigned)bl, a1, a2);
+    bl_flags[bl] = (a1 == a2 || ant_flags[a1] || ant_flags[a2]) ? 1 : 0;
+}
+
+// One workgroup: max |u|, |v| over unflagged baselines (exact in double).
+__global__ void __launch_bounds__(1024)
+k_uv_max(const float* __restrict__ u_m, const float* __restrict__ v_m,
+         const int* __restrict__ flags, int n_bl, double* __restrict__ uv_max) {
+    __shared__ double red[2][16];
+    double um = -INFINITY, vm = -INFINITY;
+    for (int i = threadIdx.x; i < n_bl; i += blockDim.x) {
+        if (flags[i]) continue;
+        um = fmax(um, fabs((double)u_m[i]));
+        vm = fmax(vm, fabs((double)v_m[i]));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        um = fmax(um, __shfl_down(um, o, 64));
+        vm = fmax(vm, __shfl_down(vm, o, 64));
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wid] = um;
+        red[1][wid] = vm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            um = fmax(um, red[0][w]);
+            vm = fmax(vm, red[1][w]);
+        }
+        uv_max[0] = um;
+        uv_max[1] = vm;
+    }
+}
+
+hipError_t launch_zenith_uvw(const double* x, const double* y, const double* z, int n_ant,
+                             float* u_m, float* v_m, float* w_m, hipStream_t stream) {
+    const int n_bl = n_ant * (n_ant + 1) / 2;
+    if (n_bl == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zenith_uvw, dim3((n_bl + 255) / 256), dim3(256), 0, stream, x, y, z,
+                       n_bl, u_m, v_m, w_m);
+    return hipGetLastError();
+}
+
+hipError_t launch_baseline_flags(const int* ant_flags, int n_ant, int* bl_flags,
+                                 hipStream_t stream) {
+    const int n_bl = n_ant * (n_ant + 1) / 2;
+    if (n_bl == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_baseline_flags, dim3((n_bl + 255) / 256), dim3(256), 0, stream,
+                       ant_flags, n_bl, bl_flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_uv_max(const float* u_m, const float* v_m, const int* flags, int n_bl,
+                         double* uv_max, hipStream_t stream) {
+    hipLaunchKernelGGL(k_uv_max, dim3(1), dim3(1024), 0, stream, u_m, v_m, flags, n_bl, uv_max);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------ (b) gridding, plain atomic
 
 // One thread per (t, ch, bl), FP32 global atomics (the plain path, in the

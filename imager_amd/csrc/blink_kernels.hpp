@@ -105,6 +105,12 @@ hipError_t launch_grid_atomic(const float* vis, int n_ant, int n_ch, int n_t, co
                               const float* v_m, const int* flags, const double* freq_hz, double du,
                               double dv, double min_uv, int n_pixels, int pol, float* grids,
                               float* counts, hipStream_t stream);
+hipError_t launch_zenith_uvw(const double* x, const double* y, const double* z, int n_ant,
+                             float* u_m, float* v_m, float* w_m, hipStream_t stream);
+hipError_t launch_baseline_flags(const int* ant_flags, int n_ant, int* bl_flags,
+                                 hipStream_t stream);
+hipError_t launch_uv_max(const float* u_m, const float* v_m, const int* flags, int n_bl,
+                         double* uv_max, hipStream_t stream);
 // Convolutional gridding with a kernel table of 1023 rows of kernel_size
 // doubles (row = oversampled fractional cell position).
 hipError_t launch_grid_kb(const float* vis, int n_ant, int n_ch, int n_t, const float* u_m,

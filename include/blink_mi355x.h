@@ -149,6 +149,29 @@ BLINK_API int blink_gridding_kernel(blink_ctx* ctx, const float* vis, int n_ant,
                                     int weighting, int kernel_size, int kernel_mode,
                                     float* grids, float* counts, void* stream);
 
+/* ------------------------------------------------------ array geometry */
+
+/* Zenith UVW of every baseline bl = a1(a1+1)/2 + a2 (a2 <= a1):
+ * u = -(x[a1] - x[a2]), v, w likewise, formed in double and stored as FP32
+ * metres (CAntennaPositions::CalculateUVW, no-metafits branch,
+ * src/antenna_positions.cpp:81-89).  x, y, z: double[n_ant] metres. */
+BLINK_API int blink_zenith_uvw(blink_ctx* ctx, const double* x, const double* y,
+                               const double* z, int n_ant, float* u_m, float* v_m, float* w_m,
+                               void* stream);
+
+/* Baseline flags from per-antenna flags (non-zero = flagged): a baseline is
+ * flagged when either antenna is, or when it is an auto-correlation
+ * (CPacerImager::UpdateFlags, src/pacer_imager.cpp:60-86). */
+BLINK_API int blink_baseline_flags(blink_ctx* ctx, const int* ant_flags, int n_ant,
+                                   int* bl_flags, void* stream);
+
+/* uv_max[0] = max |u|, uv_max[1] = max |v| over unflagged baselines, as
+ * doubles (-inf when every baseline is flagged), the inputs of
+ * delta_u = osf * u_max / N (CPacerImager::CalculateUVW / grid(),
+ * src/pacer_imager.cpp:213-221,497-499). */
+BLINK_API int blink_uv_max(blink_ctx* ctx, const float* u_m, const float* v_m,
+                           const int* bl_flags, int n_bl, double* uv_max, void* stream);
+
 /* Cell indices of every (channel, baseline): cells[(ch*n_bl + bl)*4 + k] =
  * u, v, u_conj, v_conj, or -1 for flagged / min_uv-cut baselines.  The
  * index computation of calculate_pos (src/gpu/gridding_gpu.cpp:24-46);

@@ -343,6 +343,36 @@ def test_gridding_kernel_matches_oracle(ctx, n, K, mode, pol, uniform):
     assert err < 1e-6
 
 
+@pytest.mark.parametrize("n_ant,flagged", [(1, ()), (48, (0, 7, 47)), (256, (3,)),
+                                            (512, ())])
+def test_array_geometry_matches_oracle(ctx, n_ant, flagged):
+    """Zenith UVW, baseline flags and u_max / v_max on the GPU, bit for bit
+    with the oracle (CalculateUVW, UpdateFlags)."""
+    import torch
+    rng = np.random.default_rng(n_ant)
+    x = rng.normal(0, 300.0, n_ant)
+    y = rng.normal(0, 300.0, n_ant)
+    z = rng.normal(0, 2.0, n_ant)
+    n_bl = n_ant * (n_ant + 1) // 2
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    u = torch.empty(n_bl, dtype=torch.float32, device="cuda")
+    v = torch.empty_like(u)
+    w = torch.empty_like(u)
+    ctx.zenith_uvw(dev(x), dev(y), dev(z), n_ant, u, v, w)
+    ur, vr, wr = oracle.zenith_uvw(x, y, z)
+    assert np.array_equal(_host(u), ur) and np.array_equal(_host(v), vr)
+    assert np.array_equal(_host(w), wr)
+    ant = np.zeros(n_ant, np.int32)
+    ant[list(flagged)] = 1
+    fl = torch.empty(n_bl, dtype=torch.int32, device="cuda")
+    ctx.baseline_flags(dev(ant), n_ant, fl)
+    assert np.array_equal(_host(fl), oracle.baseline_flags(n_ant, list(flagged)))
+    out = torch.empty(2, dtype=torch.float64, device="cuda")
+    ctx.uv_max(u, v, fl, n_bl, out)
+    um, vm = oracle.uv_max(ur, vr, oracle.baseline_flags(n_ant, list(flagged)))
+    assert _host(out)[0] == um and _host(out)[1] == vm
+
+
 # -------------------------------------------------------------- batched
 
 def _batch(ctx, o, corrections=3, pol="XX"):
