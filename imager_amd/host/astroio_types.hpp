@@ -142,6 +142,17 @@ struct ObservationInfo {
     std::string metadata_file;
 };
 
+// The 2-D primary image of a FITS file (the part of msfitslib's CBgFits the
+// imager's inputs use): BITPIX 8, 16, 32, 64, -32 or -64, NAXIS = 2, BSCALE /
+// BZERO applied.  data[y * nx + x]; getXY(x, y) as CBgFits::getXY.
+struct FitsImage {
+    long nx = 0, ny = 0;
+    std::vector<double> data;
+    double getXY(long x, long y) const { return data[(size_t)y * (size_t)nx + (size_t)x]; }
+    long GetXSize() const { return nx; }
+    long GetYSize() const { return ny; }
+};
+
 // Cross-correlations [t][ch][bl][4 pol products], bl = a1(a1+1)/2 + a2, a2 <= a1.
 class Visibilities : public MemoryBuffer<std::complex<float>> {
    public:

@@ -474,6 +474,113 @@ Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
                   m_MetaData.decDegs, pixscale_ra, pixscale_dec};
 }
 
+// ------------------------------------------------------------ FITS input
+
+FitsImage read_fits_image(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open FITS file " + path);
+    long bitpix = 0, naxis = -1, nx = 0, ny = 0;
+    double bscale = 1.0, bzero = 0.0;
+    bool end = false;
+    size_t header_bytes = 0;
+    char block[2880];
+    while (!end) {
+        if (!f.read(block, sizeof block)) throw std::runtime_error("truncated FITS header: " + path);
+        header_bytes += sizeof block;
+        for (int c = 0; c < 36 && !end; c++) {
+            const std::string cardtxt(block + 80 * c, 80);
+            const std::string key = cardtxt.substr(0, 8);
+            if (key.compare(0, 3, "END") == 0 && key.find_first_not_of(' ', 3) == std::string::npos) {
+                end = true;
+                break;
+            }
+            if (cardtxt.compare(8, 2, "= ") != 0) continue;
+            std::string val = cardtxt.substr(10);
+            const size_t slash = val.find('/');
+            if (slash != std::string::npos) val = val.substr(0, slash);
+            auto num = [&]() { return std::strtod(val.c_str(), nullptr); };
+            if (key == "BITPIX  ") bitpix = (long)num();
+            else if (key == "NAXIS   ") naxis = (long)num();
+            else if (key == "NAXIS1  ") nx = (long)num();
+            else if (key == "NAXIS2  ") ny = (long)num();
+            else if (key == "BSCALE  ") bscale = num();
+            else if (key == "BZERO   ") bzero = num();
+        }
+    }
+    if (naxis != 2 || nx <= 0 || ny <= 0)
+        throw std::runtime_error("FITS file " + path + " is not a 2-D image");
+    const int bytes = bitpix == 8 ? 1 : bitpix == 16 ? 2 : (bitpix == 32 || bitpix == -32) ? 4
+                      : (bitpix == 64 || bitpix == -64) ? 8 : 0;
+    if (!bytes) throw std::runtime_error("unsupported BITPIX in " + path);
+    const size_t n = (size_t)nx * (size_t)ny;
+    std::vector<unsigned char> raw(n * bytes);
+    if (!f.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)raw.size()))
+        throw std::runtime_error("truncated FITS data: " + path);
+    FitsImage img;
+    img.nx = nx;
+    img.ny = ny;
+    img.data.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        uint64_t b = 0;  // big-endian sample
+        for (int k = 0; k < bytes; k++) b = (b << 8) | raw[i * bytes + k];
+        double v;
+        if (bitpix == -64) {
+            std::memcpy(&v, &b, 8);
+        } else if (bitpix == -32) {
+            const uint32_t b32 = (uint32_t)b;
+            float x;
+            std::memcpy(&x, &b32, 4);
+            v = x;
+        } else if (bitpix == 8) {
+            v = (double)(uint8_t)b;
+        } else if (bitpix == 16) {
+            v = (double)(int16_t)(uint16_t)b;
+        } else if (bitpix == 32) {
+            v = (double)(int32_t)(uint32_t)b;
+        } else {
+            v = (double)(int64_t)b;
+        }
+        img.data[i] = bzero + bscale * v;
+    }
+    (void)header_bytes;
+    return img;
+}
+
+Visibilities visibilities_from_correlation_matrices(const FitsImage& re, const FitsImage& im,
+                                                    const FitsImage* re_yy,
+                                                    const FitsImage* im_yy,
+                                                    ObservationInfo& info) {
+    const long n_ant = re.GetXSize();
+    auto square = [&](const FitsImage& m) { return m.nx == n_ant && m.ny == n_ant; };
+    if (n_ant <= 0 || !square(re) || !square(im) || (re_yy && !square(*re_yy)) ||
+        (im_yy && !square(*im_yy)))
+        throw std::invalid_argument("correlation matrices must be n_ant x n_ant of one size");
+    // the command line's defaults (apps/pacer_imager_main.cpp:736-746)
+    info.nAntennas = (unsigned)n_ant;
+    info.nFrequencies = 1;
+    info.nPolarizations = 2;
+    info.nTimesteps = 1;
+    info.timeResolution = 1.0;
+    info.frequencyResolution = 0.040;
+    info.coarseChannelBandwidth = 1.28;
+    info.coarseChannel = 109;
+    info.startTime = 1534699078;
+    info.coarse_channel_index = 0;
+    const size_t n_bl = (size_t)n_ant * (n_ant + 1) / 2;
+    MemoryBuffer<std::complex<float>> buf(n_bl * 4);
+    std::memset((void*)buf.data(), 0, sizeof(std::complex<float>) * n_bl * 4);
+    Visibilities xcorr(std::move(buf), info, 1, 1);
+    xcorr.nFrequencies = 1;
+    const bool yy = re_yy && im_yy;
+    for (long i = 0; i < n_ant; i++)
+        for (long j = 0; j <= i; j++) {
+            std::complex<float>* rec = xcorr.at(0, 0, (size_t)i, (size_t)j);
+            rec[0] = std::complex<float>((float)re.getXY(j, i), (float)im.getXY(j, i));
+            if (yy) rec[3] = std::complex<float>((float)re_yy->getXY(j, i), (float)im_yy->getXY(j, i));
+        }
+    return xcorr;
+}
+
 // ------------------------------------------------------------ FITS output
 
 void Images::to_fits_files(const std::string& dir) {

@@ -49,6 +49,21 @@
 // FFT shift of one host image (src/pacer_imager.h:27), done on the GPU.
 void fft_shift(std::complex<float>* image, size_t image_x_side, size_t image_y_side);
 
+// Reads the 2-D primary image of a FITS file; throws std::runtime_error on a
+// missing file or an unsupported layout.
+FitsImage read_fits_image(const std::string& path);
+
+// Visibilities of one channel and one timestep from correlation matrices, as
+// the imager's command line builds them (ConvertFits2XCorr,
+// apps/pacer_imager_main.cpp:720-800): n_ant = the matrices' side; for every
+// antenna pair j <= i, XX = re.getXY(j, i) + i im.getXY(j, i) and YY likewise
+// from re_yy / im_yy when given (else 0); XY = YX = 0.  `info` gets the
+// command line's defaults for the fields the matrices do not carry.
+Visibilities visibilities_from_correlation_matrices(const FitsImage& re, const FitsImage& im,
+                                                    const FitsImage* re_yy,
+                                                    const FitsImage* im_yy,
+                                                    ObservationInfo& info);
+
 // One antenna of the array (the fields of the reference's InputMapping that
 // the path uses).
 struct InputMapping {

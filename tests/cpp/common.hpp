@@ -52,3 +52,16 @@ inline double peak_normalised_error(const std::complex<float>* a, const std::com
         fn();                                                            \
         std::cout << "'" #fn "' passed." << std::endl;                   \
     } while (0)
+
+// Directory of the committed fixtures (tests/golden), found from the path of
+// the running test program (tests/cpp/bin/<program>).
+#include <climits>
+#include <unistd.h>
+inline std::string golden_dir() {
+    char buf[PATH_MAX];
+    const ssize_t n = readlink("/proc/self/exe", buf, sizeof buf - 1);
+    if (n <= 0) return "tests/golden";
+    std::string p(buf, (size_t)n);
+    for (int k = 0; k < 3; k++) p = p.substr(0, p.find_last_of('/'));  // -> tests
+    return p + "/golden";
+}

@@ -251,6 +251,41 @@ void test_imager_kaiser_bessel() {
         }
 }
 
+// The simulated EDA2 correlation matrices the reference's command line images
+// (scripts/simul/eda2/20230205/templates, apps/pacer_imager_main.cpp:720-800),
+// read from FITS and imaged at 180^2 with the antenna file's positions, by
+// run() and by run_batched(), against the oracle's pipeline on the same input.
+void test_eda2_correlation_matrices() {
+    const std::string g = golden_dir() + "/eda2_chan204/";
+    FitsImage re = read_fits_image(g + "chan_204_20230205T034618_simul_vis_XX_CorrMatrix_RE.fits");
+    FitsImage im = read_fits_image(g + "chan_204_20230205T034618_simul_vis_XX_CorrMatrix_IM.fits");
+    ObservationInfo info;
+    Visibilities xcorr = visibilities_from_correlation_matrices(re, im, nullptr, nullptr, info);
+    std::vector<float> vis0((float*)xcorr.data(), (float*)xcorr.data() + 2 * xcorr.size());
+    CAntennaList ants;
+    if (ants.ReadAntennaPositions((g + "antenna_locations.txt").c_str()) != 256)
+        throw TestFailed("antenna_locations.txt must list 256 antennas");
+    Array a;
+    a.n_ant = 256;
+    for (const InputMapping& m : ants) {
+        a.x.push_back(m.x);
+        a.y.push_back(m.y);
+        a.z.push_back(m.z);
+        a.cable.push_back(m.cableLenDelta);
+    }
+    const std::vector<double> freq{159.375e6};  // channel 204 x 0.78125 MHz
+    for (int batched = 0; batched < 2; batched++) {
+        CPacerImagerMI355X imager(g + "antenna_locations.txt", 180, {}, false, Polarization::XX,
+                                  2.0f);
+        imager.m_fFrequencyMHz = 159.375;
+        Visibilities copy(MemoryBuffer<std::complex<float>>(xcorr.size()), info, 1, 1);
+        std::memcpy((void*)copy.data(), vis0.data(), sizeof(float) * vis0.size());
+        Images images = batched ? imager.run_batched(copy) : imager.run(copy);
+        check_against_oracle(imager, images, a, vis0, freq, 1, 1, true, true, ORACLE_POL_XX,
+                             1e-5, batched ? "EDA2 matrices, run_batched" : "EDA2 matrices, run");
+    }
+}
+
 int main() {
     CPacerImagerMI355X::SetDebugLevel(IMAGER_WARNING_LEVEL);
     try {
@@ -259,6 +294,7 @@ int main() {
         RUN_TEST(test_imager_batched_and_pols);
         RUN_TEST(test_error_paths);
         RUN_TEST(test_imager_kaiser_bessel);
+        RUN_TEST(test_eda2_correlation_matrices);
     } catch (std::exception& ex) {
         std::cerr << ex.what() << std::endl;
         return 1;
