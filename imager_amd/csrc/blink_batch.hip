@@ -97,7 +97,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     // of the tile; only LDS work waits at the barrier.
     constexpr int U = 4;
     const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
-    const float4* vis = g.vis + 2 * ((int64_t)img * g.n_bl);
+    const float2* rec = (const float2*)g.vis + 4 * ((int64_t)img * g.n_bl);
     GatherBatch<U> gb;
     gb.load_entries(ent, e0 + tid, e0, e1, NT);
     constexpr int TWK = (N + NT - 1) / NT;
@@ -107,7 +107,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         if (tid + k * NT < N) twv[k] = tw[tid + k * NT];
     const int rsv = tid <= RB ? rsg[r0 + min(tid, nr)] : 0;
     const double nu = g.freq_hz[ch];
-    gb.load_records(g, vis);
+    gb.load_records(g, rec);
     float2* twl = tile + RB * TS;  // twiddle table copy [N]
 #pragma unroll
     for (int k = 0; k < TWK; k++)
@@ -128,7 +128,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     for (int base = e0 + tid + NT * U; base < e1; base += NT * U) {
         GatherBatch<U> gm;
         gm.load_entries(ent, base, e0, e1, NT);
-        gm.load_records(g, vis);
+        gm.load_records(g, rec);
         n_ok += gm.consume(g, nu, sink);
     }
     const int tot = block_sum_int(n_ok, red);

@@ -691,6 +691,7 @@ struct SetScratch {
     int* row_start;
     uint32_t* entries;
     uint32_t* isums;
+    double* coef;  // [n_bl] phase coefficients
     int chunk;  // DFT sets: images per launch triple
 };
 
@@ -760,7 +761,8 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     // scratch sizes
     size_t rs_bytes = 0, en_bytes = 0, inter_bytes = 16;
     std::vector<size_t> rs_off(n_sets), en_off(n_sets), is_off(n_sets);
-    int64_t n_img_all = 0, n_units = 0, min_bl = INT64_MAX, max_bl = 0, n_cells = 0;
+    int64_t n_img_all = 0, n_units = 0, min_bl = INT64_MAX, max_bl = 0, n_cells = 0, n_bl_all = 0;
+    std::vector<size_t> co_off(n_sets);
     int n_max = 0;
     for (int i = 0; i < n_sets; i++) {
         const blink_image_set& st = sets[i];
@@ -774,6 +776,8 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         n_img_all += (int64_t)st.n_ch * st.n_t;
         n_units += st.n_ch;
         n_cells += (int64_t)st.n_ch * n_bl;
+        co_off[i] = (size_t)n_bl_all;
+        n_bl_all += n_bl;
         min_bl = std::min<int64_t>(min_bl, n_bl);
         max_bl = std::max<int64_t>(max_bl, n_bl);
         n_max = std::max(n_max, st.n_pixels);
@@ -804,7 +808,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     bp.isums_count = n_img_all;
     const size_t partial_bytes = sizeof(int) * (size_t)n_units * parts * n_max;
 
-    void *inter, *rs, *en, *is, *pa, *cl;
+    void *inter, *rs, *en, *is, *pa, *cl, *co;
     if ((rc = scratch(ctx, "batch_inter", inter_bytes, &inter))) return rc;
     if ((rc = scratch(ctx, "batch_rows", std::max<size_t>(rs_bytes, 16), &rs))) return rc;
     if ((rc = scratch(ctx, "batch_entries", std::max<size_t>(en_bytes, 16), &en))) return rc;
@@ -815,6 +819,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     if ((rc = scratch(ctx, "batch_plan_cells", std::max<size_t>(4 * (size_t)n_cells, 16), &cl)))
         return rc;
     bp.cells = (uint32_t*)cl;
+    if ((rc = scratch(ctx, "batch_bl_coef", std::max<size_t>(sizeof(double) * (size_t)n_bl_all, 16),
+                      &co)))
+        return rc;
     bp.inter = (float*)inter;
     bp.isums_all = (uint32_t*)is;
     bp.partial = (int*)pa;
@@ -823,6 +830,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         bp.sc[i].row_start = (int*)((char*)rs + rs_off[i]);
         bp.sc[i].entries = (uint32_t*)((char*)en + en_off[i]);
         bp.sc[i].isums = (uint32_t*)is + is_off[i];
+        bp.sc[i].coef = (double*)co + co_off[i];
         const int64_t per = 16 * (int64_t)sets[i].n_pixels * sets[i].n_pixels;
         const int n_img = sets[i].n_ch * sets[i].n_t;
         bp.sc[i].chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
@@ -846,8 +854,10 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             const blink_image_set& st = sets[i];
             SetDesc d{};
             d.g.vis = (const float4*)st.vis;
-            d.g.w_m = st.w_m;
-            d.g.cable_m = st.cable_m;
+            d.g.phase_coef = bp.sc[i].coef;
+            d.w_m = st.w_m;
+            d.cable_m = st.cable_m;
+            d.coef_out = bp.sc[i].coef;
             d.g.freq_hz = st.freq_hz;
             d.g.corrections = st.corrections & 3u;
             d.g.pol = st.pol;
@@ -1025,8 +1035,7 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const Batc
                 const int n_img = st.n_ch * st.n_t;
                 GatherArgs ga{};
                 ga.vis = (const float4*)st.vis;
-                ga.w_m = st.w_m;
-                ga.cable_m = st.cable_m;
+                ga.phase_coef = sc.coef;
                 ga.freq_hz = st.freq_hz;
                 ga.corrections = st.corrections & 3u;
                 ga.pol = st.pol;

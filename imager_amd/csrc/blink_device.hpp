@@ -117,6 +117,18 @@ __device__ __forceinline__ float2 select_pol(const float4 xx_xy, const float4 yx
     return make_float2((xx_xy.x + yx_yy.z) / 2.0f, (xx_xy.y + yx_yy.w) / 2.0f);  // I
 }
 
+// select_pol from the one or two products the polarisation needs: x0 = XX
+// (XX, I), YY (YY) or XY (V); x1 = YY (I) or YX (V).  Same arithmetic.
+__device__ __forceinline__ float2 select_pol2(float2 x0, float2 x1, int pol) {
+    if (pol <= 1) return x0;                                       // XX, YY
+    if (pol == 3) {                                                // V = (i/2)(XY - YX)
+        const float dre = x0.x - x1.x;
+        const float dim = x0.y - x1.y;
+        return make_float2(0.0f * dre - 0.5f * dim, 0.0f * dim + 0.5f * dre);
+    }
+    return make_float2((x0.x + x1.x) / 2.0f, (x0.y + x1.y) / 2.0f);  // I
+}
+
 __device__ __forceinline__ bool vis_ok(float2 v) {
     return !isnan(v.x) && !isnan(v.y) && (double)fabsf(v.x) < kMaxVis &&
            (double)fabsf(v.y) < kMaxVis;

@@ -21,9 +21,8 @@ template <int U>
 struct GatherBatch {
     int idx[U];
     uint32_t ev[U];
-    float4 a[U], b[U];
-    float wv[U];
-    double ca[U], cb[U];
+    float2 x0[U], x1[U];  // the one or two pol products the polarisation needs
+    double cf[U];         // phase coefficient of the baseline
 
     __device__ __forceinline__ void load_entries(const uint32_t* __restrict__ ent, int base,
                                                  int e0, int e1, int stride) {
@@ -35,20 +34,18 @@ struct GatherBatch {
         }
     }
 
-    __device__ __forceinline__ void load_records(const GatherArgs& g, const float4* vis) {
+    // rec: the image's records as float2 [bl][XX, XY, YX, YY]
+    __device__ __forceinline__ void load_records(const GatherArgs& g, const float2* rec) {
+        // XX: XX; YY: YY; I: XX and YY; V: XY and YX
+        const int p0 = g.pol == 1 ? 3 : (g.pol == 3 ? 1 : 0);
+        const int p1 = g.pol == 3 ? 2 : 3;
+        const bool two = g.pol >= 2;
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const unsigned bl = ev[u] >> 14;
-            a[u] = g.pol != 1 ? vis[2 * bl] : make_float4(0.f, 0.f, 0.f, 0.f);
-            b[u] = g.pol != 0 ? vis[2 * bl + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-            wv[u] = (g.corrections & 1u) ? g.w_m[bl] : 0.f;
-            ca[u] = cb[u] = 0.0;
-            if (g.corrections & 2u) {
-                unsigned a1, a2;
-                baseline_antennas(bl, a1, a2);
-                ca[u] = g.cable_m[a1];
-                cb[u] = g.cable_m[a2];
-            }
+            x0[u] = rec[4 * (size_t)bl + p0];
+            x1[u] = two ? rec[4 * (size_t)bl + p1] : make_float2(0.f, 0.f);
+            cf[u] = g.corrections ? g.phase_coef[bl] : 0.0;
         }
     }
 
@@ -58,13 +55,8 @@ struct GatherBatch {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (idx[u] < 0) continue;
-            float2 v = select_pol(a[u], b[u], g.pol);
-            if (g.corrections) {
-                double turns = 0.0;
-                if (g.corrections & 1u) turns += reduce_turns(geometric_turns(wv[u], nu));
-                if (g.corrections & 2u) turns += reduce_turns(cable_turns(cb[u] - ca[u], nu));
-                v = cmul(v, unit_phasor(turns));
-            }
+            float2 v = select_pol2(x0[u], x1[u], g.pol);
+            if (g.corrections) v = cmul(v, unit_phasor(((cf[u] * nu) / kVelLight)));
             if (!vis_ok(v)) continue;
             if (ev[u] & (1u << 13)) v.y = -v.y;
             total += sink(idx[u], (int)(ev[u] & 0x1fffu), v);
@@ -72,6 +64,21 @@ struct GatherBatch {
         return total;
     }
 };
+
+// Phase coefficient of baseline bl (metres): w - (L[a2] - L[a1]), each term
+// when its correction is on; exp(+2 pi i coef nu / c) is the product of the
+// geometric (src/gpu/corrections_gpu.cpp:61-63) and cable (:27-28) rotations.
+__device__ __forceinline__ double phase_coefficient(unsigned corrections, const float* w_m,
+                                                    const double* cable_m, unsigned bl) {
+    double c = 0.0;
+    if (corrections & 1u) c = (double)w_m[bl];
+    if (corrections & 2u) {
+        unsigned a1, a2;
+        baseline_antennas(bl, a1, a2);
+        c = c - (cable_m[a2] - cable_m[a1]);
+    }
+    return c;
+}
 
 // Gathers the plan entries [e0, e1) of one image, `stride` threads apart, U
 // entries per thread at a time (U independent load chains in flight).
@@ -81,11 +88,11 @@ __device__ __forceinline__ int gather_range(const GatherArgs& g, int64_t img, in
                                             int tid, int stride, Sink sink) {
     int total = 0;
     const double nu = g.freq_hz[ch];
-    const float4* vis = g.vis + 2 * (img * g.n_bl);
+    const float2* rec = (const float2*)g.vis + 4 * (img * g.n_bl);
     for (int base = e0 + tid; base < e1; base += stride * U) {
         GatherBatch<U> gb;
         gb.load_entries(ent, base, e0, e1, stride);
-        gb.load_records(g, vis);
+        gb.load_records(g, rec);
         total += gb.consume(g, nu, sink);
     }
     return total;

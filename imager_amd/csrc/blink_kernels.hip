@@ -494,6 +494,9 @@ k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ unit
     int b0, b1;
     part_range(n_bl, parts, part, b0, b1);
     uint32_t* cl = cells + pu.cell_off;
+    if (pu.ch == 0 && sd.g.corrections)  // one unit per set writes the phase coefficients
+        for (int bl = b0 + (int)threadIdx.x; bl < b1; bl += kUnitThreads)
+            sd.coef_out[bl] = phase_coefficient(sd.g.corrections, sd.w_m, sd.cable_m, (unsigned)bl);
     for (int bl = b0 + (int)threadIdx.x; bl < b1; bl += kUnitThreads) {
         const int f = sd.flags[bl];
         const float u = sd.u_m[bl], v = sd.v_m[bl];

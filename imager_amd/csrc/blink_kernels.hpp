@@ -26,8 +26,10 @@ extern int64_t g_lds_budget;
 
 struct GatherArgs {
     const float4* vis;        // [t][ch][bl][2 x float4]
-    const float* w_m;
-    const double* cable_m;
+    // phase coefficient of every baseline in metres, w[bl] - (L[a2] - L[a1])
+    // with each term present when its correction is on (read when
+    // corrections != 0): the correction is exp(+2 pi i coef nu / c)
+    const double* phase_coef;
     const double* freq_hz;
     unsigned corrections;
     int pol;
@@ -41,6 +43,9 @@ struct GatherArgs {
 // gi = img_base .. img_base + n_img - 1.
 struct SetDesc {
     GatherArgs g;             // visibilities, corrections, pol, n_bl, n_ch
+    const float* w_m;         // inputs of the phase coefficients (g.phase_coef),
+    const double* cable_m;    // which the plan launch writes to coef_out
+    double* coef_out;
     const float* u_m;
     const float* v_m;
     const int* flags;
