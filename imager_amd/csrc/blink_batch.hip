@@ -156,7 +156,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         if (rs[rr + 2] != rs[rr]) {
             const float2 a = tile[rr * TS + pad_index(c)];
             const float2 b = tile[(rr + 1) * TS + pad_index(c)];
-            *(float4*)&dst[(int64_t)c * HS + rr] = make_float4(a.x, a.y, b.x, b.y);
+            store_stream((float4*)&dst[(int64_t)c * HS + rr], make_float4(a.x, a.y, b.x, b.y));
         }
     }
 }
@@ -231,7 +231,7 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
         const int r = i / TEAMS, t = i % TEAMS;
         const int orow = r < H ? r + H : r - H;
         const float2 z = tile[t * TS + pad_index(r)];
-        *(float4*)&out[(int64_t)orow * N + oc0 + 2 * t] = make_float4(z.x, 0.f, z.y, 0.f);
+        store_stream((float4*)&out[(int64_t)orow * N + oc0 + 2 * t], make_float4(z.x, 0.f, z.y, 0.f));
     }
 }
 
@@ -321,7 +321,7 @@ k_grid_rows_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __r
     F::template run<1>(v, row, lane, TwSplit64{twl, twl + 64});
     if (team < nr && rs[team + 1] != rs[team]) {
         float2* dst = inter + ((int64_t)li * (H + 1) + r0 + team) * N;
-        F::store(v, lane, [&](int i, float2 x) { dst[i] = x; });
+        F::store(v, lane, [&](int i, float2 x) { store_stream(&dst[i], x); });
     }
 }
 
@@ -389,7 +389,7 @@ k_cols_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
         const int r = i / TEAMS, t = i % TEAMS;
         const int orow = r < H ? r + H : r - H;
         const float2 z = tile[t * TS + pad_index(r)];
-        *(float4*)&out[(int64_t)orow * N + oc0 + 2 * t] = make_float4(z.x, 0.f, z.y, 0.f);
+        store_stream((float4*)&out[(int64_t)orow * N + oc0 + 2 * t], make_float4(z.x, 0.f, z.y, 0.f));
     }
 }
 

@@ -982,26 +982,36 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
     const SetDesc& first = bp.host_desc(g);
     const int* is = t.img_set(g);
     const int ns = (int)g.sets.size();
-    int best_r = rows[0], best_c = cols[0];
-    float best = 1e30f;
-    for (int v = 0; v < nr; v++) {
-        float ms;
-        int rc = time_launches(ctx, s, 3, [&] {
-            return launch_grid_rows_reg(first, sd, is, ns, n, rows[v], 0, cnt, bp.inter, tw, s);
-        }, &ms);
-        if (rc) return rc;
-        if (ms < best) best = ms, best_r = rows[v];
+    // Variants are timed in interleaved rounds and each keeps its fastest
+    // round, so that clock ramp-up and neighbour noise do not favour the
+    // variants timed last.
+    constexpr int kRounds = 3;
+    float tr[8], tc[8];
+    for (int v = 0; v < 8; v++) tr[v] = tc[v] = 1e30f;
+    for (int round = 0; round < kRounds; round++) {
+        for (int v = 0; v < nr; v++) {
+            float ms;
+            int rc = time_launches(ctx, s, 2, [&] {
+                return launch_grid_rows_reg(first, sd, is, ns, n, rows[v], 0, cnt, bp.inter, tw,
+                                            s);
+            }, &ms);
+            if (rc) return rc;
+            tr[v] = std::min(tr[v], ms);
+        }
     }
+    const int best_r = rows[std::min_element(tr, tr + nr) - tr];
     HIP_TRY(launch_grid_rows_reg(first, sd, is, ns, n, best_r, 0, cnt, bp.inter, tw, s));
-    best = 1e30f;
-    for (int v = 0; v < nc; v++) {
-        float ms;
-        int rc = time_launches(ctx, s, 3, [&] {
-            return launch_cols_reg(first, sd, is, ns, n, cols[v], 0, cnt, bp.inter, tw, s);
-        }, &ms);
-        if (rc) return rc;
-        if (ms < best) best = ms, best_c = cols[v];
+    for (int round = 0; round < kRounds; round++) {
+        for (int v = 0; v < nc; v++) {
+            float ms;
+            int rc = time_launches(ctx, s, 2, [&] {
+                return launch_cols_reg(first, sd, is, ns, n, cols[v], 0, cnt, bp.inter, tw, s);
+            }, &ms);
+            if (rc) return rc;
+            tc[v] = std::min(tc[v], ms);
+        }
     }
+    const int best_c = cols[std::min_element(tc, tc + nc) - tc];
     ctx->tiles[n] = {best_r, best_c};
     return BLINK_OK;
 }

@@ -117,6 +117,20 @@ __device__ __forceinline__ float2 select_pol(const float4 xx_xy, const float4 yx
     return make_float2((xx_xy.x + yx_yy.z) / 2.0f, (xx_xy.y + yx_yy.w) / 2.0f);  // I
 }
 
+// Streaming (non-temporal) accesses for data touched once per pass: the
+// row-pass buffer and the images.  They bypass the normal cache allocation,
+// leaving L2 to the visibilities and tables that are reused.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_stream(float4* p, float4 v) {
+    const f32x4_t t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, (f32x4_t*)p);
+}
+__device__ __forceinline__ void store_stream(float2* p, float2 v) {
+    const f32x2_t t = {v.x, v.y};
+    __builtin_nontemporal_store(t, (f32x2_t*)p);
+}
+
 // select_pol from the one or two products the polarisation needs: x0 = XX
 // (XX, I), YY (YY) or XY (V); x1 = YY (I) or YX (V).  Same arithmetic.
 __device__ __forceinline__ float2 select_pol2(float2 x0, float2 x1, int pol) {

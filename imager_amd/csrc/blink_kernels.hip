@@ -1008,7 +1008,7 @@ k_grid_rows_fft(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         float2* row = tile + (size_t)r * n;
         const float2* res = fft_row<S, kBlockTeam>(row, tmp, plan, tw, lane, team);
         float2* dst = inter + ((int64_t)li * n + r0 + r) * n;
-        for (int i = lane; i < n; i += team) dst[i] = res[i];
+        for (int i = lane; i < n; i += team) store_stream(&dst[i], res[i]);
         team_sync<kBlockTeam>();
     }
 }
