@@ -793,7 +793,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             }
             continue;
         }
-        const int64_t per = g.reg ? batch_inter_bytes(g.n) : 8 * n * n;
+        const int64_t per = g.reg ? batch_inter_bytes(g.n) : 8 * (n / 2 + 1) * n;
         g.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(g.n_img, inter_budget / per));
         inter_bytes = std::max(inter_bytes, (size_t)(per * g.chunk));
     }
@@ -1088,17 +1088,8 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const Batc
             }
             LAUNCH("grid_rows_fft", s,
                    launch_grid_rows_fft(first, sd, is, ns, plan, gi0, cnt, bp.inter, tw, s));
-            ColArgs a{};
-            a.in = (const float2*)bp.inter;
-            a.in_img_stride = gs;
-            a.out_img_offset = gi0;
-            a.paired = 0;   // out of place: no shift partner needed
-            a.shift = 1;
-            a.sets = sd;
-            a.img_set = is;
-            a.n_sets = ns;
-            a.first = first;
-            LAUNCH("fft_cols", s, launch_fft_cols(a, plan, cnt, BLINK_FFT_BACKWARD, tw, s));
+            LAUNCH("fft_cols", s,
+                   launch_cols_half(first, sd, is, ns, plan, gi0, cnt, bp.inter, tw, s));
         }
     }
     for (int i = 0; i < (int)bp.sc.size(); i++)

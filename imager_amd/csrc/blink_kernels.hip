@@ -800,18 +800,10 @@ k_fft_cols(ColArgs a, FftPlan plan, int team, const float2* __restrict__ tw) {
     const int c0 = cb * a.width;
     const int w_here = min(a.width, span - c0);
     const float2* src = a.in + img * a.in_img_stride;
-    int64_t oimg = a.out_img_offset + img;
+    const int64_t oimg = a.out_img_offset + img;
     float2* dst = a.out + img * a.out_img_stride;
     const int* rs = a.row_start ? a.row_start + (oimg % a.n_ch) * (int64_t)(n + 1) : nullptr;
     const uint32_t* isums = a.isums;
-    if (a.sets) {  // group image oimg of a batched call
-        const SetDesc sd = a.n_sets == 1 ? a.first : a.sets[a.img_set[oimg]];
-        const int li = (int)(oimg - sd.img_base);
-        oimg = li;
-        dst = sd.images + (int64_t)li * n * n;
-        rs = sd.row_start + (int64_t)(li % sd.g.n_ch) * (n + 1);
-        isums = sd.isums;
-    }
 
     // load: element (r, w) of the set; columns c0+w and c0+N/2+w
     const int total = n * wt;
@@ -967,7 +959,10 @@ k_grid_rows_fft(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     __shared__ int rs[kMaxTileRows + 1];
     __shared__ int red[kFftThreads / 64];
 
-    const int n_rb = (n + rb - 1) / rb;
+    // Rows 0 .. N/2 only: the gridded plane is Hermitian, so row N - v is the
+    // conjugate mirror of row v and the column pass rebuilds it.
+    const int h = n / 2;
+    const int n_rb = (h + 1 + rb - 1) / rb;
     // XCD-aware order: consecutive workgroup ids land on different XCDs, so
     // the ids are remapped to give each XCD a contiguous run of (image, tile)
     // pairs and keep an image's visibilities in one L2.
@@ -982,7 +977,7 @@ k_grid_rows_fft(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     const int img = (int)(gi - sd.img_base);  // image within its set
     const int ch = img % g.n_ch;
     const int r0 = tile_id * rb;
-    const int nr = min(rb, n - r0);
+    const int nr = min(rb, h + 1 - r0);
     const int* rsg = sd.row_start + (int64_t)ch * (n + 1);
     const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
     if ((int)threadIdx.x <= nr) rs[threadIdx.x] = rsg[r0 + threadIdx.x];
@@ -995,7 +990,8 @@ k_grid_rows_fft(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         const int r = row_of_entry(rs, nr, e);
         atomicAdd(&tile[r * n + u_idx].x, v.x);
         atomicAdd(&tile[r * n + u_idx].y, v.y);
-        return 1;
+        const int row = r0 + r;  // an entry of rows 1 .. N/2-1 stands for its mirror too
+        return (row == 0 || row == h) ? 1 : 2;
     });
     const int tot = block_sum_int(n_ok, red);
     if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
@@ -1007,9 +1003,78 @@ k_grid_rows_fft(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         if (rs[r + 1] == rs[r]) continue;  // uniform per team
         float2* row = tile + (size_t)r * n;
         const float2* res = fft_row<S, kBlockTeam>(row, tmp, plan, tw, lane, team);
-        float2* dst = inter + ((int64_t)li * n + r0 + r) * n;
+        float2* dst = inter + ((int64_t)li * (h + 1) + r0 + r) * n;
         for (int i = lane; i < n; i += team) store_stream(&dst[i], res[i]);
         team_sync<kBlockTeam>();
+    }
+}
+
+// Column pass of the generic batched path, from the half-height row-major
+// row-pass buffer inter[img][v][x], v = 0 .. N/2.  As in the register path,
+// the images are real, so one complex FFT of z = c_x + i c_{x+1} gives
+// columns x and x+1; rows N - v are the conjugates of rows v.  A workgroup
+// owns `cb` (even) output columns = cb/2 pairs, staged through LDS with
+// 16-byte loads, transformed by the teams, scaled by 1/sum and stored
+// fftshifted with imaginary parts 0.
+template <int S, bool kBlockTeam, bool kMulti>
+__global__ void __launch_bounds__(kFftThreads)
+k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
+            FftPlan plan, int team, int cb, int64_t gi0, const float2* __restrict__ inter,
+            const float2* __restrict__ tw) {
+    extern __shared__ float4 ch_lds4[];
+    const int n = plan.n, h = n / 2, pairs = cb / 2;
+    const int ld = n + 1;                                   // padded LDS row
+    float2* rows = (float2*)ch_lds4;                        // [pairs][ld]
+    const int teams = kBlockTeam ? 1 : (int)blockDim.x / team;
+    const int t_id = kBlockTeam ? 0 : (int)threadIdx.x / team;
+    const int lane = kBlockTeam ? (int)threadIdx.x : (int)threadIdx.x % team;
+    float2* tmp = rows + (size_t)pairs * ld + (size_t)t_id * n;
+    unsigned char* nonempty = (unsigned char*)(rows + (size_t)pairs * ld + (size_t)teams * n);
+
+    const int n_cb = n / cb;
+    const int cbk = (int)(blockIdx.x % n_cb);
+    const int li = (int)(blockIdx.x / n_cb);
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const int img = (int)(gi - sd.img_base);
+    const uint32_t count = sd.isums[img];
+    const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (n + 1);
+    for (int i = threadIdx.x; i <= h; i += blockDim.x) nonempty[i] = rsg[i + 1] != rsg[i];
+    __syncthreads();
+
+    const int c0 = cbk * cb;
+    const float2* src = inter + (int64_t)li * (h + 1) * n + c0;
+    for (int i = threadIdx.x; i < (h + 1) * pairs; i += blockDim.x) {
+        const int v = i / pairs, t = i % pairs;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (nonempty[v]) p = *(const float4*)&src[(int64_t)v * n + 2 * t];
+        float2* zr = rows + (size_t)t * ld;
+        zr[v] = make_float2(p.x - p.w, p.y + p.z);              // c_x[v] + i c_{x+1}[v]
+        if (v > 0 && v < h) zr[n - v] = make_float2(p.x + p.w, p.z - p.y);  // row N - v
+    }
+    __syncthreads();
+    const int rounds = kBlockTeam ? pairs : (pairs + teams - 1) / teams;
+    for (int k = 0; k < rounds; k++) {
+        const int t = kBlockTeam ? k : k * teams + t_id;
+        if (!kBlockTeam && t >= pairs) break;
+        float2* zr = rows + (size_t)t * ld;
+        const float2* res = fft_row<S, kBlockTeam>(zr, tmp, plan, tw, lane, team);
+        if (res != zr) {
+            for (int i = lane; i < n; i += team) zr[i] = res[i];
+            team_sync<kBlockTeam>();
+        }
+    }
+    __syncthreads();
+    const float s = (float)(1.0 / (double)count);
+    float2* out = sd.images + (int64_t)img * n * n;
+    for (int i = threadIdx.x; i < n * pairs; i += blockDim.x) {
+        const int r = i / pairs, t = i % pairs;
+        const float2 z = rows[(size_t)t * ld + r];
+        const int orow = r < h ? r + h : r - h;
+        const int x = c0 + 2 * t;
+        const int ox0 = x < h ? x + h : x - h, ox1 = x + 1 < h ? x + 1 + h : x + 1 - h;
+        store_stream(&out[(int64_t)orow * n + ox0], make_float2(z.x * s, 0.f));
+        store_stream(&out[(int64_t)orow * n + ox1], make_float2(z.y * s, 0.f));
     }
 }
 
@@ -1145,9 +1210,9 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
     const int teams = block_team ? 1 : kFftThreads / team;
     // rows per tile: as many as fit, but at least ~2048 workgroups in the
     // launch, so that a few images still spread over the whole chip
-    const int64_t want = (int64_t)n * n_img / 2048;
+    const int64_t want = (int64_t)(n / 2 + 1) * n_img / 2048;
     const int rb = (int)std::max<int64_t>(1, std::min<int64_t>(grid_rows_tile_rows(n), want));
-    const int n_rb = (n + rb - 1) / rb;
+    const int n_rb = (n / 2 + 1 + rb - 1) / rb;
     const size_t lds = sizeof(float2) * ((size_t)rb * n + (size_t)teams * n);
     if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)((int64_t)n_rb * n_img));
@@ -1157,6 +1222,42 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
                  : (multi ? k_grid_rows_fft<1, false, true> : k_grid_rows_fft<1, false, false>);
     hipLaunchKernelGGL(k, grid, dim3(kFftThreads), lds, stream, first, sets, img_set, plan, team,
                        rb, gi0, (float2*)inter, (const float2*)tw);
+    return hipGetLastError();
+}
+
+// Output columns per workgroup of k_cols_half: the largest even divisor of
+// n, at most 64, whose LDS rows fit next to the team scratch.
+static int half_cols_block(int n, int teams) {
+    for (int cb = std::min(64, n); cb >= 2; cb--) {
+        if (cb % 2 || n % cb) continue;
+        const int64_t lds = sizeof(float2) * ((int64_t)(cb / 2) * (n + 1) + (int64_t)teams * n) +
+                            (n / 2 + 1);
+        if (lds <= g_lds_budget) return cb;
+    }
+    return 0;
+}
+
+hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                            int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
+                            const float* inter, const float* tw, hipStream_t stream) {
+    if (n_img == 0) return hipSuccess;
+    const int n = plan.n;
+    if (n % 2) return hipErrorInvalidValue;
+    const int team = fft_team_for(n);
+    const bool block_team = team == kFftThreads;
+    const int teams = block_team ? 1 : kFftThreads / team;
+    const int cb = half_cols_block(n, teams);
+    if (!cb) return hipErrorInvalidConfiguration;
+    const size_t lds = sizeof(float2) * ((size_t)(cb / 2) * (n + 1) + (size_t)teams * n) +
+                       (size_t)(n / 2 + 1);
+    if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
+    const int64_t blocks = (int64_t)(n / cb) * n_img;
+    if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    const bool multi = n_sets > 1;
+    auto k = block_team ? (multi ? k_cols_half<1, true, true> : k_cols_half<1, true, false>)
+                        : (multi ? k_cols_half<1, false, true> : k_cols_half<1, false, false>);
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kFftThreads), lds, stream, first, sets,
+                       img_set, plan, team, cb, gi0, (const float2*)inter, (const float2*)tw);
     return hipGetLastError();
 }
 
@@ -1193,6 +1294,10 @@ hipError_t set_kernel_attributes() {
     set((const void*)k_grid_rows_fft<1, false, false>);
     set((const void*)k_grid_rows_fft<1, true, true>);
     set((const void*)k_grid_rows_fft<1, false, true>);
+    set((const void*)k_cols_half<1, true, false>);
+    set((const void*)k_cols_half<1, false, false>);
+    set((const void*)k_cols_half<1, true, true>);
+    set((const void*)k_cols_half<1, false, true>);
     set((const void*)k_grid_tiles);
     set((const void*)k_plan_rows);
     set((const void*)k_plan_count);

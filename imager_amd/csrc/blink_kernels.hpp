@@ -84,13 +84,6 @@ struct ColArgs {
     const uint32_t* isums;     // integer sums alternative, or null
     const int* row_start;      // plan rows: skip empty rows on load, or null
     int n_ch;
-    // batched groups: when sets is set, image out_img_offset + i of the
-    // launch is a group image; its output, plan and count sum come from its
-    // set (out, isums, row_start, n_ch above are then unused)
-    const SetDesc* sets;
-    const int* img_set;
-    int n_sets;
-    SetDesc first;             // sets[0], by value
 };
 
 bool make_fft_plan(int n, FftPlan& plan);
@@ -144,9 +137,15 @@ hipError_t launch_shift_norm(float* images, int nx, int ny, int n_images, const 
                              int n_sums, hipStream_t stream);
 hipError_t launch_average(const float* images, int64_t image_size, int n_images, int real_only,
                           float* avg, hipStream_t stream);
+// Generic batched path (any even side the FFT plan supports): row pass over
+// rows 0 .. N/2 into a row-major half-height buffer [img][N/2 + 1][N], and
+// the column pass from it (two real columns per complex FFT).
 hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                 int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
                                 float* inter, const float* tw, hipStream_t stream);
+hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                            int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
+                            const float* inter, const float* tw, hipStream_t stream);
 // Gridding plans of a batched call: one unit per (set, channel); `parts`
 // workgroups per unit split its baselines (at most kPlanPartMax each).
 // Scratch: partial is [n_units][parts][n_max] ints, cells holds one packed
