@@ -147,7 +147,8 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     __syncthreads();
     // transposed store: inter[li][c][r0 + rr], RB consecutive rows per column,
     // two rows (16 bytes) per thread; a pair is written when either row has
-    // entries (an empty row of the tile holds zeros).
+    // entries (an empty row of the tile holds zeros; uniform 16-byte stores
+    // beat skipping it).
     constexpr int HS = half_stride(N);
     float2* dst = inter + (int64_t)li * N * HS + r0;
     constexpr int RP = RB / 2;

@@ -982,36 +982,33 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
     const SetDesc& first = bp.host_desc(g);
     const int* is = t.img_set(g);
     const int ns = (int)g.sets.size();
-    // Variants are timed in interleaved rounds and each keeps its fastest
-    // round, so that clock ramp-up and neighbour noise do not favour the
-    // variants timed last.
+    // Every (row tile, column block) pair is timed as the pipeline runs it
+    // (row pass then column pass), in interleaved rounds; each pair keeps its
+    // fastest round, so that clock ramp-up and neighbour noise do not favour
+    // the pairs timed last.
     constexpr int kRounds = 3;
-    float tr[8], tc[8];
-    for (int v = 0; v < 8; v++) tr[v] = tc[v] = 1e30f;
-    for (int round = 0; round < kRounds; round++) {
-        for (int v = 0; v < nr; v++) {
-            float ms;
-            int rc = time_launches(ctx, s, 2, [&] {
-                return launch_grid_rows_reg(first, sd, is, ns, n, rows[v], 0, cnt, bp.inter, tw,
-                                            s);
-            }, &ms);
-            if (rc) return rc;
-            tr[v] = std::min(tr[v], ms);
-        }
-    }
-    const int best_r = rows[std::min_element(tr, tr + nr) - tr];
-    HIP_TRY(launch_grid_rows_reg(first, sd, is, ns, n, best_r, 0, cnt, bp.inter, tw, s));
-    for (int round = 0; round < kRounds; round++) {
-        for (int v = 0; v < nc; v++) {
-            float ms;
-            int rc = time_launches(ctx, s, 2, [&] {
-                return launch_cols_reg(first, sd, is, ns, n, cols[v], 0, cnt, bp.inter, tw, s);
-            }, &ms);
-            if (rc) return rc;
-            tc[v] = std::min(tc[v], ms);
-        }
-    }
-    const int best_c = cols[std::min_element(tc, tc + nc) - tc];
+    float tp[8][8];
+    for (int r = 0; r < 8; r++)
+        for (int c = 0; c < 8; c++) tp[r][c] = 1e30f;
+    for (int round = 0; round < kRounds; round++)
+        for (int r = 0; r < nr; r++)
+            for (int c = 0; c < nc; c++) {
+                float ms;
+                int rc = time_launches(ctx, s, 2, [&] {
+                    hipError_t e = launch_grid_rows_reg(first, sd, is, ns, n, rows[r], 0, cnt,
+                                                        bp.inter, tw, s);
+                    if (e != hipSuccess) return e;
+                    return launch_cols_reg(first, sd, is, ns, n, cols[c], 0, cnt, bp.inter, tw,
+                                           s);
+                }, &ms);
+                if (rc) return rc;
+                tp[r][c] = std::min(tp[r][c], ms);
+            }
+    int best_r = rows[0], best_c = cols[0];
+    float best = 1e30f;
+    for (int r = 0; r < nr; r++)
+        for (int c = 0; c < nc; c++)
+            if (tp[r][c] < best) best = tp[r][c], best_r = rows[r], best_c = cols[c];
     ctx->tiles[n] = {best_r, best_c};
     return BLINK_OK;
 }

@@ -518,6 +518,8 @@ k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ unit
 __global__ void __launch_bounds__(kUnitThreads)
 k_plan_scan(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ units, int parts,
             int n_max, int* __restrict__ partial) {
+    extern __shared__ int unit_lds[];
+    int* row_tot = unit_lds;  // [n] entries per row over all parts, then row starts
     __shared__ int warp_tot[kUnitThreads / 64];
     const int unit = blockIdx.x;
     const PlanUnit pu = units[unit];
@@ -526,21 +528,35 @@ k_plan_scan(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ units
     int* rs = sd.row_start + (int64_t)pu.ch * (n + 1);
     int* base = partial + (int64_t)unit * parts * n_max;
     const int tid = threadIdx.x;
-    // each thread owns a contiguous chunk of rows: sum, block scan, rewrite
+    // row totals: one thread per row, parts summed in order (coalesced reads)
+    for (int i = tid; i < n; i += kUnitThreads) {
+        int t = 0;
+        for (int p = 0; p < parts; p++) t += base[(int64_t)p * n_max + i];
+        row_tot[i] = t;
+    }
+    __syncthreads();
+    // exclusive scan of the row totals: contiguous chunks per thread
     const int chunk = (n + kUnitThreads - 1) / kUnitThreads;
     const int c0 = min(tid * chunk, n), c1 = min(c0 + chunk, n);
     int local = 0;
-    for (int i = c0; i < c1; i++)
-        for (int p = 0; p < parts; p++) local += base[(int64_t)p * n_max + i];
+    for (int i = c0; i < c1; i++) local += row_tot[i];
     int total;
     int run = unit_scan(local, warp_tot, &total);
     for (int i = c0; i < c1; i++) {
-        rs[i] = run;
+        const int t = row_tot[i];
+        row_tot[i] = run;
+        run += t;
+    }
+    __syncthreads();
+    // row starts, and each part's first slot in every row (one thread per row)
+    for (int i = tid; i < n; i += kUnitThreads) {
+        int r = row_tot[i];
+        rs[i] = r;
         for (int p = 0; p < parts; p++) {
             int* q = base + (int64_t)p * n_max + i;
             const int h = *q;
-            *q = run;  // this part's first slot in row i
-            run += h;
+            *q = r;
+            r += h;
         }
     }
     if (tid == 0) rs[n] = total;
@@ -621,8 +637,8 @@ hipError_t launch_plan_units(const SetDesc* sets, const PlanUnit* units, int n_u
                        stream, sets, units, parts, n_max, partial, cells);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_plan_scan, dim3((unsigned)n_units), dim3(kUnitThreads), 0, stream, sets,
-                       units, parts, n_max, partial);
+    hipLaunchKernelGGL(k_plan_scan, dim3((unsigned)n_units), dim3(kUnitThreads), lds_count, stream,
+                       sets, units, parts, n_max, partial);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_plan_scatter, dim3((unsigned)blocks), dim3(kUnitThreads), lds_scatter,
                        stream, sets, units, parts, n_max, (const int*)partial,
