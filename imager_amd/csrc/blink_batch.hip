@@ -232,7 +232,12 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
         const int r = i / TEAMS, t = i % TEAMS;
         const int orow = r < H ? r + H : r - H;
         const float2 z = tile[t * TS + pad_index(r)];
-        store_stream((float4*)&out[(int64_t)orow * N + oc0 + 2 * t], make_float4(z.x, 0.f, z.y, 0.f));
+        float4* o = (float4*)&out[(int64_t)orow * N + oc0 + 2 * t];
+        const float4 val = make_float4(z.x, 0.f, z.y, 0.f);
+        // streaming stores when a block writes whole 128-byte lines; narrower
+        // blocks leave the merging of line halves to L2
+        if constexpr (CB >= 16) store_stream(o, val);
+        else *o = val;
     }
 }
 
@@ -390,7 +395,9 @@ k_cols_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
         const int r = i / TEAMS, t = i % TEAMS;
         const int orow = r < H ? r + H : r - H;
         const float2 z = tile[t * TS + pad_index(r)];
-        store_stream((float4*)&out[(int64_t)orow * N + oc0 + 2 * t], make_float4(z.x, 0.f, z.y, 0.f));
+        // normal stores: a row segment here is 64 bytes, and L2 has to merge
+        // the halves of each line that neighbouring column blocks write
+        *(float4*)&out[(int64_t)orow * N + oc0 + 2 * t] = make_float4(z.x, 0.f, z.y, 0.f);
     }
 }
 

@@ -931,18 +931,19 @@ static BatchTables tables_at(const BatchPlan& bp, const void* dblob) {
     return t;
 }
 
-// Default tiles when the autotuner is off: the largest built variant up to
-// 16 (8 at N = 2048), rows per row tile and output columns per column block.
+// Default tiles (rows per row tile, output columns per column block): the
+// pairs measured fastest on MI355X for the 128-tile and 256-antenna arrays;
+// the autotuner keeps them unless another pair is more than 3 % faster.
 static std::pair<int, int> default_tiles(int n) {
-    const int cap = n >= 2048 ? 8 : 16;
-    int t[8], best[2];
-    for (int kind = 0; kind < 2; kind++) {
-        const int k = kind == 0 ? batch_row_variants(n, t, 8) : batch_col_variants(n, t, 8);
-        best[kind] = k > 0 ? t[0] : 0;
-        for (int i = 0; i < k; i++)
-            if (t[i] <= cap) best[kind] = std::max(best[kind], t[i]);
+    switch (n) {
+        case 128: return {32, 64};
+        case 256: return {8, 32};
+        case 512: return {8, 16};
+        case 1024: return {8, 32};
+        case 2048: return {8, 16};
+        case 4096: return {2, 8};
+        default: return {0, 0};
     }
-    return {best[0], best[1]};
 }
 
 // Average GPU milliseconds of `reps` launches made by fn() on stream s.
@@ -976,7 +977,10 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
     int rows[8], cols[8];
     const int nr = batch_row_variants(n, rows, 8);
     const int nc = batch_col_variants(n, cols, 8);
-    const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img);
+    // timed on up to 2 GiB of row-pass buffer (all of a 640-image 1024^2
+    // chunk, 30 images at 4096^2), which keeps the tuning under a second
+    const int64_t cap = std::max<int64_t>(16, (2ll << 30) / batch_inter_bytes(n));
+    const int cnt = (int)std::min<int64_t>({(int64_t)g.chunk, g.n_img, cap});
     const float* tw = ctx->twiddles[n];
     const SetDesc* sd = t.group_sets(g);
     const SetDesc& first = bp.host_desc(g);
@@ -1005,10 +1009,14 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
                 tp[r][c] = std::min(tp[r][c], ms);
             }
     int best_r = rows[0], best_c = cols[0];
-    float best = 1e30f;
+    float best = 1e30f, t_default = 1e30f;
+    const std::pair<int, int> dflt = default_tiles(n);
     for (int r = 0; r < nr; r++)
-        for (int c = 0; c < nc; c++)
+        for (int c = 0; c < nc; c++) {
             if (tp[r][c] < best) best = tp[r][c], best_r = rows[r], best_c = cols[c];
+            if (rows[r] == dflt.first && cols[c] == dflt.second) t_default = tp[r][c];
+        }
+    if (t_default <= 1.03f * best) best_r = dflt.first, best_c = dflt.second;
     ctx->tiles[n] = {best_r, best_c};
     return BLINK_OK;
 }
