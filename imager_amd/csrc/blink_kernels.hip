@@ -531,6 +531,7 @@ k_plan_scan(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ units
     // row totals: one thread per row, parts summed in order (coalesced reads)
     for (int i = tid; i < n; i += kUnitThreads) {
         int t = 0;
+#pragma unroll 8
         for (int p = 0; p < parts; p++) t += base[(int64_t)p * n_max + i];
         row_tot[i] = t;
     }
@@ -552,11 +553,17 @@ k_plan_scan(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ units
     for (int i = tid; i < n; i += kUnitThreads) {
         int r = row_tot[i];
         rs[i] = r;
-        for (int p = 0; p < parts; p++) {
-            int* q = base + (int64_t)p * n_max + i;
-            const int h = *q;
-            *q = r;
-            r += h;
+        for (int p0 = 0; p0 < parts; p0 += 8) {
+            int h[8];  // eight independent loads in flight, then the running sums
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                h[k] = p0 + k < parts ? base[(int64_t)(p0 + k) * n_max + i] : 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (p0 + k < parts) {
+                    base[(int64_t)(p0 + k) * n_max + i] = r;
+                    r += h[k];
+                }
         }
     }
     if (tid == 0) rs[n] = total;
