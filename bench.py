@@ -87,20 +87,31 @@ def algorithmic_bytes(kernel: str, n_bl: int, n_pix: int, n_img: int) -> float |
     f = 16.0 * n_pix * n_pix
     if kernel in ("grid_rows_fft", "grid_tiles"):
         return g * n_img
+    if kernel == "grid_half_rows":   # DFT transform: half plane, complex only
+        return (16.0 * n_bl + 8.0 * (n_pix // 2 + 1) * n_pix) * n_img
     if kernel == "fft_cols":
         return f * n_img
     return None
 
 
-def dft_flops(n_pix: int, n_img: int):
-    """(useful, executed) flops of ONE dft_gemm launch (one of the two complex
-    N x N x N products per image): useful 8 N^3 per image; executed = 12 MFMA
-    32x32x16 per wave and k-step of 16 over the padded tile grid."""
+def dft_flops(kernel: str, n_pix: int, n_img: int):
+    """(useful, executed) flops of ONE launch of a DFT-as-GEMM stage over the
+    Hermitian half plane (H = N/2):
+      dft_rows: complex (H+1) x N rows times the N x N matrix per image,
+                useful 8 (H+1) N^2, executed 12 split real products;
+      dft_cols: real part of N rows times the (H+1) x N matrix per image,
+                useful 4 (H+1) N^2, executed 6 split real products;
+    executed flops count every MFMA over the padded tile grid (m in tiles of
+    256, n of 64, k of 16)."""
+    h1 = n_pix // 2 + 1
     n_pad = -(-n_pix // 64) * 64
-    k_pad = -(-n_pix // 16) * 16
-    m_pad = -(-(n_img * n_pix) // 64) * 64
-    useful = 8.0 * n_pix ** 3 * n_img
-    executed = 12 * 2.0 * 32 * 32 * 16 * (n_pad // 32) * (m_pad // 32) * (k_pad // 16)
+    if kernel == "dft_rows":
+        m, k, terms, useful = n_img * h1, n_pix, 12, 8.0 * h1 * n_pix * n_pix * n_img
+    else:
+        m, k, terms, useful = n_img * n_pix, h1, 6, 4.0 * h1 * n_pix * n_pix * n_img
+    m_pad = -(-m // 256) * 256
+    k_pad = -(-k // 16) * 16
+    executed = terms * 2.0 * m_pad * n_pad * k_pad
     return useful, executed
 
 
@@ -275,9 +286,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mwa128_64ch10t_1024", choices=WORKLOADS)
-    ap.add_argument("--transform", choices=["fft", "dft"], default="fft",
+    ap.add_argument("--transform", choices=["fft", "dft", "dft-fp16"], default="fft",
                     help="2-D transform of the batched path: in-house FP32 FFT, or the "
-                         "split-precision (bf16 hi+lo) DFT-as-GEMM on MFMA (N <= 512)")
+                         "split-precision DFT-as-GEMM on MFMA (N <= 512) with bf16 (dft) "
+                         "or fp16 (dft-fp16) hi+lo operands")
     ap.add_argument("--plain-images", type=int, default=64,
                     help="images run through the plain one-image-at-a-time GPU path")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -315,7 +327,8 @@ def main():
     wl = make_workload(args.workload)
     log(f"workload {args.workload} generated in {time.perf_counter() - t_gen:.1f} s")
     ctx = capi.Context(0)
-    transform = capi.TRANSFORM_DFT_BF16X3 if args.transform == "dft" else capi.TRANSFORM_FFT
+    transform = {"fft": capi.TRANSFORM_FFT, "dft": capi.TRANSFORM_DFT_BF16X3,
+                 "dft-fp16": capi.TRANSFORM_DFT_FP16X3}[args.transform]
     job = build_job(args.workload, wl, transform)
     obs = job["obs"]            # the observation the plain path and the CPU baseline use
     N = job["n_pixels"]         # None for mixed sizes
@@ -364,22 +377,25 @@ def main():
     roof = None
     pipeline = None
     if N is not None:
-        n_launch = max(prof.get("grid_rows_fft", (0, 0))[0],
-                       prof.get("grid_tiles", (0, 0))[0]) // args.steps
+        n_launch = max(prof.get("grid_rows_fft", (0, 0))[0], prof.get("grid_tiles", (0, 0))[0],
+                       prof.get("grid_half_rows", (0, 0))[0]) // args.steps
         chunk = -(-job["images_per_step"] // max(n_launch, 1))
         imgs_per_launch = chunk if dom in ("grid_rows_fft", "fft_cols", "grid_tiles",
-                                           "dft_gemm") else job["images_per_step"]
+                                           "grid_half_rows", "dft_rows", "dft_cols") \
+            else job["images_per_step"]
         alg = algorithmic_bytes(dom, obs.n_bl, N, imgs_per_launch)
-        if dom == "dft_gemm":
-            useful, executed = dft_flops(N, imgs_per_launch)
+        if dom in ("dft_rows", "dft_cols"):
+            useful, executed = dft_flops(dom, N, imgs_per_launch)
             avg_s = kernels[dom]["avg_ms"] * 1e-3
             ach = executed / avg_s / 1e12
+            prec = "fp16" if args.transform == "dft-fp16" else "bf16"
             roof = {"bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s",
                     "frac": ach / 2500.0, "traffic": None, "kernel": dom,
                     "images_per_launch": imgs_per_launch, "executed_flops_per_launch": executed,
                     "useful_flops_per_launch": useful,
                     "useful_tflops": useful / avg_s / 1e12,
-                    "note": "bf16 dense MFMA peak; executed = 3 split terms x 4 real products"}
+                    "note": f"dense {prec} MFMA peak; executed = 3 split terms x 4 (dft_rows) "
+                            "or 2 (dft_cols) real products over padded tiles"}
         elif alg is not None:
             avg_s = kernels[dom]["avg_ms"] * 1e-3
             ach = alg / avg_s / 1e9
@@ -482,13 +498,14 @@ def main():
     config.update({"workload": args.workload, "images_per_step": job["images_per_step"],
                    "pol": "XX", "corrections": "geometric+cable",
                    "path": "blink_image_batch (hipGraph)",
-                   "transform": "fft fp32" if args.transform == "fft" else
-                                "dft bf16 hi+lo on MFMA"})
+                   "transform": {"fft": "fft fp32", "dft": "dft bf16 hi+lo on MFMA",
+                                 "dft-fp16": "dft fp16 hi+lo on MFMA"}[args.transform]})
     out = {
         "metric": METRIC, "value": value, "unit": "images/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.transform == "fft" else "bf16 hi+lo, f32 accumulate",
+        "dtype": {"fft": "f32", "dft": "bf16 hi+lo, f32 accumulate",
+                  "dft-fp16": "fp16 hi+lo, f32 accumulate"}[args.transform],
         "data": job["data"],
         "config": config,
         "roofline": roof,

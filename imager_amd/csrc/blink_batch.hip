@@ -472,9 +472,9 @@ static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
 // Built variants: rows per row tile, and output columns per column block
 // (two columns per team).
 #define BLINK_ROW_VARIANTS(X) \
-    X(128, 16) X(128, 32) \
-    X(256, 8) X(256, 16) X(256, 32) \
-    X(512, 8) X(512, 16) \
+    X(128, 8) X(128, 16) X(128, 32) \
+    X(256, 4) X(256, 8) X(256, 16) X(256, 32) \
+    X(512, 2) X(512, 4) X(512, 8) X(512, 16) \
     X(1024, 4) X(1024, 8) X(1024, 16) \
     X(2048, 4) X(2048, 8)
 #define BLINK_ROW_WIDE_VARIANTS(X) X(4096, 1) X(4096, 2) X(4096, 4)

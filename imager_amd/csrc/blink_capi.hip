@@ -43,7 +43,7 @@ struct blink_ctx {
     hipDeviceProp_t props{};
     std::string error;
     std::map<int, float*> twiddles;               // N -> device table of N float2
-    std::map<int, uint16_t*> dft_planes;          // N -> split DFT matrix planes
+    std::map<int, uint16_t*> dft_planes;          // (N, stage, precision) -> split DFT planes
     std::map<std::pair<int, int>, double*> kb_tables;  // (size, mode) -> gridding kernels
     std::map<std::string, DeviceBuffer> scratch;  // named grow-only buffers
     std::map<std::string, GraphEntry> graphs;     // descriptor bytes -> graph
@@ -212,21 +212,36 @@ int twiddles(blink_ctx* ctx, int n, const float** out) {
     return BLINK_OK;
 }
 
-// Split DFT matrix of side n (blink_dft_mfma.hip), built once per context.
-int dft_planes(blink_ctx* ctx, int n, const uint16_t** out) {
-    auto it = ctx->dft_planes.find(n);
+static bool is_dft(int transform) {
+    return transform == BLINK_TRANSFORM_DFT_BF16X3 || transform == BLINK_TRANSFORM_DFT_FP16X3;
+}
+static int dft_prec(int transform) {
+    return transform == BLINK_TRANSFORM_DFT_FP16X3 ? kDftFp16 : kDftBf16;
+}
+
+// Split DFT matrix of side n for one stage and precision
+// (blink_dft_mfma.hip), built once per context.
+int dft_planes(blink_ctx* ctx, int n, int stage, int prec, const uint16_t** out) {
+    const int key = n * 8 + stage * 2 + prec;
+    auto it = ctx->dft_planes.find(key);
     if (it != ctx->dft_planes.end()) {
         *out = it->second;
         return BLINK_OK;
     }
     std::vector<uint16_t> h;
-    make_dft_planes(n, h);
+    make_dft_planes(n, stage, prec, h);
     uint16_t* d = nullptr;
     HIP_TRY(hipMalloc(&d, sizeof(uint16_t) * h.size()));
     HIP_TRY(hipMemcpy(d, h.data(), sizeof(uint16_t) * h.size(), hipMemcpyHostToDevice));
-    ctx->dft_planes[n] = d;
+    ctx->dft_planes[key] = d;
     *out = d;
     return BLINK_OK;
+}
+
+// Row-pass bytes per image of the DFT transform: the half plane and the
+// stage-1 product.
+static int64_t dft_inter_bytes(int64_t n) {
+    return 8 * ((n / 2 + 1) * n + n * (int64_t)dft_inter_stride((int)n));
 }
 
 int check_fft_size(blink_ctx* ctx, int n, FftPlan& plan) {
@@ -306,7 +321,8 @@ BLINK_API int blink_create(int device, blink_ctx** out) {
     if (lds <= 0) lds = 64 * 1024;
     g_lds_limit = std::min<int64_t>(lds, 160 * 1024) - 4 * 1024;
     g_lds_budget = g_lds_limit - 12 * 1024;
-    if (set_kernel_attributes() != hipSuccess || set_batch_kernel_attributes() != hipSuccess) {
+    if (set_kernel_attributes() != hipSuccess || set_batch_kernel_attributes() != hipSuccess ||
+        set_dft_kernel_attributes() != hipSuccess) {
         (void)hipGetLastError();
     }
     if (hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking) != hipSuccess) {
@@ -784,10 +800,10 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     }
     for (BatchGroup& g : bp.groups) {
         const int64_t n = g.n;
-        if (g.transform == BLINK_TRANSFORM_DFT_BF16X3) {
+        if (is_dft(g.transform)) {
             for (int i : g.sets) {
                 const int n_img = sets[i].n_ch * sets[i].n_t;
-                const int64_t per = 16 * n * n;  // the grid and the first product
+                const int64_t per = dft_inter_bytes(n);
                 const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
                 inter_bytes = std::max(inter_bytes, (size_t)(per * chunk));
             }
@@ -831,16 +847,18 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         bp.sc[i].entries = (uint32_t*)((char*)en + en_off[i]);
         bp.sc[i].isums = (uint32_t*)is + is_off[i];
         bp.sc[i].coef = (double*)co + co_off[i];
-        const int64_t per = 16 * (int64_t)sets[i].n_pixels * sets[i].n_pixels;
+        const int64_t per = dft_inter_bytes(sets[i].n_pixels);
         const int n_img = sets[i].n_ch * sets[i].n_t;
         bp.sc[i].chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
     }
     for (int i = 0; i < n_sets; i++) {
         const float* tw;
         if ((rc = twiddles(ctx, sets[i].n_pixels, &tw))) return rc;
-        if (sets[i].transform == BLINK_TRANSFORM_DFT_BF16X3) {
+        if (is_dft(sets[i].transform)) {
             const uint16_t* pl;
-            if ((rc = dft_planes(ctx, sets[i].n_pixels, &pl))) return rc;
+            for (int stage = 1; stage <= 2; stage++)
+                if ((rc = dft_planes(ctx, sets[i].n_pixels, stage, dft_prec(sets[i].transform), &pl)))
+                    return rc;
         }
     }
 
@@ -1040,10 +1058,11 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const Batc
     for (const BatchGroup& g : bp.groups) {
         const int n = g.n;
         const float* tw = ctx->twiddles[n];
-        const int64_t gs = (int64_t)n * n;
-        if (g.transform == BLINK_TRANSFORM_DFT_BF16X3) {
-            // per set: grid (overwrite) -> G D'^T -> (.) D'^T with scale, per chunk
-            const uint16_t* pl = ctx->dft_planes[n];
+        if (is_dft(g.transform)) {
+            // per set and chunk: half plane -> stage 1 (complex) -> stage 2 (real, scaled)
+            const int prec = dft_prec(g.transform);
+            const uint16_t* pl1 = ctx->dft_planes[n * 8 + 2 + prec];
+            const uint16_t* pl2 = ctx->dft_planes[n * 8 + 4 + prec];
             for (int i : g.sets) {
                 const blink_image_set& st = sets[i];
                 const SetScratch& sc = bp.sc[i];
@@ -1056,18 +1075,16 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const Batc
                 ga.pol = st.pol;
                 ga.n_bl = st.n_ant * (st.n_ant + 1) / 2;
                 ga.n_ch = st.n_ch;
-                float* gbuf = bp.inter;
-                float* tbuf = bp.inter + 2 * gs * sc.chunk;
+                float* half = bp.inter;
+                float* tbuf = bp.inter + 2 * (int64_t)(n / 2 + 1) * n * sc.chunk;
                 for (int img0 = 0; img0 < n_img; img0 += sc.chunk) {
                     const int cnt = std::min(sc.chunk, n_img - img0);
-                    LAUNCH("grid_tiles", s,
-                           launch_grid_tiles(ga, sc.row_start, sc.entries, cnt, n, gbuf, nullptr,
-                                             s, 1, sc.isums, img0));
-                    LAUNCH("dft_gemm", s,
-                           launch_dft_gemm(gbuf, pl, tbuf, n, cnt, img0, nullptr, nullptr, s));
-                    LAUNCH("dft_gemm", s,
-                           launch_dft_gemm(tbuf, pl, st.images + 2 * gs * img0, n, cnt, img0,
-                                           sc.isums, nullptr, s));
+                    LAUNCH("grid_half_rows", s,
+                           launch_grid_half_rows(ga, sc.row_start, sc.entries, n, cnt, img0, half,
+                                                 sc.isums, s));
+                    LAUNCH("dft_rows", s, launch_dft_rows(half, pl1, tbuf, n, cnt, prec, s));
+                    LAUNCH("dft_cols", s,
+                           launch_dft_cols(tbuf, pl2, st.images, n, cnt, img0, sc.isums, prec, s));
                 }
             }
             continue;
@@ -1135,16 +1152,16 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
         CHECK_ARG(st.pol >= 0 && st.pol <= 3, "set %d: pol must be 0..3", i);
         CHECK_ARG(((uintptr_t)st.vis & 15) == 0 && ((uintptr_t)st.images & 15) == 0,
                   "set %d: vis and images must be 16-byte aligned", i);
-        CHECK_ARG(st.transform == BLINK_TRANSFORM_FFT || st.transform == BLINK_TRANSFORM_DFT_BF16X3,
+        CHECK_ARG(st.transform == BLINK_TRANSFORM_FFT || is_dft(st.transform),
                   "set %d: unknown transform %d", i, st.transform);
-        if (st.transform == BLINK_TRANSFORM_DFT_BF16X3) {
+        CHECK_ARG(st.n_pixels % 2 == 0, "set %d: the batched path needs an even grid side", i);
+        if (is_dft(st.transform)) {
             if (st.n_pixels < 2 || st.n_pixels > BLINK_MAX_DFT_PIXELS)
                 return ctx->fail(BLINK_ERR_UNSUPPORTED,
                                  "set %d: the DFT transform needs 2 <= N <= %d, got %d", i,
                                  BLINK_MAX_DFT_PIXELS, st.n_pixels);
             continue;
         }
-        CHECK_ARG(st.n_pixels % 2 == 0, "set %d: the batched path needs an even grid side", i);
         FftPlan plan;
         if ((rc = check_fft_size(ctx, st.n_pixels, plan))) return rc;
     }

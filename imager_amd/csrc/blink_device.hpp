@@ -117,6 +117,15 @@ __device__ __forceinline__ float2 select_pol(const float4 xx_xy, const float4 yx
     return make_float2((xx_xy.x + yx_yy.z) / 2.0f, (xx_xy.y + yx_yy.w) / 2.0f);  // I
 }
 
+// Workgroup id in XCD-contiguous order.  Workgroups are dispatched to the 8
+// XCDs round-robin (id % 8); the returned logical ids give XCD x one
+// contiguous range, so neighbouring logical ids share an L2.  A bijection of
+// [0, nwg) for any nwg.
+__device__ __forceinline__ unsigned xcd_order(unsigned bid, unsigned nwg) {
+    const unsigned q = nwg >> 3, r = nwg & 7u, x = bid & 7u;
+    return x * q + min(x, r) + (bid >> 3);
+}
+
 // Streaming (non-temporal) accesses for data touched once per pass: the
 // row-pass buffer and the images.  They bypass the normal cache allocation,
 // leaving L2 to the visibilities and tables that are reused.

@@ -745,6 +745,75 @@ hipError_t launch_grid_tiles(const GatherArgs& g, const int* row_start, const ui
     return hipGetLastError();
 }
 
+// Hermitian half plane for the DFT transform: rows v = 0 .. N/2 of each image
+// of a chunk, gridded in LDS tiles of rb rows and written row-major
+// to half[li][v][u] (rows of a chunk contiguous); rows without entries are
+// written as zeros.  An entry in a row 0 < v < N/2 stands for its conjugate
+// partner in row N - v too, so it counts twice in the count sum.
+__global__ void __launch_bounds__(kTileThreads)
+k_grid_half_rows(GatherArgs g, const int* __restrict__ row_start,
+                 const uint32_t* __restrict__ entries, int n, int rb, int64_t img0,
+                 float2* __restrict__ half, uint32_t* __restrict__ isums) {
+    extern __shared__ float4 ghr_lds4[];
+    float4* tile4 = ghr_lds4;  // [rb][n] complex, as n/2 float4 per row
+    float2* tile = (float2*)ghr_lds4;
+    __shared__ int rs[kMaxTileRows + 1];
+    __shared__ int red[kTileThreads / 64];
+    const int h = n / 2;
+    const int n_rb = (h + 1 + rb - 1) / rb;
+    // XCD-aware order: each XCD gets a contiguous run of (image, tile) pairs,
+    // so the tiles of one image share that XCD's L2 copy of its records.
+    const unsigned bid = xcd_order(blockIdx.x, gridDim.x);
+    const int tile_id = (int)(bid % n_rb);
+    const int64_t li = bid / n_rb;
+    const int64_t img = img0 + li;
+    const int ch = (int)(img % g.n_ch);
+    const int r0 = tile_id * rb;
+    const int nr = min(rb, h + 1 - r0);
+    const int* rsg = row_start + (int64_t)ch * (n + 1);
+    if ((int)threadIdx.x <= nr) rs[threadIdx.x] = rsg[r0 + threadIdx.x];
+    __syncthreads();
+    float4* dst = (float4*)(half + (li * (h + 1) + r0) * (int64_t)n);
+    const int q = nr * n / 2;
+    if (rs[nr] == rs[0]) {  // no entries: zero rows
+        for (int i = threadIdx.x; i < q; i += blockDim.x) dst[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    for (int i = threadIdx.x; i < q; i += blockDim.x) tile4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    const uint32_t* ent = entries + (int64_t)ch * 2 * g.n_bl;
+    const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[nr], (int)threadIdx.x,
+                                     (int)blockDim.x, [&](int e, int u_idx, float2 v) {
+        const int r = row_of_entry(rs, nr, e);
+        atomicAdd(&tile[r * n + u_idx].x, v.x);
+        atomicAdd(&tile[r * n + u_idx].y, v.y);
+        const int row = r0 + r;
+        return (row == 0 || row == h) ? 1 : 2;
+    });
+    const int tot = block_sum_int(n_ok, red);
+    if (threadIdx.x == 0 && tot) atomicAdd(&isums[img], (uint32_t)tot);
+    __syncthreads();
+    for (int i = threadIdx.x; i < q; i += blockDim.x) dst[i] = tile4[i];
+}
+
+hipError_t launch_grid_half_rows(const GatherArgs& g, const int* row_start, const uint32_t* entries,
+                                 int n, int64_t n_img, int64_t img0, float* half,
+                                 uint32_t* isums, hipStream_t stream) {
+    if (n_img == 0) return hipSuccess;
+    if (n % 2) return hipErrorInvalidValue;
+    // Two rows per workgroup: the gather of plan entries is latency bound, and
+    // small tiles keep the most workgroups (and loads) in flight per CU; this
+    // measured fastest against 4 to 16 rows on 768-channel 256-antenna sets.
+    const int rb = 2;
+    const int64_t n_rb = (n / 2 + 1 + rb - 1) / rb;
+    const int64_t blocks = n_rb * n_img;
+    const size_t lds = sizeof(float2) * (size_t)rb * n;
+    if (!lds_fits(lds) || blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k_grid_half_rows, dim3((unsigned)blocks), dim3(kTileThreads), lds, stream, g,
+                       row_start, entries, n, rb, img0, (float2*)half, isums);
+    return hipGetLastError();
+}
+
 // grid /= count per cell, the "U" weighting of src/pacer_imager.cpp:421-428.
 __global__ void k_uniform_weight(float2* __restrict__ grids, const float* __restrict__ counts,
                                  int64_t n) {
@@ -1231,10 +1300,13 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
     const int team = fft_team_for(n);
     const bool block_team = team == kFftThreads;
     const int teams = block_team ? 1 : kFftThreads / team;
-    // rows per tile: as many as fit, but at least ~2048 workgroups in the
-    // launch, so that a few images still spread over the whole chip
+    // rows per tile: as many as fit up to 4, and at least ~2048 workgroups in
+    // the launch, so that a few images still spread over the whole chip.  The
+    // gather of plan entries is latency bound: small tiles keep more
+    // workgroups in flight (4 rows measured fastest against 2 to 16 at 180^2).
     const int64_t want = (int64_t)(n / 2 + 1) * n_img / 2048;
-    const int rb = (int)std::max<int64_t>(1, std::min<int64_t>(grid_rows_tile_rows(n), want));
+    const int rb = (int)std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)grid_rows_tile_rows(n), want, 4}));
     const int n_rb = (n / 2 + 1 + rb - 1) / rb;
     const size_t lds = sizeof(float2) * ((size_t)rb * n + (size_t)teams * n);
     if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
@@ -1322,6 +1394,7 @@ hipError_t set_kernel_attributes() {
     set((const void*)k_cols_half<1, true, true>);
     set((const void*)k_cols_half<1, false, true>);
     set((const void*)k_grid_tiles);
+    set((const void*)k_grid_half_rows);
     set((const void*)k_plan_rows);
     set((const void*)k_plan_count);
     set((const void*)k_plan_scatter);

@@ -159,13 +159,25 @@ hipError_t launch_plan_units(const SetDesc* sets, const PlanUnit* units, int n_u
                              int n_max, int* partial, uint32_t* cells, hipStream_t stream);
 hipError_t launch_isums_to_double(const uint32_t* isums, double* sums, int n, hipStream_t stream);
 
-// Split-precision DFT-as-GEMM (blink_dft_mfma.hip): out = (D' applied to the
-// rows of a), stored transposed per image, scaled by 1/sum when sums are given.
-hipError_t launch_dft_gemm(const float* a, const uint16_t* planes, float* out, int n, int64_t n_img,
-                           int64_t img0, const uint32_t* isums, const double* sums,
+// Split-precision DFT-as-GEMM (blink_dft_mfma.hip) over the Hermitian half
+// plane: half[img][v][u] (v = 0 .. N/2, row-major) -> stage 1 (complex) ->
+// inter[img][x][v] (row stride dft_inter_stride(n)) -> stage 2 (real, scaled
+// by 1/sum counts, fftshifted, imaginary parts 0) -> images[img][y][x].
+enum { kDftBf16 = 0, kDftFp16 = 1 };
+int dft_inter_stride(int n);
+hipError_t launch_grid_half_rows(const GatherArgs& g, const int* row_start, const uint32_t* entries,
+                                 int n, int64_t n_img, int64_t img0, float* half,
+                                 uint32_t* isums, hipStream_t stream);
+hipError_t launch_dft_rows(const float* half, const uint16_t* planes1, float* inter, int n,
+                           int64_t n_img, int prec, hipStream_t stream);
+hipError_t launch_dft_cols(const float* inter, const uint16_t* planes2, float* images, int n,
+                           int64_t n_img, int64_t img0, const uint32_t* isums, int prec,
                            hipStream_t stream);
-void make_dft_planes(int n, std::vector<uint16_t>& planes);
-size_t dft_planes_bytes(int n);
+// Allows the DFT kernels' LDS-resident matrix slices (up to 160 KiB).
+hipError_t set_dft_kernel_attributes();
+// Split planes of D1 (stage 1) or D2 (stage 2) for side n and precision prec.
+void make_dft_planes(int n, int stage, int prec, std::vector<uint16_t>& planes);
+size_t dft_planes_bytes(int n, int stage);
 
 // Register-FFT batched kernels (power-of-two sides 128..2048); the row-pass
 // buffer is transposed: inter[img][u][v].

@@ -69,10 +69,16 @@ enum blink_corrections { BLINK_CORR_GEOMETRIC = 1u, BLINK_CORR_CABLE = 2u };
  * BACKWARD, src/gpu/pacer_imager_hip.cpp:95), unnormalised. */
 enum blink_fft_direction { BLINK_FFT_FORWARD = -1, BLINK_FFT_BACKWARD = 1 };
 
-/* 2-D transform of blink_image_batch.  FFT: in-house FP32 FFT.  DFT_BF16X3:
- * image = D G D^T as two dense GEMMs on MFMA with every FP32 operand split
- * into bf16 hi + lo and FP32 accumulation (N <= 512). */
-enum blink_transform { BLINK_TRANSFORM_FFT = 0, BLINK_TRANSFORM_DFT_BF16X3 = 1 };
+/* 2-D transform of blink_image_batch.  FFT: in-house FP32 FFT.  DFT_BF16X3 /
+ * DFT_FP16X3: image = D G D^T as two dense GEMMs on MFMA over the Hermitian
+ * half of the grid, with every FP32 operand split into hi + lo (bf16, or fp16
+ * with a power-of-two scale per data row), products hi*hi + hi*lo + lo*hi and
+ * FP32 accumulation (even N <= 512). */
+enum blink_transform {
+    BLINK_TRANSFORM_FFT = 0,
+    BLINK_TRANSFORM_DFT_BF16X3 = 1,
+    BLINK_TRANSFORM_DFT_FP16X3 = 2
+};
 #define BLINK_MAX_DFT_PIXELS 512
 
 /* Gridding algorithm for blink_gridding. */
@@ -238,10 +244,9 @@ typedef struct blink_image_set {
     int n_pixels;
     int pol;                 /* blink_pol */
     float* images;           /* [n_t][n_ch][N][N] complex64 output, 16-byte aligned.  The
-                                gridded plane is Hermitian, so the images are real: the FFT
-                                transform at N = 128 .. 2048 (powers of two) writes imaginary
-                                parts 0; other sides and transforms leave FP32 rounding
-                                noise there, as the reference's FFT does */
+                                gridded plane is Hermitian, so the images are real and every
+                                transform writes imaginary parts 0 (the reference's FFT leaves
+                                FP32 rounding noise there) */
     double* count_sums;      /* optional [n_t*n_ch] output: sum of counts per image */
     int transform;           /* blink_transform: how the 2-D transform is computed */
 } blink_image_set;

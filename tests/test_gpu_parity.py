@@ -537,15 +537,25 @@ def test_error_paths(ctx):
 
 # ------------------------------------------ split-precision DFT on MFMA
 
-@pytest.mark.parametrize("n,n_ant", [(64, 32), (180, 128), (256, 128), (512, 64), (45, 24)])
-def test_image_batch_dft_bf16x3(ctx, n, n_ant):
-    """image = D G D^T on MFMA with bf16 hi+lo operands: within 1e-5
-    (peak-normalised) of the FP32 oracle; its error to the FP64 oracle is
-    printed next to the FP32 FFT path's."""
+DFT_TRANSFORMS = {"bf16x3": "TRANSFORM_DFT_BF16X3", "fp16x3": "TRANSFORM_DFT_FP16X3"}
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("n,n_ant,flux", [(64, 32, 1.0), (180, 128, 1.0), (256, 128, 1.0),
+                                          (512, 64, 1.0), (90, 24, 1.0), (128, 48, 1e6),
+                                          (128, 48, 1e-6)])
+def test_image_batch_dft(ctx, prec, n, n_ant, flux):
+    """image = D G D^T on MFMA over the Hermitian half plane with split
+    bf16 or fp16 hi+lo operands: within 1e-5 (peak-normalised) of the FP32
+    oracle and of the FP64 oracle, imaginary parts exactly 0, count sums
+    exact.  Fluxes scaled by 1e6 and 1e-6 take fp16 outside its exponent
+    range without the per-row power-of-two scale.  The error to the FP64
+    oracle is printed next to the FP32 FFT path's."""
     import torch
     from imager_amd import capi
     o = synth.small(n_ant=n_ant, n_ch=3, n_t=2, n_pixels=n, seed=900 + n, height_sigma_m=1.0)
-    _say("dft", n, n_ant)
+    o.vis *= np.float32(flux)
+    _say("dft", prec, n, n_ant, flux)
     ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
                             o.delta_u, o.delta_v, o.min_uv, n)
     ref64 = oracle.run_f64(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
@@ -555,23 +565,24 @@ def test_image_batch_dft_bf16x3(ctx, n, n_ant):
     sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
     st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
                        d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, n, "XX", imgs,
-                       sums, transform=capi.TRANSFORM_DFT_BF16X3)
+                       sums, transform=getattr(capi, DFT_TRANSFORMS[prec]))
     ctx.image_batch([st])
     got = _host(imgs)
     e32 = oracle.peak_normalised_error(got, ref)
     e64 = oracle.peak_normalised_error(got, ref64)
-    line = f"  DFT bf16x3 N={n}: vs FP32 oracle {e32[0]:.2e}, vs FP64 max {e64[0]:.2e} rms {e64[1]:.2e}"
-    if n % 2 == 0:
-        imgs2 = torch.empty_like(imgs)
-        st2 = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
-                            d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, n, "XX",
-                            imgs2)
-        ctx.image_batch([st2])
-        f64 = oracle.peak_normalised_error(_host(imgs2), ref64)
-        line += f"; FP32 FFT vs FP64 max {f64[0]:.2e} rms {f64[1]:.2e}"
+    line = (f"  DFT {prec} N={n} flux {flux:g}: vs FP32 oracle {e32[0]:.2e}, "
+            f"vs FP64 max {e64[0]:.2e} rms {e64[1]:.2e}")
+    imgs2 = torch.empty_like(imgs)
+    st2 = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                        d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, n, "XX",
+                        imgs2)
+    ctx.image_batch([st2])
+    f64 = oracle.peak_normalised_error(_host(imgs2), ref64)
+    line += f"; FP32 FFT vs FP64 max {f64[0]:.2e} rms {f64[1]:.2e}"
     _say(line)
     assert e32[0] < IMAGE_TOL
     assert e64[0] < IMAGE_TOL
+    assert np.all(got.imag == 0)
     _, c_ref, _ = oracle.gridding(oracle.cable_corrections(
         oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq), o.n_ant, o.cable, o.freq),
         o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v, o.min_uv, n, "XX")
