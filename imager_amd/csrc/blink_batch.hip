@@ -192,7 +192,8 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (N + 1);
     float2* twl = tile + TEAMS * TS;  // twiddle table copy [N]
     for (int i = tid; i < N; i += NT) {
-        nonempty[i] = rsg[i + 1] != rsg[i];
+        const int j = i <= N / 2 ? i : N - i;  // the plan holds rows 0 .. N/2
+        nonempty[i] = rsg[j + 1] != rsg[j];
         twl[i] = tw[i];
     }
     __syncthreads();

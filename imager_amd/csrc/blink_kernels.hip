@@ -505,8 +505,12 @@ k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ unit
         if (!f && plan_cell(u, v, wl, sd, up, vp)) {
             const int r = wrap_index(vp, n);
             packed = ((uint32_t)r << 16) | (uint32_t)wrap_index(up, n);
-            atomicAdd(&hist[r], 1);
-            atomicAdd(&hist[r == 0 ? 0 : n - r], 1);
+            // only rows 0 .. N/2 are planned (the consumers rebuild the rest
+            // from the Hermitian symmetry): the cell or its conjugate, or both
+            // in rows 0 and N/2
+            const int r2 = r == 0 ? 0 : n - r;
+            if (r <= n / 2) atomicAdd(&hist[r], 1);
+            if (r2 <= n / 2) atomicAdd(&hist[r2], 1);
         }
         cl[bl] = packed;
     }
@@ -614,13 +618,17 @@ k_plan_scatter(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ un
         const uint32_t packed = cl[bl];
         if (packed == kNoCell) continue;
         const int r = (int)(packed >> 16), u = (int)(packed & 0xffffu);
-        const int p = lstart[r] + atomicAdd(&lfill[r], 1);
-        lent[p] = (uint32_t)u | ((uint32_t)bl << 14);
-        lrow[p] = (unsigned short)r;
+        if (r <= n / 2) {
+            const int p = lstart[r] + atomicAdd(&lfill[r], 1);
+            lent[p] = (uint32_t)u | ((uint32_t)bl << 14);
+            lrow[p] = (unsigned short)r;
+        }
         const int r2 = r == 0 ? 0 : n - r, u2 = u == 0 ? 0 : n - u;
-        const int q = lstart[r2] + atomicAdd(&lfill[r2], 1);
-        lent[q] = (uint32_t)u2 | (1u << 13) | ((uint32_t)bl << 14);
-        lrow[q] = (unsigned short)r2;
+        if (r2 <= n / 2) {
+            const int q = lstart[r2] + atomicAdd(&lfill[r2], 1);
+            lent[q] = (uint32_t)u2 | (1u << 13) | ((uint32_t)bl << 14);
+            lrow[q] = (unsigned short)r2;
+        }
     }
     __syncthreads();
     uint32_t* ent = sd.entries + (int64_t)pu.ch * 2 * n_bl;

@@ -147,7 +147,10 @@ hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int
                             int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
                             const float* inter, const float* tw, hipStream_t stream);
 // Gridding plans of a batched call: one unit per (set, channel); `parts`
-// workgroups per unit split its baselines (at most kPlanPartMax each).
+// workgroups per unit split its baselines (at most kPlanPartMax each).  A plan
+// lists the cells of rows 0 .. N/2 only (each baseline's cell and conjugate
+// cell that fall there): the batched kernels grid the Hermitian half plane,
+// and rows N/2+1 .. N-1 of row_start are left empty.
 // Scratch: partial is [n_units][parts][n_max] ints, cells holds one packed
 // cell per (unit, baseline) from cell_off on.
 constexpr int kPlanPartMax = 2048;
