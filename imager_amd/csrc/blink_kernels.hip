@@ -433,6 +433,9 @@ hipError_t launch_plan_rows(const float* u_m, const float* v_m, const int* flags
 
 constexpr int kUnitThreads = 256;
 constexpr uint32_t kNoCell = 0xffffffffu;
+// baselines per thread in a plan part (parts hold at most kPlanPartMax)
+constexpr int kPlanPerThread = kPlanPartMax / kUnitThreads;
+static_assert(kPlanPartMax % kUnitThreads == 0, "plan parts split evenly over the threads");
 
 __device__ __forceinline__ void part_range(int n_bl, int parts, int part, int& b0, int& b1) {
     const int per = (n_bl + parts - 1) / parts;
@@ -441,16 +444,32 @@ __device__ __forceinline__ void part_range(int n_bl, int parts, int part, int& b
 }
 
 // cell_of with the min_uv test reduced to a NaN test when min_uv is -inf
-// (sqrt(u^2 + v^2) > -inf holds for every non-NaN u, v): same results.
-__device__ __forceinline__ bool plan_cell(float u_m, float v_m, double wl, const SetDesc& sd,
-                                          int& up, int& vp) {
+// (sqrt(u^2 + v^2) > -inf holds for every non-NaN u, v) and the divisions by
+// the wavelength and the cell sizes done with div_rn and per-unit reciprocals:
+// the same quotients, so the same cells.  Operands outside div_rn_range (a
+// zero, huge, infinite or NaN coordinate or cell size) take the IEEE division.
+__device__ __forceinline__ bool plan_cell(float u_m, float v_m, double wl, double r_wl,
+                                          double r_du, double r_dv, bool unit_fast,
+                                          const SetDesc& sd, int& up, int& vp) {
 #pragma clang fp contract(off)
     if (sd.min_uv == -INFINITY) {
-        const double u = (double)u_m / wl;
-        const double v = (double)v_m / wl;
+        const double um = (double)u_m, vm = (double)v_m;
+        const bool fast = unit_fast && div_rn_range(um) && div_rn_range(vm);
+        double u, v, qu, qv;
+        if (fast) {
+            u = div_rn(um, wl, r_wl);
+            v = div_rn(vm, wl, r_wl);
+            qu = div_rn(u, sd.du, r_du);
+            qv = div_rn(v, sd.dv, r_dv);
+        } else {
+            u = um / wl;
+            v = vm / wl;
+            qu = u / sd.du;
+            qv = v / sd.dv;
+        }
         if (isnan(u) || isnan(v)) return false;
-        const double ru = round(u / sd.du);
-        const double rv = round(v / sd.dv);
+        const double ru = round(qu);
+        const double rv = round(qv);
         if (!(fabs(ru) < 1073741824.0) || !(fabs(rv) < 1073741824.0)) return false;
         up = (int)ru;
         vp = (int)rv;
@@ -489,6 +508,8 @@ k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ unit
     const SetDesc sd = sets[pu.set];
     const int n = sd.n_pixels, n_bl = sd.g.n_bl;
     const double wl = wavelength_of(sd.g.freq_hz[pu.ch]);
+    const double r_wl = 1.0 / wl, r_du = 1.0 / sd.du, r_dv = 1.0 / sd.dv;
+    const bool unit_fast = div_rn_range(wl) && div_rn_range(sd.du) && div_rn_range(sd.dv);
     for (int i = threadIdx.x; i < n; i += kUnitThreads) hist[i] = 0;
     __syncthreads();
     int b0, b1;
@@ -497,12 +518,26 @@ k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ unit
     if (pu.ch == 0 && sd.g.corrections)  // one unit per set writes the phase coefficients
         for (int bl = b0 + (int)threadIdx.x; bl < b1; bl += kUnitThreads)
             sd.coef_out[bl] = phase_coefficient(sd.g.corrections, sd.w_m, sd.cable_m, (unsigned)bl);
-    for (int bl = b0 + (int)threadIdx.x; bl < b1; bl += kUnitThreads) {
-        const int f = sd.flags[bl];
-        const float u = sd.u_m[bl], v = sd.v_m[bl];
+    // at most kPlanPerThread baselines per thread: every load is issued first
+    int fl[kPlanPerThread];
+    float uu[kPlanPerThread], vv[kPlanPerThread];
+#pragma unroll
+    for (int k = 0; k < kPlanPerThread; k++) {
+        const int bl = b0 + (int)threadIdx.x + k * kUnitThreads;
+        fl[k] = 1;
+        if (bl < b1) {
+            fl[k] = sd.flags[bl];
+            uu[k] = sd.u_m[bl];
+            vv[k] = sd.v_m[bl];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kPlanPerThread; k++) {
+        const int bl = b0 + (int)threadIdx.x + k * kUnitThreads;
+        if (bl >= b1) break;
         int up, vp;
         uint32_t packed = kNoCell;
-        if (!f && plan_cell(u, v, wl, sd, up, vp)) {
+        if (!fl[k] && plan_cell(uu[k], vv[k], wl, r_wl, r_du, r_dv, unit_fast, sd, up, vp)) {
             const int r = wrap_index(vp, n);
             packed = ((uint32_t)r << 16) | (uint32_t)wrap_index(up, n);
             // only rows 0 .. N/2 are planned (the consumers rebuild the rest
@@ -614,8 +649,16 @@ k_plan_scatter(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ un
     int b0, b1;
     part_range(n_bl, parts, part, b0, b1);
     const uint32_t* cl = cells + pu.cell_off;
-    for (int bl = b0 + tid; bl < b1; bl += kUnitThreads) {
-        const uint32_t packed = cl[bl];
+    uint32_t pk[kPlanPerThread];
+#pragma unroll
+    for (int k = 0; k < kPlanPerThread; k++) {
+        const int bl = b0 + tid + k * kUnitThreads;
+        pk[k] = bl < b1 ? cl[bl] : kNoCell;
+    }
+#pragma unroll
+    for (int k = 0; k < kPlanPerThread; k++) {
+        const int bl = b0 + tid + k * kUnitThreads;
+        const uint32_t packed = pk[k];
         if (packed == kNoCell) continue;
         const int r = (int)(packed >> 16), u = (int)(packed & 0xffffu);
         if (r <= n / 2) {

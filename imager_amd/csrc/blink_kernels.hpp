@@ -6,6 +6,7 @@
 
 #include <vector>
 
+#include "blink_divrn.hpp"
 #include "blink_fft.hpp"
 
 namespace blink {

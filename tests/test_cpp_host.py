@@ -23,7 +23,8 @@ def test_cpp_program(prog):
 
 
 def test_cpp_programs_are_built():
-    for prog in ("test_gpu_utils", "test_corrections", "test_imager", "test_fits"):
+    for prog in ("test_gpu_utils", "test_corrections", "test_imager", "test_fits",
+                 "test_div_rn"):
         assert os.path.exists(os.path.join(BIN, prog)), prog
 
 
@@ -31,6 +32,15 @@ def test_cpp_fits_io():
     """FITS reader / writer and the correlation-matrix conversion (no GPU)."""
     exe = os.path.join(BIN, "test_fits")
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "All tests passed." in r.stdout
+
+
+def test_cpp_div_rn():
+    """The gridding plans' reciprocal-based FP64 division equals IEEE division
+    on 2e7 random operand pairs (no GPU)."""
+    r = subprocess.run([os.path.join(BIN, "test_div_rn")], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "All tests passed." in r.stdout
 
