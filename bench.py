@@ -403,6 +403,21 @@ def main():
             roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                     "images_per_launch": imgs_per_launch, "alg_bytes_per_launch": alg}
+    # the DFT-as-GEMM stages against the dense MFMA peak, whichever kernel dominates
+    dft_mfma = None
+    if N is not None and any(k in kernels for k in ("dft_rows", "dft_cols")):
+        dft_mfma = {}
+        n_img = -(-job["images_per_step"] // max(kernels.get("dft_rows", {}).get(
+            "launches_per_step", 1), 1))
+        for k in ("dft_rows", "dft_cols"):
+            if k not in kernels:
+                continue
+            useful, executed = dft_flops(k, N, n_img)
+            avg_s = kernels[k]["avg_ms"] * 1e-3
+            dft_mfma[k] = {"executed_tflops": executed / avg_s / 1e12,
+                           "useful_tflops": useful / avg_s / 1e12,
+                           "frac": executed / avg_s / 1e12 / 2500.0, "peak": 2500.0,
+                           "images_per_launch": n_img, "avg_us": avg_s * 1e6}
     pipe_bytes = sum(16.0 * o.n_bl + 28.0 * o.n_pixels ** 2 for o in job["pipeline_obs"])
     pipe_bytes *= job["pipeline_repeat"]
     pipeline = {"alg_bytes_per_step": pipe_bytes,
@@ -509,6 +524,7 @@ def main():
         "data": job["data"],
         "config": config,
         "roofline": roof,
+        "dft_mfma": dft_mfma,
         "pipeline_roofline": pipeline,
         "cpu_baseline": cpu,
         "plain_gpu": plain,
