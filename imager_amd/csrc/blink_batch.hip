@@ -3,12 +3,13 @@
 //
 //   k_grid_rows_reg : (row tile, image) -> plan entries gathered (pol, phase
 //                     correction, NaN/MAX_VIS) and accumulated in LDS, each
-//                     non-empty row transformed, the tile written TRANSPOSED
-//                     to the row-pass buffer inter[img][x][v];
-//   k_cols_reg      : (column block, image) -> each column read contiguously
-//                     from inter (rows with no plan entries read as zero),
-//                     transformed, scaled by (float)(1/sum counts) and stored
-//                     fft-shifted into images[img][y][x].
+//                     non-empty row transformed and written row-major to the
+//                     row-pass buffer inter[img][v][x] (whole lines);
+//   k_cols_reg      : (column block, image) -> the block's column pairs staged
+//                     through LDS from row segments of inter (rows with no
+//                     plan entries read as zero), transformed, scaled by
+//                     (float)(1/sum counts) and stored fft-shifted into
+//                     images[img][y][x].
 //
 // Hermitian half: every visibility is gridded at (u, v) and conjugated at
 // (-u, -v), so the grid satisfies G[-v][-u] = conj(G[v][u]) and the row
@@ -32,10 +33,6 @@ __host__ __device__ constexpr int tile_stride(int n, int rows) {
     const int p = padded_len(n);
     return p + ((want - p % 32) + 32) % 32;
 }
-
-// Column stride of the row-pass buffer inter[img][x][v], v = 0 .. N/2:
-// N/2 + 16 elements keeps every column 128-byte aligned.
-__host__ __device__ constexpr int half_stride(int n) { return n / 2 + 16; }
 
 // Lanes per row: 16 values per lane up to N = 1024, 32 at N = 2048.
 template <int N>
@@ -135,6 +132,9 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();
 
+    // each team transforms its row and stores it from registers, row-major
+    // (consecutive lanes write consecutive elements: whole 128-byte lines);
+    // rows without entries are not written
     const int team = tid / L, lane = tid % L;
     float2* row = tile + team * TS;
     if (rs[team + 1] != rs[team]) {
@@ -142,23 +142,8 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
         team_sync<false>();
         F::template run<1>(v, row, lane, TwTable{twl});
-        F::store(v, lane, [&](int i, float2 x) { row[pad_index(i)] = x; });
-    }
-    __syncthreads();
-    // transposed store: inter[li][c][r0 + rr], RB consecutive rows per column,
-    // two rows (16 bytes) per thread; a pair is written when either row has
-    // entries (an empty row of the tile holds zeros; uniform 16-byte stores
-    // beat skipping it).
-    constexpr int HS = half_stride(N);
-    float2* dst = inter + (int64_t)li * N * HS + r0;
-    constexpr int RP = RB / 2;
-    for (int i = tid; i < RP * N; i += NT) {
-        const int c = i / RP, rr = 2 * (i % RP);
-        if (rs[rr + 2] != rs[rr]) {
-            const float2 a = tile[rr * TS + pad_index(c)];
-            const float2 b = tile[(rr + 1) * TS + pad_index(c)];
-            store_stream((float4*)&dst[(int64_t)c * HS + rr], make_float4(a.x, a.y, b.x, b.y));
-        }
+        float2* dst = inter + ((int64_t)li * (H + 1) + r0 + team) * N;
+        F::store(v, lane, [&](int i, float2 x) { store_stream(&dst[i], x); });
     }
 }
 
@@ -179,7 +164,8 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     using F = RegFft<N, L>;
     extern __shared__ float4 cols_reg_lds4[];
     float2* tile = (float2*)cols_reg_lds4;  // [TEAMS][TS]
-    __shared__ unsigned char nonempty[N];
+    constexpr int H = N / 2;
+    __shared__ unsigned char nonempty[H + 1];
 
     constexpr int n_cb = N / CB;
     const int cb = blockIdx.x % n_cb;
@@ -191,32 +177,43 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     const uint32_t count = sd.isums[img];
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (N + 1);
     float2* twl = tile + TEAMS * TS;  // twiddle table copy [N]
-    for (int i = tid; i < N; i += NT) {
-        const int j = i <= N / 2 ? i : N - i;  // the plan holds rows 0 .. N/2
-        nonempty[i] = rsg[j + 1] != rsg[j];
-        twl[i] = tw[i];
+    for (int i = tid; i < N; i += NT) twl[i] = tw[i];
+    for (int i = tid; i <= H; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
+    __syncthreads();
+
+    // stage z_t[v] and z_t[N - v] from inter[li][v][c0 + 2t .. c0 + 2t + 1]:
+    // consecutive threads read consecutive 16-byte column pairs of one row
+    // (CB * 8 contiguous bytes per row); rows without plan entries are zero
+    // (all of a thread's loads are issued before its LDS writes)
+    const int c0 = cb * CB;
+    const float2* src = inter + (int64_t)li * (H + 1) * N + c0;
+    constexpr int STAGE = (H + 1) * TEAMS, ITER = (STAGE + NT - 1) / NT;
+    float4 p[ITER];
+#pragma unroll
+    for (int k = 0; k < ITER; k++) {
+        const int i = tid + k * NT;
+        const int vr = i / TEAMS, t = i % TEAMS;
+        p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < STAGE && nonempty[vr]) p[k] = *(const float4*)&src[(int64_t)vr * N + 2 * t];
+    }
+#pragma unroll
+    for (int k = 0; k < ITER; k++) {
+        const int i = tid + k * NT;
+        if (i >= STAGE) break;
+        const int vr = i / TEAMS, t = i % TEAMS;
+        // a = c_x[v] = (p.x, p.y), b = c_{x+1}[v] = (p.z, p.w); row N - v holds conjugates
+        float2* zr = tile + t * TS;
+        zr[pad_index(vr)] = make_float2(p[k].x - p[k].w, p[k].y + p[k].z);
+        if (vr > 0 && vr < H)
+            zr[pad_index(N - vr)] = make_float2(p[k].x + p[k].w, p[k].z - p[k].y);
     }
     __syncthreads();
 
     const int team = tid / L, lane = tid % L;
-    const int c0 = cb * CB;
-    constexpr int HS = half_stride(N);
-    const float2* src = inter + ((int64_t)li * N + c0 + 2 * team) * HS;
-    float2 v[F::V];
-    // Row i > N/2 is the conjugate of stored row N - i.  Rows without plan
-    // entries read element 0 (one cached line) and are cleared with a bit
-    // mask, so every load is unconditional and all are in flight together.
-    F::load(v, lane, [&](int i) {
-        const int m = nonempty[i] ? -1 : 0;
-        const int j = (i <= N / 2 ? i : N - i) & m;
-        const float sg = i <= N / 2 ? 1.f : -1.f;
-        const float2 a = src[j];       // column c0 + 2 team
-        const float2 b = src[HS + j];  // column c0 + 2 team + 1
-        const float re = a.x - sg * b.y, im = sg * a.y + b.x;
-        return make_float2(__int_as_float(__float_as_int(re) & m),
-                           __int_as_float(__float_as_int(im) & m));
-    });
     float2* row = tile + team * TS;
+    float2 v[F::V];
+    F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
+    team_sync<false>();
     F::template run<1>(v, row, lane, TwTable{twl});
     const float s = (float)(1.0 / (double)count);
 #pragma unroll
@@ -227,7 +224,6 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     // shifted store: images[img][shift(r)][shift(c0 + 2 t) + {0, 1}] =
     // (Re Z_t[r], 0), (Im Z_t[r], 0) -- 16 bytes per thread, CB columns per row
     float2* out = sd.images + (int64_t)img * N * N;
-    constexpr int H = N / 2;
     const int oc0 = c0 < H ? c0 + H : c0 - H;  // shift of a block that does not straddle N/2
     for (int i = tid; i < TEAMS * N; i += NT) {
         const int r = i / TEAMS, t = i % TEAMS;
@@ -245,12 +241,12 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
 // ------------------------------------------------------------- N = 4096
 //
 // At N = 4096 one row needs 4 wavefronts (256 lanes x 16 values) and 34 KiB
-// of LDS, so a workgroup holds one to four rows or column pairs.  Too few
-// rows per tile for 64-byte segments of a transposed store, so the row-pass
-// buffer is row-major here: inter[img][v][x], v = 0 .. N/2.  The row kernel
-// stores each transformed row with unit-stride lanes; the column kernel stages
-// its CB columns of rows 0 .. N/2 through LDS (16-byte loads, two columns per
-// thread), forming z = c_x + i c_{x+1} for rows v and N - v on the way.
+// of LDS, so a workgroup holds one to four rows or column pairs.  The row-pass
+// buffer is row-major as for the smaller sides: inter[img][v][x], v = 0 ..
+// N/2.  The row kernel stores each transformed row with unit-stride lanes; the
+// column kernel stages its CB columns of rows 0 .. N/2 through LDS (16-byte
+// loads, two columns per thread), forming z = c_x + i c_{x+1} for rows v and
+// N - v on the way.
 // Twiddles come from two 64-entry tables (w(m) = lo[m % 64] hi[m / 64]) kept
 // in LDS, instead of a 32 KiB table.
 
@@ -497,8 +493,7 @@ bool batch_reg_supported(int n) {
 }
 
 int64_t batch_inter_bytes(int n) {
-    if (n >= 4096) return (int64_t)sizeof(float2) * (n / 2 + 1) * n;  // row-major
-    return (int64_t)sizeof(float2) * n * half_stride(n);                // transposed
+    return (int64_t)sizeof(float2) * (n / 2 + 1) * n;  // row-major, rows 0 .. N/2
 }
 
 int batch_row_variants(int n, int* tiles, int max_tiles) {
