@@ -290,6 +290,9 @@ def main():
                     help="2-D transform of the batched path: in-house FP32 FFT, or the "
                          "split-precision DFT-as-GEMM on MFMA (N <= 512) with bf16 (dft) "
                          "or fp16 (dft-fp16) hi+lo operands")
+    ap.add_argument("--batch-tiles", action="append", default=[], metavar="N:ROWS:COLS",
+                    help="pin the register kernels' row tile and column block for side N "
+                         "(blink_set_batch_tiles) instead of autotuning; repeatable")
     ap.add_argument("--plain-images", type=int, default=64,
                     help="images run through the plain one-image-at-a-time GPU path")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -327,6 +330,9 @@ def main():
     wl = make_workload(args.workload)
     log(f"workload {args.workload} generated in {time.perf_counter() - t_gen:.1f} s")
     ctx = capi.Context(0)
+    for spec in args.batch_tiles:
+        n_t, r_t, c_t = (int(x) for x in spec.split(":"))
+        ctx.set_batch_tiles(n_t, r_t, c_t)
     transform = {"fft": capi.TRANSFORM_FFT, "dft": capi.TRANSFORM_DFT_BF16X3,
                  "dft-fp16": capi.TRANSFORM_DFT_FP16X3}[args.transform]
     job = build_job(args.workload, wl, transform)
