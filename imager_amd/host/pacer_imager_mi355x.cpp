@@ -455,7 +455,7 @@ Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
     st.pol = (int)pol_to_image;
     st.images = (float*)images_buffer.data();
     st.count_sums = nullptr;
-    st.transform = BLINK_TRANSFORM_FFT;
+    st.transform = m_nBatchTransform;
     check(blink_image_batch(ctx_, &st, 1, nullptr));
     const double pixscale_ra = (1.00 / (oversampling_factor * u_max)) * (180.00 / kPi);
     const double pixscale_dec = (1.00 / (oversampling_factor * v_max)) * (180.00 / kPi);

@@ -147,6 +147,11 @@ class CPacerImagerMI355X {
     // (src/pacer_imager.h:168-169)
     GridMode _gridMode{GridMode::KaiserBesselKernel};
     int m_nConvolvingKernelSize{0};
+    // 2-D transform of run_batched(): BLINK_TRANSFORM_FFT (in-house FP32 FFT),
+    // or the split-precision DFT-as-GEMM on MFMA for n_pixels <= 512,
+    // BLINK_TRANSFORM_DFT_BF16X3 / BLINK_TRANSFORM_DFT_FP16X3 (no reference
+    // counterpart; the reference always uses FFTW / hipFFT)
+    int m_nBatchTransform{BLINK_TRANSFORM_FFT};
 
     CPacerImagerMI355X(const std::string metadata_file, int n_pixels,
                        const std::vector<int>& flagged_antennas, bool average_images = false,

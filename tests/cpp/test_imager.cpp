@@ -274,15 +274,22 @@ void test_eda2_correlation_matrices() {
         a.cable.push_back(m.cableLenDelta);
     }
     const std::vector<double> freq{159.375e6};  // channel 204 x 0.78125 MHz
-    for (int batched = 0; batched < 2; batched++) {
+    // run(), then run_batched() with the FFT and both split-precision DFTs
+    const int transforms[] = {-1, BLINK_TRANSFORM_FFT, BLINK_TRANSFORM_DFT_BF16X3,
+                              BLINK_TRANSFORM_DFT_FP16X3};
+    const char* names[] = {"EDA2 matrices, run", "EDA2 matrices, run_batched FFT",
+                           "EDA2 matrices, run_batched DFT bf16x3",
+                           "EDA2 matrices, run_batched DFT fp16x3"};
+    for (int k = 0; k < 4; k++) {
         CPacerImagerMI355X imager(g + "antenna_locations.txt", 180, {}, false, Polarization::XX,
                                   2.0f);
         imager.m_fFrequencyMHz = 159.375;
+        if (transforms[k] >= 0) imager.m_nBatchTransform = transforms[k];
         Visibilities copy(MemoryBuffer<std::complex<float>>(xcorr.size()), info, 1, 1);
         std::memcpy((void*)copy.data(), vis0.data(), sizeof(float) * vis0.size());
-        Images images = batched ? imager.run_batched(copy) : imager.run(copy);
+        Images images = transforms[k] >= 0 ? imager.run_batched(copy) : imager.run(copy);
         check_against_oracle(imager, images, a, vis0, freq, 1, 1, true, true, ORACLE_POL_XX,
-                             1e-5, batched ? "EDA2 matrices, run_batched" : "EDA2 matrices, run");
+                             1e-5, names[k]);
     }
 }
 
