@@ -409,6 +409,12 @@ def main():
             roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                     "images_per_launch": imgs_per_launch, "alg_bytes_per_launch": alg}
+            if traffic:
+                # the bytes the kernel actually moves (PMC) over the same launch time: the
+                # physical bandwidth, below the algorithmic figure when the fused kernels
+                # skip empty rows and the Hermitian half
+                roof["traffic_gbs"] = traffic / avg_s / 1e9
+                roof["traffic_frac"] = traffic / avg_s / 1e9 / HBM_PEAK_GBS
     # the DFT-as-GEMM stages against the dense MFMA peak, whichever kernel dominates
     dft_mfma = None
     if N is not None and any(k in kernels for k in ("dft_rows", "dft_cols")):
