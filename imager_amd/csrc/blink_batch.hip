@@ -67,11 +67,15 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     using F = RegFft<N, L>;
     extern __shared__ float4 rows_reg_lds4[];
     float2* tile = (float2*)rows_reg_lds4;  // [RB][TS]
-    __shared__ int rs[RB + 1];
+    __shared__ int rs[RB + 1];   // first entry of each tile row, then the end
+    __shared__ int rrow[RB];     // grid row of each tile row
     __shared__ int red[NT / 64 + 1];
 
+    // A tile is RB consecutive NON-EMPTY rows of the plan's row list, so no
+    // workgroup carries rows without entries; tiles past the last non-empty
+    // row find an empty entry range and leave.
     constexpr int H = N / 2;
-    constexpr int n_rb = (H + 1 + RB - 1) / RB;  // tiles covering rows 0 .. N/2
+    constexpr int n_rb = (H + 1 + RB - 1) / RB;  // enough tiles for rows 0 .. N/2
     const unsigned nwg = gridDim.x;
     unsigned bid = blockIdx.x;
     if ((nwg & 7u) == 0) bid = (bid & 7u) * (nwg >> 3) + (bid >> 3);  // XCD-contiguous images
@@ -82,12 +86,12 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     const GatherArgs& g = sd.g;
     const int img = (int)(gi - sd.img_base);  // image within its set
     const int ch = img % g.n_ch;
-    const int r0 = tile_id * RB;
-    const int nr = min(RB, H + 1 - r0);  // rows of this tile that are <= N/2
+    const int s0 = tile_id * RB;  // first list slot of this tile
     const int tid = threadIdx.x;
-    const int* rsg = sd.row_start + (int64_t)ch * (N + 1);
-    const int e0 = rsg[r0], e1 = rsg[r0 + nr];  // uniform
-    if (e0 == e1) return;                       // nothing lands in this tile
+    const int* rl = sd.row_list + (int64_t)ch * row_list_len(N);
+    const int* rst = rl + H + 1;  // first entries per slot, then the end
+    const int e0 = rst[s0], e1 = rst[min(s0 + RB, H + 1)];  // uniform
+    if (e0 == e1) return;                                  // past the last non-empty row
 
     // The first batch of gathers, the twiddle table and the row starts are
     // requested before the LDS set-up, so their latency overlaps the zeroing
@@ -102,7 +106,8 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
 #pragma unroll
     for (int k = 0; k < TWK; k++)
         if (tid + k * NT < N) twv[k] = tw[tid + k * NT];
-    const int rsv = tid <= RB ? rsg[r0 + min(tid, nr)] : 0;
+    const int rsv = tid <= RB ? rst[min(s0 + tid, H + 1)] : 0;
+    const int rrv = tid < RB ? rl[min(s0 + tid, H)] : 0;
     const double nu = g.freq_hz[ch];
     gb.load_records(g, rec);
     float2* twl = tile + RB * TS;  // twiddle table copy [N]
@@ -110,6 +115,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     for (int k = 0; k < TWK; k++)
         if (tid + k * NT < N) twl[tid + k * NT] = twv[k];
     if (tid <= RB) rs[tid] = rsv;
+    if (tid < RB) rrow[tid] = rrv;
     for (int i = tid; i < RB * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
     auto sink = [&](int e, int u_idx, float2 v) {
@@ -118,7 +124,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         float2* c = &tile[r * TS + pad_index(u_idx)];
         atomicAdd(&c->x, v.x);
         atomicAdd(&c->y, v.y);
-        const int row = r0 + r;
+        const int row = rrow[r];
         return (row == 0 || row == H) ? 1 : 2;
     };
     int n_ok = gb.consume(g, nu, sink);
@@ -142,7 +148,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
         team_sync<false>();
         F::template run<1>(v, row, lane, TwTable{twl});
-        float2* dst = inter + ((int64_t)li * (H + 1) + r0 + team) * N;
+        float2* dst = inter + ((int64_t)li * (H + 1) + rrow[team]) * N;
         F::store(v, lane, [&](int i, float2 x) { store_stream(&dst[i], x); });
     }
 }

@@ -705,6 +705,7 @@ namespace {
 
 struct SetScratch {
     int* row_start;
+    int* row_list;
     uint32_t* entries;
     uint32_t* isums;
     double* coef;  // [n_bl] phase coefficients
@@ -785,7 +786,8 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         const int64_t n = st.n_pixels;
         const int n_bl = st.n_ant * (st.n_ant + 1) / 2;
         rs_off[i] = rs_bytes;
-        rs_bytes += align16(sizeof(int) * (size_t)st.n_ch * (n + 1));
+        rs_bytes += align16(sizeof(int) * (size_t)st.n_ch * (n + 1)) +
+                    align16(sizeof(int) * (size_t)st.n_ch * row_list_len((int)n));
         en_off[i] = en_bytes;
         en_bytes += align16(sizeof(uint32_t) * 2 * (size_t)st.n_ch * std::max(n_bl, 1));
         is_off[i] = (size_t)n_img_all;
@@ -844,6 +846,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     bp.sc.resize(n_sets);
     for (int i = 0; i < n_sets; i++) {
         bp.sc[i].row_start = (int*)((char*)rs + rs_off[i]);
+        bp.sc[i].row_list =
+            (int*)((char*)rs + rs_off[i] +
+                   align16(sizeof(int) * (size_t)sets[i].n_ch * (sets[i].n_pixels + 1)));
         bp.sc[i].entries = (uint32_t*)((char*)en + en_off[i]);
         bp.sc[i].isums = (uint32_t*)is + is_off[i];
         bp.sc[i].coef = (double*)co + co_off[i];
@@ -891,6 +896,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.n_img = st.n_ch * st.n_t;
             d.img_base = base;
             d.row_start = bp.sc[i].row_start;
+            d.row_list = bp.sc[i].row_list;
             d.entries = bp.sc[i].entries;
             d.isums = bp.sc[i].isums;
             d.images = (float2*)st.images;

@@ -606,6 +606,24 @@ k_plan_scan(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ units
         }
     }
     if (tid == 0) rs[n] = total;
+    // the non-empty rows 0 .. N/2 and their first entries (row_list_len)
+    const int h = n / 2, h1 = h + 1;
+    int* rl = sd.row_list + (int64_t)pu.ch * row_list_len(n);
+    auto start = [&](int i) { return i < n ? row_tot[i] : total; };
+    const int hc = (h1 + kUnitThreads - 1) / kUnitThreads;
+    const int q0 = min(tid * hc, h1), q1 = min(q0 + hc, h1);
+    int cnt = 0;
+    for (int i = q0; i < q1; i++) cnt += start(i + 1) != start(i);
+    __syncthreads();  // warp_tot is reused
+    int n_rows;
+    int at = unit_scan(cnt, warp_tot, &n_rows);
+    for (int i = q0; i < q1; i++)
+        if (start(i + 1) != start(i)) {
+            rl[at] = i;
+            rl[h1 + at] = start(i);
+            at++;
+        }
+    for (int p = n_rows + tid; p <= h1; p += kUnitThreads) rl[h1 + p] = start(h1);
 }
 
 __global__ void __launch_bounds__(kUnitThreads)

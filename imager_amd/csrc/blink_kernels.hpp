@@ -37,6 +37,14 @@ struct GatherArgs {
     int n_bl, n_ch;
 };
 
+// Row list of a plan, per channel: the non-empty rows 0 .. N/2 in ascending
+// order (N/2 + 1 slots; the first k used when k rows are non-empty), then their
+// first entries (N/2 + 2 slots: slot p < k holds row_start of the p-th
+// non-empty row, every later slot the end of the entries of rows 0 .. N/2).
+// Empty rows have empty entry ranges, so a tile of consecutive non-empty rows
+// p .. p + R - 1 owns the contiguous entries [start[p], start[p + R]).
+__host__ __device__ constexpr int row_list_len(int n) { return n + 3; }
+
 // One image set of a batched call as the kernels see it (a table of these
 // lives in device memory for the duration of the call or captured graph).
 // Images of all sets that share a grid side and path form a group; a launch
@@ -55,6 +63,7 @@ struct SetDesc {
     int n_img;                // n_ch * n_t
     int64_t img_base;         // first group image index of the set
     int* row_start;           // plan: [n_ch][N + 1]
+    int* row_list;            // plan: [n_ch][row_list_len(N)], see row_list_len
     uint32_t* entries;        // plan: [n_ch][2 n_bl]
     uint32_t* isums;          // [n_img] gridded visibilities per image
     float2* images;           // [n_img][N][N]
