@@ -27,6 +27,9 @@ __device__ __forceinline__ float2 cmuli(float2 a) { return make_float2(-a.y, a.x
 // as a Euclidean modulo: identical wherever the reference stays in bounds
 // (i >= -side), and aliased instead of out of bounds below that.
 __device__ __forceinline__ int wrap_index(int i, int side) {
+    // one conditional add or subtract for -side <= i < 2 side (every cell of
+    // an oversampled grid), the integer division otherwise
+    if (i >= -side && i < 2 * side) return i < 0 ? i + side : (i >= side ? i - side : i);
     int r = i % side;
     return r < 0 ? r + side : r;
 }
