@@ -181,6 +181,7 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     const int img = (int)(gi - sd.img_base);  // image within its set
     const int tid = threadIdx.x;
     const uint32_t count = sd.isums[img];
+    if (cb == 0 && tid == 0 && sd.count_sums) sd.count_sums[img] = (double)count;
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (N + 1);
     float2* twl = tile + TEAMS * TS;  // twiddle table copy [N]
     for (int i = tid; i < N; i += NT) twl[i] = tw[i];
@@ -360,6 +361,7 @@ k_cols_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     const int img = (int)(gi - sd.img_base);
     const int tid = threadIdx.x;
     const uint32_t count = sd.isums[img];
+    if (cb == 0 && tid == 0 && sd.count_sums) sd.count_sums[img] = (double)count;
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (N + 1);
     for (int i = tid; i <= H; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
     if (tid < 128) twl[tid] = tws[tid];

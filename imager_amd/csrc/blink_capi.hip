@@ -729,8 +729,6 @@ struct BatchPlan {
     std::vector<unsigned char> blob;
     size_t units_off = 0;
     int n_units = 0, parts = 1, n_max = 0;
-    uint32_t* isums_all = nullptr;
-    int64_t isums_count = 0;
     int* partial = nullptr;
     uint32_t* cells = nullptr;
     float* inter = nullptr;
@@ -823,7 +821,6 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     bp.parts = (int)parts;
     bp.n_units = (int)n_units;
     bp.n_max = n_max;
-    bp.isums_count = n_img_all;
     const size_t partial_bytes = sizeof(int) * (size_t)n_units * parts * n_max;
 
     void *inter, *rs, *en, *is, *pa, *cl, *co;
@@ -841,7 +838,6 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
                       &co)))
         return rc;
     bp.inter = (float*)inter;
-    bp.isums_all = (uint32_t*)is;
     bp.partial = (int*)pa;
     bp.sc.resize(n_sets);
     for (int i = 0; i < n_sets; i++) {
@@ -900,6 +896,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.entries = bp.sc[i].entries;
             d.isums = bp.sc[i].isums;
             d.images = (float2*)st.images;
+            d.count_sums = st.count_sums;
             base += d.n_img;
             desc_of[i] = (int)desc.size();
             desc.push_back(d);
@@ -1048,7 +1045,6 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
 // Zeroes the count sums and builds every plan of the call on s.
 static int enqueue_plans(blink_ctx* ctx, const BatchPlan& bp, const BatchTables& t,
                          hipStream_t s) {
-    LAUNCH("zero", s, launch_zero(bp.isums_all, sizeof(uint32_t) * bp.isums_count, s));
     LAUNCH("plan", s,
            launch_plan_units(t.sets, t.units, bp.n_units, bp.parts, bp.n_max, bp.partial,
                              bp.cells, s));
@@ -1090,7 +1086,8 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const Batc
                                                  sc.isums, s));
                     LAUNCH("dft_rows", s, launch_dft_rows(half, pl1, tbuf, n, cnt, prec, s));
                     LAUNCH("dft_cols", s,
-                           launch_dft_cols(tbuf, pl2, st.images, n, cnt, img0, sc.isums, prec, s));
+                           launch_dft_cols(tbuf, pl2, st.images, n, cnt, img0, sc.isums,
+                                           st.count_sums, prec, s));
                 }
             }
             continue;
@@ -1120,11 +1117,6 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const Batc
                    launch_cols_half(first, sd, is, ns, plan, gi0, cnt, bp.inter, tw, s));
         }
     }
-    for (int i = 0; i < (int)bp.sc.size(); i++)
-        if (sets[i].count_sums)
-            LAUNCH("isums_to_double", s,
-                   launch_isums_to_double(bp.sc[i].isums, sets[i].count_sums,
-                                          sets[i].n_ch * sets[i].n_t, s));
     return BLINK_OK;
 }
 

@@ -83,6 +83,7 @@ struct DftGemmArgs {
     int64_t img_stride, n_stride;  // C[n][m] at out[img * img_stride + n * n_stride + r]
     float2* out;
     const uint32_t* isums;    // stage 2: per-image count sums (set-relative), else null
+    double* count_sums;       // stage 2: optional, isums as doubles
     int64_t img0;             // set-relative index of the first image
 };
 
@@ -235,7 +236,12 @@ __global__ void __launch_bounds__(kGemmThreads) k_dft_gemm(DftGemmArgs g) {
         const int64_t img = mrow / g.R;
         const int r = (int)(mrow % g.R);
         float scale = unsc;
-        if (g.isums) scale *= (float)(1.0 / (double)g.isums[g.img0 + img]);
+        if (g.isums) {
+            const uint32_t count = g.isums[g.img0 + img];
+            scale *= (float)(1.0 / (double)count);
+            if (g.count_sums && r == 0 && n_base == 0 && lane < 32)
+                g.count_sums[g.img0 + img] = (double)count;
+        }
         float2* ob = g.out + img * g.img_stride + r;
 #pragma unroll
         for (int s = 0; s < NSUB; s++) {
@@ -343,8 +349,8 @@ hipError_t launch_dft_rows(const float* half, const uint16_t* planes1, float* in
 }
 
 hipError_t launch_dft_cols(const float* inter, const uint16_t* planes2, float* images, int n,
-                           int64_t n_img, int64_t img0, const uint32_t* isums, int prec,
-                           hipStream_t stream) {
+                           int64_t n_img, int64_t img0, const uint32_t* isums,
+                           double* count_sums, int prec, hipStream_t stream) {
     const int h = n / 2;
     DftGemmArgs g{};
     g.a = (const float2*)inter;
@@ -360,6 +366,7 @@ hipError_t launch_dft_cols(const float* inter, const uint16_t* planes2, float* i
     g.img_stride = (int64_t)n * n;
     g.out = (float2*)images + img0 * (int64_t)n * n;
     g.isums = isums;
+    g.count_sums = count_sums;
     g.img0 = img0;
     return launch_gemm<false>(g, prec, stream);
 }

@@ -515,6 +515,9 @@ k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ unit
     int b0, b1;
     part_range(n_bl, parts, part, b0, b1);
     uint32_t* cl = cells + pu.cell_off;
+    if (part == 0)  // the count sums of this channel's images start at zero
+        for (int t = threadIdx.x; t < sd.n_img / sd.g.n_ch; t += kUnitThreads)
+            sd.isums[(int64_t)t * sd.g.n_ch + pu.ch] = 0u;
     if (pu.ch == 0 && sd.g.corrections)  // one unit per set writes the phase coefficients
         for (int bl = b0 + (int)threadIdx.x; bl < b1; bl += kUnitThreads)
             sd.coef_out[bl] = phase_coefficient(sd.g.corrections, sd.w_m, sd.cable_m, (unsigned)bl);
@@ -1199,6 +1202,7 @@ k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
     const int img = (int)(gi - sd.img_base);
     const uint32_t count = sd.isums[img];
+    if (cbk == 0 && threadIdx.x == 0 && sd.count_sums) sd.count_sums[img] = (double)count;
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (n + 1);
     for (int i = threadIdx.x; i <= h; i += blockDim.x) nonempty[i] = rsg[i + 1] != rsg[i];
     __syncthreads();
@@ -1390,15 +1394,20 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
 }
 
 // Output columns per workgroup of k_cols_half: the largest even divisor of
-// n, at most 64, whose LDS rows fit next to the team scratch.
-static int half_cols_block(int n, int teams) {
+// n, at most 64, whose LDS rows fit next to the team scratch and that still
+// gives at least 256 workgroups over the n_img images (one per CU), else the
+// smallest, 2 (a single 180^2 image then runs 90 workgroups, not 3).
+static int half_cols_block(int n, int teams, int64_t n_img) {
+    int fallback = 0;
     for (int cb = std::min(64, n); cb >= 2; cb--) {
         if (cb % 2 || n % cb) continue;
         const int64_t lds = sizeof(float2) * ((int64_t)(cb / 2) * (n + 1) + (int64_t)teams * n) +
                             (n / 2 + 1);
-        if (lds <= g_lds_budget) return cb;
+        if (lds > g_lds_budget) continue;
+        if ((int64_t)(n / cb) * n_img >= 256) return cb;
+        fallback = cb;
     }
-    return 0;
+    return fallback;
 }
 
 hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int* img_set,
@@ -1410,7 +1419,7 @@ hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int
     const int team = fft_team_for(n);
     const bool block_team = team == kFftThreads;
     const int teams = block_team ? 1 : kFftThreads / team;
-    const int cb = half_cols_block(n, teams);
+    const int cb = half_cols_block(n, teams, n_img);
     if (!cb) return hipErrorInvalidConfiguration;
     const size_t lds = sizeof(float2) * ((size_t)(cb / 2) * (n + 1) + (size_t)teams * n) +
                        (size_t)(n / 2 + 1);

@@ -65,8 +65,11 @@ struct SetDesc {
     int* row_start;           // plan: [n_ch][N + 1]
     int* row_list;            // plan: [n_ch][row_list_len(N)], see row_list_len
     uint32_t* entries;        // plan: [n_ch][2 n_bl]
-    uint32_t* isums;          // [n_img] gridded visibilities per image
+    uint32_t* isums;          // [n_img] gridded visibilities per image; the plan's
+                              // part 0 of each channel zeroes that channel's entries
     float2* images;           // [n_img][N][N]
+    double* count_sums;       // optional [n_img]: isums as doubles, written by the
+                              // column pass (first column block of each image)
 };
 
 // The set of group image gi, by value.  Kernels are built twice: for a
@@ -184,8 +187,8 @@ hipError_t launch_grid_half_rows(const GatherArgs& g, const int* row_start, cons
 hipError_t launch_dft_rows(const float* half, const uint16_t* planes1, float* inter, int n,
                            int64_t n_img, int prec, hipStream_t stream);
 hipError_t launch_dft_cols(const float* inter, const uint16_t* planes2, float* images, int n,
-                           int64_t n_img, int64_t img0, const uint32_t* isums, int prec,
-                           hipStream_t stream);
+                           int64_t n_img, int64_t img0, const uint32_t* isums,
+                           double* count_sums, int prec, hipStream_t stream);
 // Allows the DFT kernels' LDS-resident matrix slices (up to 160 KiB).
 hipError_t set_dft_kernel_attributes();
 // Split planes of D1 (stage 1) or D2 (stage 2) for side n and precision prec.
