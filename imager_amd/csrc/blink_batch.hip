@@ -58,12 +58,21 @@ constexpr size_t batch_lds_bytes() {
     return sizeof(float2) * ((size_t)RB * tile_stride(N, RB) + N);
 }
 
+// Threads of a row-tile workgroup: the RB teams of L lanes that run the row
+// FFTs, and at least 256 for the gather of plan entries, which is latency
+// bound: small sides with many baselines (128^2: 8 lanes per row) would
+// otherwise leave each workgroup a few dozen threads for thousands of entries.
+template <int N, int RB>
+constexpr int rows_reg_threads() {
+    return RB * BatchLanes<N>::L > 256 ? RB * BatchLanes<N>::L : 256;
+}
+
 template <int N, int RB_, bool kMulti>
-__global__ void __launch_bounds__(RB_ * BatchLanes<N>::L)
+__global__ void __launch_bounds__((rows_reg_threads<N, RB_>()))
 k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
                 int64_t gi0, float2* __restrict__ inter, const float2* __restrict__ tw) {
     constexpr int L = BatchLanes<N>::L, RB = RB_, TS = tile_stride(N, RB_);
-    constexpr int NT = RB * L;
+    constexpr int NT = rows_reg_threads<N, RB_>();
     using F = RegFft<N, L>;
     extern __shared__ float4 rows_reg_lds4[];
     float2* tile = (float2*)rows_reg_lds4;  // [RB][TS]
@@ -143,7 +152,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     // rows without entries are not written
     const int team = tid / L, lane = tid % L;
     float2* row = tile + team * TS;
-    if (rs[team + 1] != rs[team]) {
+    if (team < RB && rs[team + 1] != rs[team]) {
         float2 v[F::V];
         F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
         team_sync<false>();
@@ -440,7 +449,7 @@ template <int N, int RB>
 static hipError_t launch_rows_reg_t(const SetDesc& first, const SetDesc* sets,
                                     const int* img_set, int n_sets, int64_t gi0, int n_img,
                                     float* inter, const float* tw, hipStream_t stream) {
-    constexpr int NT = RB * BatchLanes<N>::L;
+    constexpr int NT = rows_reg_threads<N, RB>();
     constexpr size_t lds = batch_lds_bytes<N, RB>();
     if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
     const int64_t blocks = (int64_t)((N / 2 + 1 + RB - 1) / RB) * n_img;
