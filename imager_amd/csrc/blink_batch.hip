@@ -3,11 +3,11 @@
 //
 //   k_grid_rows_reg : (row tile, image) -> plan entries gathered (pol, phase
 //                     correction, NaN/MAX_VIS) and accumulated in LDS, each
-//                     non-empty row transformed and written row-major to the
-//                     row-pass buffer inter[img][v][x] (whole lines);
+//                     non-empty row transformed and written to the
+//                     column-blocked row-pass buffer (whole lines);
 //   k_cols_reg      : (column block, image) -> the block's column pairs staged
-//                     through LDS from row segments of inter (rows with no
-//                     plan entries read as zero), transformed, scaled by
+//                     through LDS from one contiguous run of the buffer (rows
+//                     with no plan entries read as zero), transformed, scaled by
 //                     (float)(1/sum counts) and stored fft-shifted into
 //                     images[img][y][x].
 //
@@ -32,6 +32,19 @@ __host__ __device__ constexpr int tile_stride(int n, int rows) {
     const int want = 32 / rows;
     const int p = padded_len(n);
     return p + ((want - p % 32) + 32) % 32;
+}
+
+// Row-pass buffer layout of the register kernels (N = 128 .. 2048): rows
+// v = 0 .. N/2 of each image, cut into blocks of kInterBlock columns, each
+// block stored row after row: element (v, x) of image li at
+//   li (N/2 + 1) N + (x / W) (N/2 + 1) W + v W + x % W,   W = kInterBlock.
+// A transformed row is written as N / W segments of W * 8 = 256 bytes (whole
+// lines), and a column block of up to W columns reads one contiguous run.
+constexpr int kInterBlock = 32;
+__device__ __forceinline__ int64_t inter_index(int n, int64_t li, int v, int x) {
+    const int64_t h1 = n / 2 + 1;
+    return li * h1 * n + (int64_t)(x / kInterBlock) * h1 * kInterBlock + (int64_t)v * kInterBlock +
+           x % kInterBlock;
 }
 
 // Lanes per row: 16 values per lane up to N = 1024, 32 at N = 2048.
@@ -157,8 +170,8 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
         team_sync<false>();
         F::template run<1>(v, row, lane, TwTable{twl});
-        float2* dst = inter + ((int64_t)li * (H + 1) + rrow[team]) * N;
-        F::store(v, lane, [&](int i, float2 x) { store_stream(&dst[i], x); });
+        const int vrow = rrow[team];
+        F::store(v, lane, [&](int i, float2 x) { store_stream(&inter[inter_index(N, li, vrow, i)], x); });
     }
 }
 
@@ -197,12 +210,12 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     for (int i = tid; i <= H; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
     __syncthreads();
 
-    // stage z_t[v] and z_t[N - v] from inter[li][v][c0 + 2t .. c0 + 2t + 1]:
-    // consecutive threads read consecutive 16-byte column pairs of one row
-    // (CB * 8 contiguous bytes per row); rows without plan entries are zero
-    // (all of a thread's loads are issued before its LDS writes)
+    // stage z_t[v] and z_t[N - v] from columns c0 + 2t, c0 + 2t + 1 of row v:
+    // consecutive threads read consecutive 16-byte column pairs, and with the
+    // column-blocked buffer consecutive rows of a block follow each other, so
+    // the block's rows are one contiguous run; rows without plan entries are
+    // zero (all of a thread's loads are issued before its LDS writes)
     const int c0 = cb * CB;
-    const float2* src = inter + (int64_t)li * (H + 1) * N + c0;
     constexpr int STAGE = (H + 1) * TEAMS, ITER = (STAGE + NT - 1) / NT;
     float4 p[ITER];
 #pragma unroll
@@ -210,7 +223,8 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
         const int i = tid + k * NT;
         const int vr = i / TEAMS, t = i % TEAMS;
         p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < STAGE && nonempty[vr]) p[k] = *(const float4*)&src[(int64_t)vr * N + 2 * t];
+        if (i < STAGE && nonempty[vr])
+            p[k] = *(const float4*)&inter[inter_index(N, li, vr, c0 + 2 * t)];
     }
 #pragma unroll
     for (int k = 0; k < ITER; k++) {
