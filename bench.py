@@ -293,6 +293,8 @@ def main():
     ap.add_argument("--batch-tiles", action="append", default=[], metavar="N:ROWS:COLS",
                     help="pin the register kernels' row tile and column block for side N "
                          "(blink_set_batch_tiles) instead of autotuning; repeatable")
+    ap.add_argument("--input-copies", type=int, default=1,
+                    help="device copies of the visibilities used in turn by successive steps")
     ap.add_argument("--plain-images", type=int, default=64,
                     help="images run through the plain one-image-at-a-time GPU path")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -341,9 +343,25 @@ def main():
     stream = torch.cuda.current_stream()
     log(job["describe"])
 
+    # --input-copies K: K device copies of the visibilities, used in turn by
+    # successive steps, so that a step cannot find its inputs in the 256 MiB
+    # Infinity Cache from the step before (the 1024^2 workloads hold 169-203 MB)
+    rotation = [job["calls"]]
+    if args.input_copies > 1 and job.get("dev") is not None and len(job["calls"]) == 1:
+        (st0,), = job["calls"]
+        o = job["obs"]
+        for _ in range(args.input_copies - 1):
+            vis_k = job["dev"]["vis"].clone()
+            job.setdefault("extra_vis", []).append(vis_k)
+            st_k = capi.ImageSet.from_buffer_copy(st0)
+            st_k.vis = vis_k.data_ptr()
+            rotation.append([[st_k]])
+    step_no = [0]
+
     def step():
-        for call in job["calls"]:
+        for call in rotation[step_no[0] % len(rotation)]:
             ctx.image_batch(call, stream)
+        step_no[0] += 1
 
     for _ in range(args.warmup):
         step()
@@ -522,6 +540,7 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     config = dict(job["config"])
+    config["input_copies"] = len(rotation)
     config.update({"workload": args.workload, "images_per_step": job["images_per_step"],
                    "pol": "XX", "corrections": "geometric+cable",
                    "path": "blink_image_batch (hipGraph)",
