@@ -443,6 +443,52 @@ def test_image_batch_every_tile_variant(ctx, n):
     assert len(built["row"]) >= 2 and len(built["col"]) >= 3, built
 
 
+@pytest.mark.parametrize("n", [256, 1024, 180])
+def test_image_batch_sparse_rows_nan_and_maxvis(ctx, n):
+    """A 4-antenna array leaves almost every grid row empty (the register row
+    tiles then hold only the few non-empty rows); NaN and |V| >= 1e12
+    visibilities planted in the batched path are skipped as the reference
+    does (src/pacer_imager.cpp:327-329), in the images and the count sums."""
+    o = synth.small(n_ant=4, n_ch=3, n_t=2, n_pixels=n, seed=40 + n, height_sigma_m=1.0)
+    good = np.flatnonzero(o.flags == 0)
+    o.vis[0, 1, good[0], 0] = np.nan
+    o.vis[1, 2, good[1], 0] = np.complex64(3e12)
+    o.vis[1, 0, good[2], 0] = np.complex64(complex(0.5, -2e12))
+    _say("sparse batch", n)
+    ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, n)
+    d, imgs, sums, st = _batch(ctx, o)
+    ctx.image_batch([st])
+    got = _host(imgs)
+    assert oracle.peak_normalised_error(got, ref)[0] < IMAGE_TOL
+    _, c_ref, _ = oracle.gridding(oracle.cable_corrections(
+        oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq), o.n_ant, o.cable, o.freq),
+        o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v, o.min_uv, n, "XX")
+    expect = c_ref.reshape(o.n_images, -1).sum(1).astype(np.float64)
+    assert np.array_equal(_host(sums), expect)
+    assert expect[1] == expect[0] - 2 and expect[5] == expect[0] - 2  # one visibility + conjugate
+
+
+def test_image_batch_fully_flagged_image_set(ctx):
+    """Every antenna flagged: no visibility is gridded, the count sums are 0 and
+    the images are 0 * (float)(1/0) = NaN, the reference's arithmetic
+    (src/pacer_imager.cpp:171-178) on an empty grid; a second set in the same
+    call is unaffected."""
+    import torch
+    from imager_amd import capi
+    o = synth.small(n_ant=16, n_ch=2, n_t=1, n_pixels=256, seed=61,
+                    flagged_antennas=tuple(range(16)))
+    o2 = synth.small(n_ant=16, n_ch=2, n_t=1, n_pixels=256, seed=62)
+    d, imgs, sums, st = _batch(ctx, o)
+    d2, imgs2, sums2, st2 = _batch(ctx, o2)
+    ctx.image_batch([st, st2])
+    assert np.all(_host(sums) == 0)
+    assert np.all(np.isnan(_host(imgs).real))
+    ref2, _ = oracle.run_f32(o2.vis, o2.n_ant, o2.u, o2.v, o2.w, o2.flags, o2.freq, o2.cable,
+                             True, True, o2.delta_u, o2.delta_v, o2.min_uv, 256)
+    assert oracle.peak_normalised_error(_host(imgs2), ref2)[0] < IMAGE_TOL
+
+
 def test_image_batch_graph_replays_are_identical(ctx):
     import torch
     o = synth.small(n_ant=64, n_ch=4, n_t=2, n_pixels=256, seed=77)
