@@ -49,6 +49,10 @@ struct blink_ctx {
     std::map<std::string, GraphEntry> graphs;     // descriptor bytes -> graph
     uint64_t graph_clock = 0;
     hipStream_t capture_stream = nullptr;
+    // a call with several grid sides runs the largest side's group on the
+    // call's stream and the others beside it on aux_stream
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t fork_event = nullptr, join_event = nullptr;
     bool use_graphs = true;
     bool autotune = true;
     // tile choice of the batched kernels per grid side: {row tile, column block}
@@ -325,7 +329,13 @@ BLINK_API int blink_create(int device, blink_ctx** out) {
         set_dft_kernel_attributes() != hipSuccess) {
         (void)hipGetLastError();
     }
-    if (hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->fork_event, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming) != hipSuccess) {
+        if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+        if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+        if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
         delete ctx;
         return BLINK_ERR_HIP;
     }
@@ -358,6 +368,9 @@ BLINK_API void blink_destroy(blink_ctx* ctx) {
     prof_collect(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+    if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
+    if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
     delete ctx;
 }
 
@@ -732,6 +745,10 @@ struct BatchPlan {
     int* partial = nullptr;
     uint32_t* cells = nullptr;
     float* inter = nullptr;
+    // several groups: the group of index `main` (most image bytes) uses inter
+    // on the call's stream, the others inter_aux on the aux stream
+    float* inter_aux = nullptr;
+    int main = 0;
     // host copy of a group's first set descriptor (a kernel argument)
     const SetDesc& host_desc(const BatchGroup& g) const {
         return ((const SetDesc*)blob.data())[g.first_desc];
@@ -775,6 +792,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     }
     // scratch sizes
     size_t rs_bytes = 0, en_bytes = 0, inter_bytes = 16;
+    std::vector<size_t> group_inter(bp.groups.size(), 16);
     std::vector<size_t> rs_off(n_sets), en_off(n_sets), is_off(n_sets);
     int64_t n_img_all = 0, n_units = 0, min_bl = INT64_MAX, max_bl = 0, n_cells = 0, n_bl_all = 0;
     std::vector<size_t> co_off(n_sets);
@@ -805,13 +823,27 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
                 const int n_img = sets[i].n_ch * sets[i].n_t;
                 const int64_t per = dft_inter_bytes(n);
                 const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
-                inter_bytes = std::max(inter_bytes, (size_t)(per * chunk));
+                group_inter[&g - bp.groups.data()] =
+                    std::max(group_inter[&g - bp.groups.data()], (size_t)(per * chunk));
             }
             continue;
         }
         const int64_t per = g.reg ? batch_inter_bytes(g.n) : 8 * (n / 2 + 1) * n;
         g.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(g.n_img, inter_budget / per));
-        inter_bytes = std::max(inter_bytes, (size_t)(per * g.chunk));
+        group_inter[&g - bp.groups.data()] = (size_t)(per * g.chunk);
+    }
+    // the main group (most image bytes) keeps its own row-pass buffer; the
+    // others share a second one on the aux stream
+    {
+        int64_t best = -1;
+        for (size_t k = 0; k < bp.groups.size(); k++) {
+            const int64_t bytes = bp.groups[k].n_img * (int64_t)bp.groups[k].n * bp.groups[k].n;
+            if (bytes > best) best = bytes, bp.main = (int)k;
+        }
+        size_t aux = 0;
+        for (size_t k = 0; k < bp.groups.size(); k++)
+            if ((int)k != bp.main) aux = std::max(aux, group_inter[k]);
+        inter_bytes = align16(group_inter[bp.main]) + aux;
     }
     // plan parts: about 2048 workgroups over the units with at least 256
     // baselines per part, and never more than kPlanPartMax baselines per part
@@ -838,6 +870,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
                       &co)))
         return rc;
     bp.inter = (float*)inter;
+    bp.inter_aux = (float*)((char*)inter + align16(group_inter[bp.main]));
     bp.partial = (int*)pa;
     bp.sc.resize(n_sets);
     for (int i = 0; i < n_sets; i++) {
@@ -1051,72 +1084,91 @@ static int enqueue_plans(blink_ctx* ctx, const BatchPlan& bp, const BatchTables&
     return BLINK_OK;
 }
 
+// Enqueues one group's launches on s, with row-pass buffer `inter`.
+static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const BatchPlan& bp,
+                         const BatchTables& t, const BatchGroup& g, float* inter,
+                         hipStream_t s) {
+    const int n = g.n;
+    const float* tw = ctx->twiddles[n];
+    if (is_dft(g.transform)) {
+        // per set and chunk: half plane -> stage 1 (complex) -> stage 2 (real, scaled)
+        const int prec = dft_prec(g.transform);
+        const uint16_t* pl1 = ctx->dft_planes[n * 8 + 2 + prec];
+        const uint16_t* pl2 = ctx->dft_planes[n * 8 + 4 + prec];
+        for (int i : g.sets) {
+            const blink_image_set& st = sets[i];
+            const SetScratch& sc = bp.sc[i];
+            const int n_img = st.n_ch * st.n_t;
+            GatherArgs ga{};
+            ga.vis = (const float4*)st.vis;
+            ga.phase_coef = sc.coef;
+            ga.freq_hz = st.freq_hz;
+            ga.corrections = st.corrections & 3u;
+            ga.pol = st.pol;
+            ga.n_bl = st.n_ant * (st.n_ant + 1) / 2;
+            ga.n_ch = st.n_ch;
+            float* half = inter;
+            float* tbuf = inter + 2 * (int64_t)(n / 2 + 1) * n * sc.chunk;
+            for (int img0 = 0; img0 < n_img; img0 += sc.chunk) {
+                const int cnt = std::min(sc.chunk, n_img - img0);
+                LAUNCH("grid_half_rows", s,
+                       launch_grid_half_rows(ga, sc.row_start, sc.entries, n, cnt, img0, half,
+                                             sc.isums, s));
+                LAUNCH("dft_rows", s, launch_dft_rows(half, pl1, tbuf, n, cnt, prec, s));
+                LAUNCH("dft_cols", s,
+                       launch_dft_cols(tbuf, pl2, st.images, n, cnt, img0, sc.isums,
+                                       st.count_sums, prec, s));
+            }
+        }
+        return BLINK_OK;
+    }
+    const SetDesc* sd = t.group_sets(g);
+    const SetDesc& first = bp.host_desc(g);
+    const int* is = t.img_set(g);
+    const int ns = (int)g.sets.size();
+    FftPlan plan;
+    make_fft_plan(n, plan);
+    for (int64_t gi0 = 0; gi0 < g.n_img; gi0 += g.chunk) {
+        const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img - gi0);
+        if (g.reg) {
+            auto it = ctx->tiles.find(n);
+            const std::pair<int, int> tl = it != ctx->tiles.end() ? it->second : default_tiles(n);
+            LAUNCH("grid_rows_fft", s,
+                   launch_grid_rows_reg(first, sd, is, ns, n, tl.first, gi0, cnt, inter,
+                                        tw, s));
+            LAUNCH("fft_cols", s,
+                   launch_cols_reg(first, sd, is, ns, n, tl.second, gi0, cnt, inter, tw,
+                                   s));
+            continue;
+        }
+        LAUNCH("grid_rows_fft", s,
+               launch_grid_rows_fft(first, sd, is, ns, plan, gi0, cnt, inter, tw, s));
+        LAUNCH("fft_cols", s,
+               launch_cols_half(first, sd, is, ns, plan, gi0, cnt, inter, tw, s));
+    }
+    return BLINK_OK;
+}
+
 // Enqueues the whole call on s.  All scratch was reserved by build_batch, so
-// this can be captured.
+// this can be captured.  With several grid sides, the main group (most image
+// bytes) runs on s and the others beside it, one after another, on the aux
+// stream with their own row-pass buffer: the small sides' latency-bound
+// launches overlap the large side's bandwidth-bound ones.
 static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const BatchPlan& bp,
                          const BatchTables& t, hipStream_t s) {
     int rc = enqueue_plans(ctx, bp, t, s);
     if (rc) return rc;
-    for (const BatchGroup& g : bp.groups) {
-        const int n = g.n;
-        const float* tw = ctx->twiddles[n];
-        if (is_dft(g.transform)) {
-            // per set and chunk: half plane -> stage 1 (complex) -> stage 2 (real, scaled)
-            const int prec = dft_prec(g.transform);
-            const uint16_t* pl1 = ctx->dft_planes[n * 8 + 2 + prec];
-            const uint16_t* pl2 = ctx->dft_planes[n * 8 + 4 + prec];
-            for (int i : g.sets) {
-                const blink_image_set& st = sets[i];
-                const SetScratch& sc = bp.sc[i];
-                const int n_img = st.n_ch * st.n_t;
-                GatherArgs ga{};
-                ga.vis = (const float4*)st.vis;
-                ga.phase_coef = sc.coef;
-                ga.freq_hz = st.freq_hz;
-                ga.corrections = st.corrections & 3u;
-                ga.pol = st.pol;
-                ga.n_bl = st.n_ant * (st.n_ant + 1) / 2;
-                ga.n_ch = st.n_ch;
-                float* half = bp.inter;
-                float* tbuf = bp.inter + 2 * (int64_t)(n / 2 + 1) * n * sc.chunk;
-                for (int img0 = 0; img0 < n_img; img0 += sc.chunk) {
-                    const int cnt = std::min(sc.chunk, n_img - img0);
-                    LAUNCH("grid_half_rows", s,
-                           launch_grid_half_rows(ga, sc.row_start, sc.entries, n, cnt, img0, half,
-                                                 sc.isums, s));
-                    LAUNCH("dft_rows", s, launch_dft_rows(half, pl1, tbuf, n, cnt, prec, s));
-                    LAUNCH("dft_cols", s,
-                           launch_dft_cols(tbuf, pl2, st.images, n, cnt, img0, sc.isums,
-                                           st.count_sums, prec, s));
-                }
-            }
-            continue;
-        }
-        const SetDesc* sd = t.group_sets(g);
-        const SetDesc& first = bp.host_desc(g);
-        const int* is = t.img_set(g);
-        const int ns = (int)g.sets.size();
-        FftPlan plan;
-        make_fft_plan(n, plan);
-        for (int64_t gi0 = 0; gi0 < g.n_img; gi0 += g.chunk) {
-            const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img - gi0);
-            if (g.reg) {
-                auto it = ctx->tiles.find(n);
-                const std::pair<int, int> tl = it != ctx->tiles.end() ? it->second : default_tiles(n);
-                LAUNCH("grid_rows_fft", s,
-                       launch_grid_rows_reg(first, sd, is, ns, n, tl.first, gi0, cnt, bp.inter,
-                                            tw, s));
-                LAUNCH("fft_cols", s,
-                       launch_cols_reg(first, sd, is, ns, n, tl.second, gi0, cnt, bp.inter, tw,
-                                       s));
-                continue;
-            }
-            LAUNCH("grid_rows_fft", s,
-                   launch_grid_rows_fft(first, sd, is, ns, plan, gi0, cnt, bp.inter, tw, s));
-            LAUNCH("fft_cols", s,
-                   launch_cols_half(first, sd, is, ns, plan, gi0, cnt, bp.inter, tw, s));
-        }
-    }
+    if (bp.groups.size() == 1) return enqueue_group(ctx, sets, bp, t, bp.groups[0], bp.inter, s);
+    hipStream_t a = ctx->aux_stream;
+    HIP_TRY(hipEventRecord(ctx->fork_event, s));
+    HIP_TRY(hipStreamWaitEvent(a, ctx->fork_event, 0));
+    for (size_t k = 0; k < bp.groups.size(); k++)
+        if ((int)k != bp.main &&
+            (rc = enqueue_group(ctx, sets, bp, t, bp.groups[k], bp.inter_aux, a)))
+            return rc;
+    HIP_TRY(hipEventRecord(ctx->join_event, a));
+    if ((rc = enqueue_group(ctx, sets, bp, t, bp.groups[bp.main], bp.inter, s))) return rc;
+    HIP_TRY(hipStreamWaitEvent(s, ctx->join_event, 0));
     return BLINK_OK;
 }
 
