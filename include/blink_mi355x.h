@@ -23,7 +23,11 @@
  *     the message.  BLINK_ERR_INVALID_ARGUMENT corresponds to the reference's
  *     std::invalid_argument, BLINK_ERR_RUNTIME to std::runtime_error.
  *   - A context owns its scratch memory, FFT twiddle tables and captured
- *     graphs.  Contexts are not thread-safe: use one per host thread.
+ *     graphs.  Contexts are not thread-safe: use one per host thread.  Calls
+ *     on one context share its scratch, so they must not run concurrently on
+ *     different streams (order them, or use one context per stream); a
+ *     blink_image_batch call with several grid sides also uses the context's
+ *     own second stream, forked from and joined back to `stream`.
  */
 #ifndef BLINK_MI355X_H
 #define BLINK_MI355X_H
