@@ -309,8 +309,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py measures one GPU: every image is independent and this build runs on "
-              f"one MI355X; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported.",
+        print(f"bench.py measures one GPU: every image is independent, so images shard over "
+              f"GPUs with no exchange, and only the one-GPU path is implemented; --gpus "
+              f"{args.gpus} (WORLD_SIZE={world}) is not supported.",
               file=sys.stderr, flush=True)
         sys.exit(2)
 

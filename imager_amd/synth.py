@@ -9,7 +9,7 @@ visibilities made of point sources plus complex Gaussian noise:
 with zenith u, v, w = -(x_a1 - x_a2) metres stored as FP32 (the convention
 of CAntennaPositions::CalculateUVW, src/antenna_positions.cpp:81-89).
 Randomness comes from numpy's PCG64 generator with a fixed seed per workload,
-so every run (CPU or GPU box) sees identical inputs.
+so every run, on a CPU or a GPU host, sees identical inputs.
 """
 from __future__ import annotations
 
