@@ -1209,13 +1209,26 @@ k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
 
     const int c0 = cbk * cb;
     const float2* src = inter + (int64_t)li * (h + 1) * n + c0;
-    for (int i = threadIdx.x; i < (h + 1) * pairs; i += blockDim.x) {
-        const int v = i / pairs, t = i % pairs;
-        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (nonempty[v]) p = *(const float4*)&src[(int64_t)v * n + 2 * t];
-        float2* zr = rows + (size_t)t * ld;
-        zr[v] = make_float2(p.x - p.w, p.y + p.z);              // c_x[v] + i c_{x+1}[v]
-        if (v > 0 && v < h) zr[n - v] = make_float2(p.x + p.w, p.z - p.y);  // row N - v
+    // four loads in flight per thread, then their LDS writes
+    const int stage = (h + 1) * pairs, nt = (int)blockDim.x;
+    for (int i0 = threadIdx.x; i0 < stage; i0 += 4 * nt) {
+        float4 p[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int i = i0 + k * nt;
+            const int v = i / pairs, t = i % pairs;
+            p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < stage && nonempty[v]) p[k] = *(const float4*)&src[(int64_t)v * n + 2 * t];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int i = i0 + k * nt;
+            if (i >= stage) break;
+            const int v = i / pairs, t = i % pairs;
+            float2* zr = rows + (size_t)t * ld;
+            zr[v] = make_float2(p[k].x - p[k].w, p[k].y + p[k].z);  // c_x[v] + i c_{x+1}[v]
+            if (v > 0 && v < h) zr[n - v] = make_float2(p[k].x + p[k].w, p[k].z - p[k].y);  // row N - v
+        }
     }
     __syncthreads();
     const int rounds = kBlockTeam ? pairs : (pairs + teams - 1) / teams;
@@ -1232,14 +1245,22 @@ k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     __syncthreads();
     const float s = (float)(1.0 / (double)count);
     float2* out = sd.images + (int64_t)img * n * n;
+    // columns x and x + 1 (x even) stay adjacent after the shift unless N/2 is
+    // odd and x + 1 = N/2; with N/2 even the pair starts at an even (16-byte
+    // aligned) column: one 16-byte store per pair, else two 8-byte ones
     for (int i = threadIdx.x; i < n * pairs; i += blockDim.x) {
         const int r = i / pairs, t = i % pairs;
         const float2 z = rows[(size_t)t * ld + r];
         const int orow = r < h ? r + h : r - h;
         const int x = c0 + 2 * t;
         const int ox0 = x < h ? x + h : x - h, ox1 = x + 1 < h ? x + 1 + h : x + 1 - h;
-        store_stream(&out[(int64_t)orow * n + ox0], make_float2(z.x * s, 0.f));
-        store_stream(&out[(int64_t)orow * n + ox1], make_float2(z.y * s, 0.f));
+        float2* o = &out[(int64_t)orow * n];
+        if (ox1 == ox0 + 1 && (ox0 & 1) == 0) {
+            store_stream((float4*)&o[ox0], make_float4(z.x * s, 0.f, z.y * s, 0.f));
+        } else {
+            store_stream(&o[ox0], make_float2(z.x * s, 0.f));
+            store_stream(&o[ox1], make_float2(z.y * s, 0.f));
+        }
     }
 }
 
