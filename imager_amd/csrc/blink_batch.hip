@@ -1,23 +1,28 @@
 // Batched visibilities -> images kernels for power-of-two grid sides, built on
 // the register-resident FFT (blink_fft_reg.hpp).
 //
-//   k_grid_rows_reg : (row tile, image) -> plan entries gathered (pol, phase
-//                     correction, NaN/MAX_VIS) and accumulated in LDS, each
-//                     non-empty row transformed and written to the
+// Up to N = 2048 the plan is transposed (SetDesc::uv_swap): its rows are grid
+// columns u and an entry's position is v, so the first pass transforms along v
+// and the second along u, the image's fast axis, and writes whole image rows.
+//
+//   k_grid_rows_reg : (tile of plan rows, image) -> plan entries gathered (pol,
+//                     phase correction, NaN/MAX_VIS) and accumulated in LDS,
+//                     each non-empty plan row transformed and written to the
 //                     column-blocked row-pass buffer (whole lines);
-//   k_cols_reg      : (column block, image) -> the block's column pairs staged
-//                     through LDS from one contiguous run of the buffer (rows
-//                     with no plan entries read as zero), transformed, scaled by
-//                     (float)(1/sum counts) and stored fft-shifted into
-//                     images[img][y][x].
+//   k_cols_reg      : (block of image rows, image) -> the block's sequence pairs
+//                     staged through LDS from one contiguous run of the buffer
+//                     (plan rows with no entries read as zero), transformed,
+//                     scaled by (float)(1/sum counts) and stored fft-shifted into
+//                     images[img][y][x], two whole image rows per pair.
 //
 // Hermitian half: every visibility is gridded at (u, v) and conjugated at
-// (-u, -v), so the grid satisfies G[-v][-u] = conj(G[v][u]) and the row
-// transforms R_v(x) = sum_u G[v][u] w^(ux) satisfy R_{-v}(x) = conj(R_v(x)).
-// Only rows v = 0 .. N/2 are gridded and transformed; the column pass reads
-// R_v(x) for v <= N/2 and uses conj(R_{N-v}(x)) for the others.  The count
-// sum stays exact: an entry in a row 0 < v < N/2 stands for itself and for its
-// conjugate partner in row N - v, so it counts twice.
+// (-u, -v), so the grid satisfies G[-v][-u] = conj(G[v][u]).  With p the plan
+// row coordinate (u up to N = 2048, v at N = 4096) and q the other, the
+// first-pass transforms R_p(s) = sum_q G w^(qs) satisfy R_{-p}(s) =
+// conj(R_p(s)).  Only plan rows p = 0 .. N/2 are gridded and transformed; the
+// second pass reads R_p(s) for p <= N/2 and uses conj(R_{N-p}(s)) for the
+// others.  The count sum stays exact: an entry in a plan row 0 < p < N/2 stands
+// for itself and for its conjugate partner in row N - p, so it counts twice.
 #include "blink_fft_reg.hpp"
 #include "blink_gather.hpp"
 #include "blink_kernels.hpp"
@@ -34,9 +39,10 @@ __host__ __device__ constexpr int tile_stride(int n, int rows) {
     return p + ((want - p % 32) + 32) % 32;
 }
 
-// Row-pass buffer layout of the register kernels (N = 128 .. 2048): rows
-// v = 0 .. N/2 of each image, cut into blocks of kInterBlock columns, each
-// block stored row after row: element (v, x) of image li at
+// Row-pass buffer layout of the register kernels (N = 128 .. 2048): plan
+// rows v = 0 .. N/2 of each image (grid columns u, with the transposed plan),
+// cut into blocks of kInterBlock columns, each block stored row after row:
+// element (v, x) of image li at
 //   li (N/2 + 1) N + (x / W) (N/2 + 1) W + v W + x % W,   W = kInterBlock.
 // A transformed row is written as N / W segments of W * 8 = 256 bytes (whole
 // lines), and a column block of up to W columns reads one contiguous run.
@@ -175,13 +181,14 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     }
 }
 
-// Column pass.  The gridded plane is Hermitian by construction (every
+// Second pass.  The gridded plane is Hermitian by construction (every
 // visibility lands at (u, v) and its conjugate at (-u, -v)), so every image
-// is real and each column sequence c_x[v] = R_v(x) obeys c_x[N-v] = conj(c_x[v]).
-// One complex FFT therefore transforms two columns at once:
-//   z[v] = c_x[v] + i c_{x+1}[v]  ->  Z[y] = img_x[y] + i img_{x+1}[y],
-// with both image columns real; the imaginary parts of the output are 0.
-// CB output columns per workgroup, CB / 2 teams of L lanes.
+// is real and each sequence c_y[p] = R_p(y) over the plan rows obeys
+// c_y[N-p] = conj(c_y[p]).  One complex FFT therefore transforms two at once:
+//   z[p] = c_y[p] + i c_{y+1}[p]  ->  Z[x] = img[y][x] + i img[y+1][x],
+// with both image rows real; the imaginary parts of the output are 0.  The
+// plan is transposed (p = u), so the buffer's column index is the image row y.
+// CB image rows per workgroup, CB / 2 teams of L lanes.
 template <int N, int CB_, bool kMulti>
 __global__ void __launch_bounds__(CB_ / 2 * BatchLanes<N>::L)
 k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
@@ -210,11 +217,11 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     for (int i = tid; i <= H; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
     __syncthreads();
 
-    // stage z_t[v] and z_t[N - v] from columns c0 + 2t, c0 + 2t + 1 of row v:
-    // consecutive threads read consecutive 16-byte column pairs, and with the
-    // column-blocked buffer consecutive rows of a block follow each other, so
-    // the block's rows are one contiguous run; rows without plan entries are
-    // zero (all of a thread's loads are issued before its LDS writes)
+    // stage z_t[p] and z_t[N - p] from buffer columns c0 + 2t, c0 + 2t + 1 of
+    // plan row p: consecutive threads read consecutive 16-byte column pairs,
+    // and with the column-blocked buffer consecutive rows of a block follow
+    // each other, so the block's rows are one contiguous run; rows without plan
+    // entries are zero (all of a thread's loads are issued before its LDS writes)
     const int c0 = cb * CB;
     constexpr int STAGE = (H + 1) * TEAMS, ITER = (STAGE + NT - 1) / NT;
     float4 p[ITER];
@@ -231,7 +238,7 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
         const int i = tid + k * NT;
         if (i >= STAGE) break;
         const int vr = i / TEAMS, t = i % TEAMS;
-        // a = c_x[v] = (p.x, p.y), b = c_{x+1}[v] = (p.z, p.w); row N - v holds conjugates
+        // a = c_y[p] = (p.x, p.y), b = c_{y+1}[p] = (p.z, p.w); row N - p holds conjugates
         float2* zr = tile + t * TS;
         zr[pad_index(vr)] = make_float2(p[k].x - p[k].w, p[k].y + p[k].z);
         if (vr > 0 && vr < H)
@@ -251,20 +258,20 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     team_sync<false>();
     F::store(v, lane, [&](int i, float2 x) { row[pad_index(i)] = x; });
     __syncthreads();
-    // shifted store: images[img][shift(r)][shift(c0 + 2 t) + {0, 1}] =
-    // (Re Z_t[r], 0), (Im Z_t[r], 0) -- 16 bytes per thread, CB columns per row
+    // shifted store: images[img][shift(c0 + 2 t) + {0, 1}][shift(x) + {0, 1}]
+    // = (Re Z_t[x], 0), (Re Z_t[x + 1], 0) and the same of Im Z_t in the next
+    // row -- consecutive threads on consecutive column pairs, so each wave
+    // writes 1 KiB runs of one image row (even x: shift(x + 1) = shift(x) + 1)
     float2* out = sd.images + (int64_t)img * N * N;
-    const int oc0 = c0 < H ? c0 + H : c0 - H;  // shift of a block that does not straddle N/2
-    for (int i = tid; i < TEAMS * N; i += NT) {
-        const int r = i / TEAMS, t = i % TEAMS;
-        const int orow = r < H ? r + H : r - H;
-        const float2 z = tile[t * TS + pad_index(r)];
-        float4* o = (float4*)&out[(int64_t)orow * N + oc0 + 2 * t];
-        const float4 val = make_float4(z.x, 0.f, z.y, 0.f);
-        // streaming stores when a block writes whole 128-byte lines; narrower
-        // blocks leave the merging of line halves to L2
-        if constexpr (CB >= 16) store_stream(o, val);
-        else *o = val;
+    for (int i = tid; i < TEAMS * H; i += NT) {
+        const int t = i / H, ox = 2 * (i % H);
+        const int x = ox < H ? ox + H : ox - H;
+        const int y = c0 + 2 * t;
+        const int oy = y < H ? y + H : y - H;
+        const float2 z0 = tile[t * TS + pad_index(x)], z1 = tile[t * TS + pad_index(x + 1)];
+        float4* o = (float4*)&out[(int64_t)oy * N + ox];
+        store_stream(o, make_float4(z0.x, 0.f, z1.x, 0.f));
+        store_stream(o + N / 2, make_float4(z0.y, 0.f, z1.y, 0.f));
     }
 }
 

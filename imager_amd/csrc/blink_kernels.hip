@@ -541,8 +541,8 @@ k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ unit
         int up, vp;
         uint32_t packed = kNoCell;
         if (!fl[k] && plan_cell(uu[k], vv[k], wl, r_wl, r_du, r_dv, unit_fast, sd, up, vp)) {
-            const int r = wrap_index(vp, n);
-            packed = ((uint32_t)r << 16) | (uint32_t)wrap_index(up, n);
+            const int r = wrap_index(sd.uv_swap ? up : vp, n);
+            packed = ((uint32_t)r << 16) | (uint32_t)wrap_index(sd.uv_swap ? vp : up, n);
             // only rows 0 .. N/2 are planned (the consumers rebuild the rest
             // from the Hermitian symmetry): the cell or its conjugate, or both
             // in rows 0 and N/2

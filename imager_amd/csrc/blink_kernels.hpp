@@ -70,6 +70,8 @@ struct SetDesc {
     float2* images;           // [n_img][N][N]
     double* count_sums;       // optional [n_img]: isums as doubles, written by the
                               // column pass (first column block of each image)
+    int uv_swap;              // transposed plan: its rows are grid columns u and an
+                              // entry's position is v (rows are v otherwise)
 };
 
 // The set of group image gi, by value.  Kernels are built twice: for a
