@@ -299,14 +299,7 @@ static hipError_t launch_gemm(const DftGemmArgs& g, int prec, hipStream_t stream
     const int nsub = (g.n_pad % 64 == 0 && gemm_lds(g.k_pad, 2) <= (size_t)kGemmLdsMax) ? 2 : 1;
     const size_t lds = gemm_lds(g.k_pad, nsub);
     if (lds > (size_t)kGemmLdsMax) return hipErrorInvalidConfiguration;
-    static int n_cu = 0;
-    if (!n_cu) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            n_cu <= 0)
-            n_cu = 256;
-    }
+    const int n_cu = device_cu_count();
     const int n_slices = g.n_pad / (32 * nsub);
     const int64_t m_tiles = (g.m_rows + kGemmRows - 1) / kGemmRows;
     const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(2, kGemmLdsMax / lds));

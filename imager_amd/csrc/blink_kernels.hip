@@ -11,6 +11,18 @@ namespace blink {
 int64_t g_lds_limit = 64 * 1024;
 int64_t g_lds_budget = 48 * 1024;
 
+int device_cu_count() {
+    static const int n_cu = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        return n;
+    }();
+    return n_cu;
+}
+
 // Launch-time guard: a kernel whose LDS request exceeds the device limit is
 // not launched.
 static bool lds_fits(size_t bytes) { return (int64_t)bytes <= g_lds_limit; }

@@ -101,6 +101,10 @@ struct ColArgs {
     int n_ch;
 };
 
+// Compute units of the current device (queried once per process: every device
+// the library runs on is the same type).
+int device_cu_count();
+
 bool make_fft_plan(int n, FftPlan& plan);
 // Whether the in-place (paired) shifted column pass fits in LDS for side n.
 bool cols_paired_fits(int n);
