@@ -998,7 +998,7 @@ static std::pair<int, int> default_tiles(int n) {
         case 128: return {32, 64};
         case 256: return {8, 32};
         case 512: return {8, 16};
-        case 1024: return {8, 32};
+        case 1024: return {8, 16};
         case 2048: return {8, 16};
         case 4096: return {2, 8};
         default: return {0, 0};
