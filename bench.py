@@ -147,11 +147,17 @@ def cpu_baseline(obs, budget_s: float):
     n_t = min(obs.n_t, max(1, want // n_ch))
     dt = run(n_t, n_ch)
     k = n_t * n_ch
-    return {"value": k / dt, "unit": "images/s", "cores": threads, "kind": "port",
-            "sample": f"{n_t} timestep(s) x {n_ch} channel(s) = {k} images of the workload "
-                      f"({obs.n_ant} antennas, {obs.n_pixels}^2): geometric+cable corrections, "
-                      f"gridding, FFT, shift+norm in oracle/blink_oracle.c with {threads} OpenMP "
-                      f"threads, {dt:.1f} s"}
+    # a workload smaller than the budget is imaged again, so that the figure
+    # rests on at least two thirds of the budget of CPU work
+    reps = 1
+    while dt < 0.67 * budget_s and reps < 8:
+        dt += run(n_t, n_ch)
+        reps += 1
+    return {"value": reps * k / dt, "unit": "images/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} pass(es) over {n_t} timestep(s) x {n_ch} channel(s) = {k} images "
+                      f"of the workload ({obs.n_ant} antennas, {obs.n_pixels}^2): geometric+cable "
+                      f"corrections, gridding, FFT, shift+norm in oracle/blink_oracle.c with "
+                      f"{threads} OpenMP threads, {dt:.1f} s"}
 
 
 def fft_mkl_rate(n_pix: int, seconds: float = 3.0):
