@@ -160,6 +160,46 @@ def cpu_baseline(obs, budget_s: float):
                       f"{threads} OpenMP threads, {dt:.1f} s"}
 
 
+def cpu_stages(obs, max_images: int = 16, reps: int = 5):
+    """The oracle's stages timed one by one on the first images of the workload
+    (one warm-up, then the median of `reps` runs each), in images/s:
+    corrections (geometric + cable), gridding, FFT, and shift+norm (each image
+    scaled by (float)(1 / sum counts), then the oracle's fft_shift)."""
+    import oracle
+    n_ch = min(obs.n_ch, max_images)
+    n_t = min(obs.n_t, max(1, max_images // n_ch))
+    k = n_t * n_ch
+    vis, freq = obs.vis[:n_t, :n_ch], obs.freq[:n_ch]
+
+    def median_s(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    def corrections():
+        v = oracle.geometric_corrections(vis, obs.n_ant, obs.w, freq)
+        return oracle.cable_corrections(v, obs.n_ant, obs.cable, freq)
+
+    vis_c = corrections()
+    grid = lambda: oracle.gridding(vis_c, obs.n_ant, obs.u, obs.v, obs.flags, freq, obs.delta_u,
+                                   obs.delta_v, obs.min_uv, obs.n_pixels, "XX")
+    g, c, _ = grid()
+    t_corr = median_s(corrections)
+    t_grid = median_s(grid)
+    img = oracle.fft2d(g, +1)
+    t_fft = median_s(lambda: oracle.fft2d(g, +1))
+    flat = img.reshape(k, obs.n_pixels, obs.n_pixels)
+    scale = (1.0 / c.reshape(k, -1).sum(1, dtype=np.float64)).astype(np.float32)
+    t_shift = median_s(lambda: [oracle.fft_shift(flat[i] * scale[i]) for i in range(k)])
+    return {"images": k, "reps": reps,
+            "corrections_images_per_s": k / t_corr, "gridding_images_per_s": k / t_grid,
+            "fft_images_per_s": k / t_fft, "shift_norm_images_per_s": k / t_shift}
+
+
 def fft_mkl_rate(n_pix: int, seconds: float = 3.0):
     """Batched 2-D FFT images/s on this host's CPU through torch (MKL), an
     FFTW-class bound for the FFT stage of the CPU baseline."""
@@ -538,6 +578,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(obs, args.cpu_seconds)
+        cpu["stages"] = cpu_stages(obs)
         try:
             rate, thr = fft_mkl_rate(obs.n_pixels)
             cpu["fft_stage_mkl_images_per_s"] = rate
