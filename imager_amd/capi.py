@@ -282,7 +282,8 @@ class Context:
         self._check(load().blink_release_cache(self._h))
 
     def batch_tiles(self, n_pixels):
-        """(rows per row tile, columns per column block) of the batched kernels."""
+        """(lines per first-pass tile, output lines per second-pass block) of the
+        batched kernels."""
         r, c = ctypes.c_int(), ctypes.c_int()
         self._check(load().blink_batch_tiles(self._h, int(n_pixels), ctypes.byref(r),
                                              ctypes.byref(c)))

@@ -504,8 +504,8 @@ static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
     return hipGetLastError();
 }
 
-// Built variants: rows per row tile, and output columns per column block
-// (two columns per team).
+// Built variants: plan rows per row tile, and output lines per second-pass
+// block (image rows up to N = 2048, image columns at 4096; two per team).
 #define BLINK_ROW_VARIANTS(X) \
     X(128, 8) X(128, 16) X(128, 32) \
     X(256, 4) X(256, 8) X(256, 16) X(256, 32) \

@@ -990,7 +990,7 @@ static BatchTables tables_at(const BatchPlan& bp, const void* dblob) {
     return t;
 }
 
-// Default tiles (rows per row tile, output columns per column block): the
+// Default tiles (plan rows per row tile, output lines per second-pass block): the
 // pairs measured fastest on MI355X for the 128-tile and 256-antenna arrays;
 // the autotuner keeps them unless another pair is more than 3 % faster.
 static std::pair<int, int> default_tiles(int n) {

@@ -206,7 +206,7 @@ size_t dft_planes_bytes(int n, int stage);
 bool batch_reg_supported(int n);
 // Bytes of the row-pass buffer per image of side n.
 int64_t batch_inter_bytes(int n);
-// Built tile sizes for side n: rows per row tile, output columns per column block.
+// Built tile sizes for side n: plan rows per row tile, output lines per second-pass block.
 int batch_row_variants(int n, int* tiles, int max_tiles);
 int batch_col_variants(int n, int* tiles, int max_tiles);
 hipError_t set_batch_kernel_attributes();

@@ -286,15 +286,18 @@ BLINK_API int blink_profile_get(blink_ctx* ctx, const char* kernel, int64_t* lau
 BLINK_API int blink_profile_kernels(blink_ctx* ctx, char* buf, int buf_len);
 BLINK_API int blink_profile_reset(blink_ctx* ctx);
 
-/* Tile sizes of the batched kernels for grid side n_pixels: rows per row
- * tile and columns per column block.  Positive when chosen by the built-in
+/* Tile sizes of the batched kernels for grid side n_pixels: grid rows per
+ * first-pass tile (grid columns u up to 2048^2, where the first pass runs
+ * along v) and output lines per second-pass block (image rows up to 2048^2,
+ * image columns at 4096^2).  Positive when chosen by the built-in
  * autotuner (the first blink_image_batch call on a new side times every
  * built variant unless BLINK_AUTOTUNE=0 is set in the environment),
  * negative for the defaults not yet tuned, 0 when the side uses the generic
  * kernels. */
 BLINK_API int blink_batch_tiles(blink_ctx* ctx, int n_pixels, int* row_tile, int* col_tile);
 /* Pins the tile sizes of the batched kernels for grid side n_pixels (row_tile
- * rows per row tile, col_tile output columns per column block; both must be
+ * lines per first-pass tile, col_tile output lines per second-pass block, as
+ * for blink_batch_tiles; both must be
  * built variants) and turns the autotuner off for that side; 0, 0 returns the
  * side to the autotuner.  No counterpart in the reference: for callers that
  * need reproducible launch shapes, and for tests of every variant. */
