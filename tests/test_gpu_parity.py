@@ -518,6 +518,40 @@ def test_image_batch_two_sets(ctx):
         assert oracle.peak_normalised_error(_host(im), ref)[0] < IMAGE_TOL
 
 
+@pytest.mark.parametrize("n,dft", [(256, False), (1024, False), (180, False), (64, True)])
+def test_image_batch_min_uv(ctx, n, dft):
+    """A finite min_uv (the reference's sqrt(u^2 + v^2) <= min_uv skip) takes
+    the plan's exact cell route: with a third of the baselines cut, images and
+    exact count sums against the oracle on the register (256, 1024: transposed
+    plan), generic (180) and DFT (64) paths."""
+    import dataclasses
+    import torch
+    from imager_amd import capi
+    o = synth.small(n_ant=40, n_ch=2, n_t=2, n_pixels=n, seed=900 + n, height_sigma_m=1.0)
+    lam = 299792458.0 / o.freq[0]
+    uvd = np.hypot(o.u.astype(np.float64), o.v.astype(np.float64)) / lam
+    o = dataclasses.replace(o, min_uv=float(np.quantile(uvd[o.flags == 0], 0.33)))
+    ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, n)
+    _, c_ref, n_grid = oracle.gridding(oracle.cable_corrections(
+        oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq), o.n_ant, o.cable, o.freq),
+        o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v, o.min_uv, n, "XX")
+    _, c_all, n_all = oracle.gridding(o.vis, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u,
+                                      o.delta_v, -np.inf, n, "XX")
+    assert 0 < n_grid < 0.8 * n_all   # the cut removes a good share of the visibilities
+    d = _obs_dev(o)
+    imgs = torch.empty((o.n_t, o.n_ch, n, n), dtype=torch.complex64, device="cuda")
+    sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
+    st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                       d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, n, "XX", imgs,
+                       sums, transform=capi.TRANSFORM_DFT_BF16X3 if dft else capi.TRANSFORM_FFT)
+    _say(f"batch min_uv={o.min_uv:.3g} N={n} dft={dft}")
+    ctx.image_batch([st])
+    got = _host(imgs)
+    assert oracle.peak_normalised_error(got, ref)[0] < IMAGE_TOL
+    assert np.array_equal(_host(sums), c_ref.reshape(o.n_images, -1).sum(1).astype(np.float64))
+
+
 def test_image_batch_grouped_sets(ctx):
     """Several sets per grid side in one call (one launch pair per side):
     different arrays, channel and timestep counts, flags and polarisations,
