@@ -1,9 +1,9 @@
 // Batched visibilities -> images kernels for power-of-two grid sides, built on
 // the register-resident FFT (blink_fft_reg.hpp).
 //
-// Up to N = 2048 the plan is transposed (SetDesc::uv_swap): its rows are grid
-// columns u and an entry's position is v, so the first pass transforms along v
-// and the second along u, the image's fast axis, and writes whole image rows.
+// The plan is transposed (SetDesc::uv_swap): its rows are grid columns u and an
+// entry's position is v, so the first pass transforms along v and the second
+// along u, the image's fast axis, and writes whole image rows.
 //
 //   k_grid_rows_reg : (tile of plan rows, image) -> plan entries gathered (pol,
 //                     phase correction, NaN/MAX_VIS) and accumulated in LDS,
@@ -17,7 +17,7 @@
 //
 // Hermitian half: every visibility is gridded at (u, v) and conjugated at
 // (-u, -v), so the grid satisfies G[-v][-u] = conj(G[v][u]).  With p the plan
-// row coordinate (u up to N = 2048, v at N = 4096) and q the other, the
+// row coordinate (the grid column u) and q the other (v), the
 // first-pass transforms R_p(s) = sum_q G w^(qs) satisfy R_{-p}(s) =
 // conj(R_p(s)).  Only plan rows p = 0 .. N/2 are gridded and transformed; the
 // second pass reads R_p(s) for p <= N/2 and uses conj(R_{N-p}(s)) for the
@@ -278,12 +278,13 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
 // ------------------------------------------------------------- N = 4096
 //
 // At N = 4096 one row needs 4 wavefronts (256 lanes x 16 values) and 34 KiB
-// of LDS, so a workgroup holds one to four rows or column pairs.  The row-pass
-// buffer is row-major as for the smaller sides: inter[img][v][x], v = 0 ..
-// N/2.  The row kernel stores each transformed row with unit-stride lanes; the
-// column kernel stages its CB columns of rows 0 .. N/2 through LDS (16-byte
-// loads, two columns per thread), forming z = c_x + i c_{x+1} for rows v and
-// N - v on the way.
+// of LDS, so a workgroup holds one to four rows or column pairs.  The plan is
+// transposed as for the smaller sides, and the row-pass buffer is row-major:
+// inter[img][p][y], plan rows p = 0 .. N/2.  The row kernel stores each
+// transformed row with unit-stride lanes; the second-pass kernel stages its
+// CB columns y of plan rows 0 .. N/2 through LDS (16-byte loads, two columns
+// per thread), forming z = c_y + i c_{y+1} for rows p and N - p on the way,
+// and writes image rows y and y + 1 whole.
 // Twiddles come from two 64-entry tables (w(m) = lo[m % 64] hi[m / 64]) kept
 // in LDS, instead of a 32 KiB table.
 
@@ -423,16 +424,18 @@ k_cols_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     __syncthreads();
     F::store(v, lane, [&](int i, float2 x) { row[pad_index(i)] = x; });
     __syncthreads();
-    // shifted store, as in k_cols_reg: (Re Z_t[r], 0), (Im Z_t[r], 0)
+    // shifted store of whole image rows, as in k_cols_reg (the plan is
+    // transposed, so pair t holds image rows c0 + 2t and c0 + 2t + 1)
     float2* out = sd.images + (int64_t)img * N * N;
-    const int oc0 = c0 < H ? c0 + H : c0 - H;
-    for (int i = tid; i < TEAMS * N; i += NT) {
-        const int r = i / TEAMS, t = i % TEAMS;
-        const int orow = r < H ? r + H : r - H;
-        const float2 z = tile[t * TS + pad_index(r)];
-        // normal stores: a row segment here is 64 bytes, and L2 has to merge
-        // the halves of each line that neighbouring column blocks write
-        *(float4*)&out[(int64_t)orow * N + oc0 + 2 * t] = make_float4(z.x, 0.f, z.y, 0.f);
+    for (int i = tid; i < TEAMS * H; i += NT) {
+        const int t = i / H, ox = 2 * (i % H);
+        const int x = ox < H ? ox + H : ox - H;
+        const int y = c0 + 2 * t;
+        const int oy = y < H ? y + H : y - H;
+        const float2 z0 = tile[t * TS + pad_index(x)], z1 = tile[t * TS + pad_index(x + 1)];
+        float4* o = (float4*)&out[(int64_t)oy * N + ox];
+        store_stream(o, make_float4(z0.x, 0.f, z1.x, 0.f));
+        store_stream(o + N / 2, make_float4(z0.y, 0.f, z1.y, 0.f));
     }
 }
 

@@ -932,9 +932,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.isums = bp.sc[i].isums;
             d.images = (float2*)st.images;
             d.count_sums = st.count_sums;
-            // the register kernels up to 2048^2 transform along v first, so that
-            // their second pass writes whole image rows
-            d.uv_swap = g.reg && g.n <= 2048;
+            // the register kernels transform along v first, so that their
+            // second pass writes whole image rows
+            d.uv_swap = g.reg;
             base += d.n_img;
             desc_of[i] = (int)desc.size();
             desc.push_back(d);
