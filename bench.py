@@ -116,14 +116,16 @@ def dft_flops(kernel: str, n_pix: int, n_img: int):
 
 
 def read_traffic(kernel: str, workload: str):
-    """HBM bytes per launch measured with rocprofv3 PMC counters, when a summary
-    for this kernel and workload has been recorded under profiles/."""
+    """HBM bytes per image measured with rocprofv3 PMC counters (the recorded
+    launch's bytes over its images), when a summary for this kernel and
+    workload has been recorded under profiles/."""
     p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
+            d = json.load(f)
+        return float(d["hbm_bytes_per_launch"]) / float(d["images_per_launch"])
     except Exception:
         return None
 
@@ -470,7 +472,8 @@ def main():
         elif alg is not None:
             avg_s = kernels[dom]["avg_ms"] * 1e-3
             ach = alg / avg_s / 1e9
-            traffic = read_traffic(dom, args.workload)
+            per_image = read_traffic(dom, args.workload)
+            traffic = per_image * imgs_per_launch if per_image else None
             roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                     "images_per_launch": imgs_per_launch, "alg_bytes_per_launch": alg}
