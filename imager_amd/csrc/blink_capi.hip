@@ -932,9 +932,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.isums = bp.sc[i].isums;
             d.images = (float2*)st.images;
             d.count_sums = st.count_sums;
-            // the register kernels transform along v first, so that their
-            // second pass writes whole image rows
-            d.uv_swap = g.reg;
+            // the FFT kernels transform along v first, so that their second
+            // pass writes whole image rows (the DFT stages keep rows v)
+            d.uv_swap = g.transform == BLINK_TRANSFORM_FFT;
             base += d.n_img;
             desc_of[i] = (int)desc.size();
             desc.push_back(d);

@@ -1185,13 +1185,14 @@ k_grid_rows_fft(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     }
 }
 
-// Column pass of the generic batched path, from the half-height row-major
-// row-pass buffer inter[img][v][x], v = 0 .. N/2.  As in the register path,
-// the images are real, so one complex FFT of z = c_x + i c_{x+1} gives
-// columns x and x+1; rows N - v are the conjugates of rows v.  A workgroup
-// owns `cb` (even) output columns = cb/2 pairs, staged through LDS with
-// 16-byte loads, transformed by the teams, scaled by 1/sum and stored
-// fftshifted with imaginary parts 0.
+// Second pass of the generic batched path, from the half-height row-major
+// row-pass buffer inter[img][p][y], plan rows p = 0 .. N/2 (the plan is
+// transposed: plan rows are grid columns u, transformed along v by the row
+// pass).  As in the register path, the images are real, so one complex FFT
+// over p of z = c_y + i c_{y+1} gives image rows y and y+1; plan rows N - p
+// are the conjugates of rows p.  A workgroup owns `cb` (even) buffer columns
+// = cb/2 pairs, staged through LDS with 16-byte loads, transformed by the
+// teams, scaled by 1/sum and stored fftshifted with imaginary parts 0.
 template <int S, bool kBlockTeam, bool kMulti>
 __global__ void __launch_bounds__(kFftThreads)
 k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
@@ -1257,22 +1258,18 @@ k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     __syncthreads();
     const float s = (float)(1.0 / (double)count);
     float2* out = sd.images + (int64_t)img * n * n;
-    // columns x and x + 1 (x even) stay adjacent after the shift unless N/2 is
-    // odd and x + 1 = N/2; with N/2 even the pair starts at an even (16-byte
-    // aligned) column: one 16-byte store per pair, else two 8-byte ones
-    for (int i = threadIdx.x; i < n * pairs; i += blockDim.x) {
-        const int r = i / pairs, t = i % pairs;
-        const float2 z = rows[(size_t)t * ld + r];
-        const int orow = r < h ? r + h : r - h;
-        const int x = c0 + 2 * t;
-        const int ox0 = x < h ? x + h : x - h, ox1 = x + 1 < h ? x + 1 + h : x + 1 - h;
-        float2* o = &out[(int64_t)orow * n];
-        if (ox1 == ox0 + 1 && (ox0 & 1) == 0) {
-            store_stream((float4*)&o[ox0], make_float4(z.x * s, 0.f, z.y * s, 0.f));
-        } else {
-            store_stream(&o[ox0], make_float2(z.x * s, 0.f));
-            store_stream(&o[ox1], make_float2(z.y * s, 0.f));
-        }
+    // the plan is transposed: pair t holds Z_t[x] = img[y][x] + i img[y + 1][x]
+    // for y = c0 + 2t, so it writes image rows shift(y) and shift(y + 1) (not
+    // adjacent when N/2 is odd and y + 1 = N/2) whole, consecutive threads on
+    // consecutive output column pairs (even, so 16-byte aligned as N is even)
+    for (int i = threadIdx.x; i < h * pairs; i += blockDim.x) {
+        const int t = i / h, ox = 2 * (i % h);
+        const int x0 = ox < h ? ox + h : ox - h, x1 = ox + 1 < h ? ox + 1 + h : ox + 1 - h;
+        const float2 z0 = rows[(size_t)t * ld + x0], z1 = rows[(size_t)t * ld + x1];
+        const int y = c0 + 2 * t;
+        const int oy0 = y < h ? y + h : y - h, oy1 = y + 1 < h ? y + 1 + h : y + 1 - h;
+        store_stream((float4*)&out[(int64_t)oy0 * n + ox], make_float4(z0.x * s, 0.f, z1.x * s, 0.f));
+        store_stream((float4*)&out[(int64_t)oy1 * n + ox], make_float4(z0.y * s, 0.f, z1.y * s, 0.f));
     }
 }
 
@@ -1426,7 +1423,7 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
     return hipGetLastError();
 }
 
-// Output columns per workgroup of k_cols_half: the largest even divisor of
+// Buffer columns (image rows) per workgroup of k_cols_half: the largest even divisor of
 // n, at most 64, whose LDS rows fit next to the team scratch and that still
 // gives at least 256 workgroups over the n_img images (one per CU), else the
 // smallest, 2 (a single 180^2 image then runs 90 workgroups, not 3).

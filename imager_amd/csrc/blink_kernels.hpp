@@ -157,8 +157,9 @@ hipError_t launch_shift_norm(float* images, int nx, int ny, int n_images, const 
 hipError_t launch_average(const float* images, int64_t image_size, int n_images, int real_only,
                           float* avg, hipStream_t stream);
 // Generic batched path (any even side the FFT plan supports): row pass over
-// rows 0 .. N/2 into a row-major half-height buffer [img][N/2 + 1][N], and
-// the column pass from it (two real columns per complex FFT).
+// plan rows 0 .. N/2 (grid columns u, transposed plan) into a row-major
+// half-height buffer [img][N/2 + 1][N], and the second pass from it (two real
+// image rows per complex FFT).
 hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                 int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
                                 float* inter, const float* tw, hipStream_t stream);
