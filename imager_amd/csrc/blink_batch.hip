@@ -136,6 +136,17 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         if (tid + k * NT < N) twv[k] = tw[tid + k * NT];
     const int rsv = tid <= RB ? rst[min(s0 + tid, H + 1)] : 0;
     const int rrv = tid < RB ? rl[min(s0 + tid, H)] : 0;
+    // the gather's copies, uniform: row starts of the tile, and a bit per tile
+    // row whose plan row is 0 or N/2 (entries there count once, else twice)
+    int rsr[RB + 1];
+    unsigned edge = 0;
+#pragma unroll
+    for (int q = 0; q <= RB; q++) rsr[q] = rst[min(s0 + q, H + 1)];
+#pragma unroll
+    for (int q = 0; q < RB; q++) {
+        const int pr = rl[min(s0 + q, H)];
+        if (pr == 0 || pr == H) edge |= 1u << q;
+    }
     const double nu = g.freq_hz[ch];
     gb.load_records(g, rec);
     float2* twl = tile + RB * TS;  // twiddle table copy [N]
@@ -147,13 +158,16 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     for (int i = tid; i < RB * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
     auto sink = [&](int e, int u_idx, float2 v) {
+        // tile rows are consecutive non-empty plan rows (rows past the last one
+        // start at the end): the row of entry e counts the later starts <= e
         int r = 0;
-        while (r + 1 < RB && rs[r + 1] <= e) r++;
+#pragma unroll
+        for (int k = 1; k < RB; k++)
+            if (rsr[k] <= e) r = k;
         float2* c = &tile[r * TS + pad_index(u_idx)];
         atomicAdd(&c->x, v.x);
         atomicAdd(&c->y, v.y);
-        const int row = rrow[r];
-        return (row == 0 || row == H) ? 1 : 2;
+        return ((edge >> r) & 1u) ? 1 : 2;
     };
     int n_ok = gb.consume(g, nu, sink);
     for (int base = e0 + tid + NT * U; base < e1; base += NT * U) {
