@@ -127,6 +127,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     constexpr int U = 4;
     const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
     const float2* rec = (const float2*)g.vis + 4 * ((int64_t)img * g.n_bl);
+    const float2* cimg = sd.compact ? sd.compact + (int64_t)img * g.n_bl : nullptr;  // uniform
     GatherBatch<U> gb;
     gb.load_entries(ent, e0 + tid, e0, e1, NT);
     constexpr int TWK = (N + NT - 1) / NT;
@@ -148,7 +149,8 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         if (pr == 0 || pr == H) edge |= 1u << q;
     }
     const double nu = g.freq_hz[ch];
-    gb.load_records(g, rec);
+    if (cimg) gb.load_compact(cimg);
+    else gb.load_records(g, rec);
     float2* twl = tile + RB * TS;  // twiddle table copy [N]
 #pragma unroll
     for (int k = 0; k < TWK; k++)
@@ -169,12 +171,17 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         atomicAdd(&c->y, v.y);
         return ((edge >> r) & 1u) ? 1 : 2;
     };
-    int n_ok = gb.consume(g, nu, sink);
+    int n_ok = cimg ? gb.consume_compact(sink) : gb.consume(g, nu, sink);
     for (int base = e0 + tid + NT * U; base < e1; base += NT * U) {
         GatherBatch<U> gm;
         gm.load_entries(ent, base, e0, e1, NT);
-        gm.load_records(g, rec);
-        n_ok += gm.consume(g, nu, sink);
+        if (cimg) {
+            gm.load_compact(cimg);
+            n_ok += gm.consume_compact(sink);
+        } else {
+            gm.load_records(g, rec);
+            n_ok += gm.consume(g, nu, sink);
+        }
     }
     const int tot = block_sum_int(n_ok, red);
     if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
@@ -287,6 +294,42 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
         store_stream(o, make_float4(z0.x, 0.f, z1.x, 0.f));
         store_stream(o + N / 2, make_float4(z0.y, 0.f, z1.y, 0.f));
     }
+}
+
+// Visibilities of a set's images compacted for the row tiles: one float2
+// per (image, baseline), after the pol combination and the phase rotation, as
+// GatherBatch::consume forms them, or NaN where consume would skip the value
+// (NaN or |re|, |im| >= MAX_VIS).  A streaming pass over the records, so that
+// the row tiles' random gathers read 8 bytes per baseline instead of 32-byte
+// records, from a quarter of the cache lines, and do no FP64 work.
+__global__ void __launch_bounds__(256)
+k_vis_compact(SetDesc sd, int64_t n_items) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_items) return;
+    const GatherArgs& g = sd.g;
+    const int64_t img = i / g.n_bl;
+    const int bl = (int)(i % g.n_bl);
+    const float2* rec = (const float2*)g.vis + 4 * i;  // record of (img, bl)
+    const int p0 = g.pol == 1 ? 3 : (g.pol == 3 ? 1 : 0);
+    const int p1 = g.pol == 3 ? 2 : 3;
+    const float2 x0 = rec[p0];
+    const float2 x1 = g.pol >= 2 ? rec[p1] : make_float2(0.f, 0.f);
+    float2 v = select_pol2(x0, x1, g.pol);
+    if (g.corrections) {
+        const double nu = g.freq_hz[img % g.n_ch];
+        v = cmul(v, unit_phasor(((g.phase_coef[bl] * nu) / kVelLight)));
+    }
+    if (!vis_ok(v)) v = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+    sd.compact[i] = v;
+}
+
+hipError_t launch_vis_compact(const SetDesc& sd, int64_t n_img, hipStream_t stream) {
+    const int64_t n_items = n_img * sd.g.n_bl;
+    if (n_items == 0 || !sd.compact) return hipSuccess;
+    const int64_t blocks = (n_items + 255) / 256;
+    if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k_vis_compact, dim3((unsigned)blocks), dim3(256), 0, stream, sd, n_items);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------- N = 4096

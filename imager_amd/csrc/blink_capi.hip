@@ -723,6 +723,7 @@ struct SetScratch {
     uint32_t* isums;
     double* coef;  // [n_bl] phase coefficients
     int chunk;  // DFT sets: images per launch triple
+    float2* compact = nullptr;  // [n_img][n_bl] compacted visibilities, or none
 };
 
 struct BatchGroup {
@@ -857,8 +858,31 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     bp.n_max = n_max;
     const size_t partial_bytes = sizeof(int) * (size_t)n_units * parts * n_max;
 
-    void *inter, *rs, *en, *is, *pa, *cl, *co;
+    // A register-path side imaged from a single set, with at most 64
+    // baselines per plan row (a sparse grid), gathers compacted visibilities
+    // (one float2 per image and baseline) instead of the records: its row
+    // tiles hold few entries each, so the first-touch misses of the 32-byte
+    // records are exposed in every tile (640 x 1024^2, 8256 baselines: row
+    // pass 480 -> 350 us plus 36 us of compaction).  Dense grids amortise
+    // them over thousands of entries per tile and lose (768 x 256^2, 32896
+    // baselines: 412 -> 293 us plus 185 us).  BLINK_COMPACT=0 turns it off.
+    std::vector<size_t> cp_off(n_sets, SIZE_MAX);
+    size_t cp_bytes = 0;
+    {
+        const char* env = getenv("BLINK_COMPACT");
+        const bool on = !(env && env[0] == '0');
+        for (const BatchGroup& g : bp.groups) {
+            if (!on || !g.reg || g.n > 2048 || g.sets.size() != 1) continue;
+            const int i = g.sets[0];
+            const int64_t n_bl = (int64_t)sets[i].n_ant * (sets[i].n_ant + 1) / 2;
+            if (n_bl > 64 * (int64_t)(g.n / 2 + 1)) continue;
+            cp_off[i] = cp_bytes;
+            cp_bytes += align16(sizeof(float2) * (size_t)g.n_img * (size_t)n_bl);
+        }
+    }
+    void *inter, *rs, *en, *is, *pa, *cl, *co, *cp = nullptr;
     if ((rc = scratch(ctx, "batch_inter", inter_bytes, &inter))) return rc;
+    if (cp_bytes && (rc = scratch(ctx, "batch_compact", cp_bytes, &cp))) return rc;
     if ((rc = scratch(ctx, "batch_rows", std::max<size_t>(rs_bytes, 16), &rs))) return rc;
     if ((rc = scratch(ctx, "batch_entries", std::max<size_t>(en_bytes, 16), &en))) return rc;
     if ((rc = scratch(ctx, "batch_isums", std::max<size_t>(4 * (size_t)n_img_all, 16), &is)))
@@ -883,6 +907,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         bp.sc[i].entries = (uint32_t*)((char*)en + en_off[i]);
         bp.sc[i].isums = (uint32_t*)is + is_off[i];
         bp.sc[i].coef = (double*)co + co_off[i];
+        if (cp_off[i] != SIZE_MAX) bp.sc[i].compact = (float2*)((char*)cp + cp_off[i]);
         const int64_t per = dft_inter_bytes(sets[i].n_pixels);
         const int n_img = sets[i].n_ch * sets[i].n_t;
         bp.sc[i].chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
@@ -935,6 +960,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             // the FFT kernels transform along v first, so that their second
             // pass writes whole image rows (the DFT stages keep rows v)
             d.uv_swap = g.transform == BLINK_TRANSFORM_FFT;
+            d.compact = bp.sc[i].compact;
             base += d.n_img;
             desc_of[i] = (int)desc.size();
             desc.push_back(d);
@@ -1133,6 +1159,7 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
     const int ns = (int)g.sets.size();
     FftPlan plan;
     make_fft_plan(n, plan);
+    if (first.compact) LAUNCH("vis_compact", s, launch_vis_compact(first, g.n_img, s));
     for (int64_t gi0 = 0; gi0 < g.n_img; gi0 += g.chunk) {
         const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img - gi0);
         if (g.reg) {
@@ -1186,8 +1213,13 @@ static int autotune_batch(blink_ctx* ctx, const BatchPlan& bp, const BatchTables
     if (!need) return BLINK_OK;
     int rc = enqueue_plans(ctx, bp, t, s);
     if (rc) return rc;
-    for (const BatchGroup& g : bp.groups)
-        if (g.reg && !ctx->tiles.count(g.n) && (rc = autotune_group(ctx, bp, g, t, s))) return rc;
+    for (const BatchGroup& g : bp.groups) {
+        if (!g.reg || ctx->tiles.count(g.n)) continue;
+        // the timed row passes gather what the call will gather
+        const SetDesc& first = bp.host_desc(g);
+        if (first.compact) HIP_TRY(launch_vis_compact(first, g.n_img, s));
+        if ((rc = autotune_group(ctx, bp, g, t, s))) return rc;
+    }
     HIP_TRY(hipStreamSynchronize(s));
     return BLINK_OK;
 }

@@ -49,6 +49,27 @@ struct GatherBatch {
         }
     }
 
+    // the U compacted visibilities (cimg: the image's [n_bl] values, NaN where
+    // not gridded) instead of the records
+    __device__ __forceinline__ void load_compact(const float2* cimg) {
+#pragma unroll
+        for (int u = 0; u < U; u++) x0[u] = cimg[ev[u] >> 14];
+    }
+
+    template <class Sink>
+    __device__ __forceinline__ int consume_compact(Sink sink) const {
+        int total = 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (idx[u] < 0) continue;
+            float2 v = x0[u];
+            if (isnan(v.x)) continue;
+            if (ev[u] & (1u << 13)) v.y = -v.y;
+            total += sink(idx[u], (int)(ev[u] & 0x1fffu), v);
+        }
+        return total;
+    }
+
     template <class Sink>
     __device__ __forceinline__ int consume(const GatherArgs& g, double nu, Sink sink) const {
         int total = 0;

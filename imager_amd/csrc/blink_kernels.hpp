@@ -72,6 +72,10 @@ struct SetDesc {
                               // column pass (first column block of each image)
     int uv_swap;              // transposed plan: its rows are grid columns u and an
                               // entry's position is v (rows are v otherwise)
+    float2* compact;          // optional [n_img][n_bl]: each baseline's visibility
+                              // after the pol combination and the phase rotation,
+                              // NaN where it is not gridded (k_vis_compact); the row
+                              // tiles then gather these instead of the records
 };
 
 // The set of group image gi, by value.  Kernels are built twice: for a
@@ -217,5 +221,8 @@ hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const
 hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
                            int n_sets, int n, int tile, int64_t gi0, int n_img,
                            const float* inter, const float* tw, hipStream_t stream);
+// Fills sd.compact for all n_img images of the set (after the plan, which
+// writes the phase coefficients).
+hipError_t launch_vis_compact(const SetDesc& sd, int64_t n_img, hipStream_t stream);
 
 }  // namespace blink
