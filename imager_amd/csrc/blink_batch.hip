@@ -302,13 +302,9 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
 // (NaN or |re|, |im| >= MAX_VIS).  A streaming pass over the records, so that
 // the row tiles' random gathers read 8 bytes per baseline instead of 32-byte
 // records, from a quarter of the cache lines, and do no FP64 work.
-__global__ void __launch_bounds__(256)
-k_vis_compact(SetDesc sd, int64_t n_items) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n_items) return;
+__device__ __forceinline__ void vis_compact_one(const SetDesc& sd, int64_t img, int bl) {
     const GatherArgs& g = sd.g;
-    const int64_t img = i / g.n_bl;
-    const int bl = (int)(i % g.n_bl);
+    const int64_t i = img * g.n_bl + bl;
     const float2* rec = (const float2*)g.vis + 4 * i;  // record of (img, bl)
     const int p0 = g.pol == 1 ? 3 : (g.pol == 3 ? 1 : 0);
     const int p1 = g.pol == 3 ? 2 : 3;
@@ -323,12 +319,42 @@ k_vis_compact(SetDesc sd, int64_t n_items) {
     sd.compact[i] = v;
 }
 
+// One set: item i = (image, baseline) of its n_img x n_bl.
+__global__ void __launch_bounds__(256)
+k_vis_compact(SetDesc sd, int64_t n_items) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_items) return;
+    vis_compact_one(sd, i / sd.g.n_bl, (int)(i % sd.g.n_bl));
+}
+
+// Several sets of one side: bpi workgroups per group image gi, each 256
+// baselines of it; images of sets without a compact buffer are skipped.
+__global__ void __launch_bounds__(256)
+k_vis_compact_sets(const SetDesc* __restrict__ sets, const int* __restrict__ img_set, int bpi) {
+    const int64_t gi = blockIdx.x / bpi;
+    const SetDesc sd = sets[img_set[gi]];
+    const int bl = (int)(blockIdx.x % bpi) * 256 + (int)threadIdx.x;
+    if (!sd.compact || bl >= sd.g.n_bl) return;
+    vis_compact_one(sd, gi - sd.img_base, bl);
+}
+
 hipError_t launch_vis_compact(const SetDesc& sd, int64_t n_img, hipStream_t stream) {
     const int64_t n_items = n_img * sd.g.n_bl;
     if (n_items == 0 || !sd.compact) return hipSuccess;
     const int64_t blocks = (n_items + 255) / 256;
     if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_vis_compact, dim3((unsigned)blocks), dim3(256), 0, stream, sd, n_items);
+    return hipGetLastError();
+}
+
+hipError_t launch_vis_compact_sets(const SetDesc* sets, const int* img_set, int64_t n_img,
+                                   int max_bl, hipStream_t stream) {
+    if (n_img == 0 || max_bl == 0) return hipSuccess;
+    const int bpi = (max_bl + 255) / 256;
+    const int64_t blocks = n_img * bpi;
+    if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k_vis_compact_sets, dim3((unsigned)blocks), dim3(256), 0, stream, sets,
+                       img_set, bpi);
     return hipGetLastError();
 }
 

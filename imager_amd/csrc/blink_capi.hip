@@ -735,6 +735,7 @@ struct BatchGroup {
     int64_t n_img = 0;
     int chunk = 1;               // FFT groups: images per row/column launch pair
     size_t img_set_off = 0;      // blob offset of the image -> set table (2+ sets)
+    int compact_max_bl = 0;      // largest n_bl of its sets with compacted visibilities
 };
 
 struct BatchPlan {
@@ -858,8 +859,8 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     bp.n_max = n_max;
     const size_t partial_bytes = sizeof(int) * (size_t)n_units * parts * n_max;
 
-    // A register-path side imaged from a single set, with at most 64
-    // baselines per plan row (a sparse grid), gathers compacted visibilities
+    // A set of a register-path side with at most 64 baselines per plan row
+    // (a sparse grid) is gathered from compacted visibilities
     // (one float2 per image and baseline) instead of the records: its row
     // tiles hold few entries each, so the first-touch misses of the 32-byte
     // records are exposed in every tile (640 x 1024^2, 8256 baselines: row
@@ -871,13 +872,16 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     {
         const char* env = getenv("BLINK_COMPACT");
         const bool on = !(env && env[0] == '0');
-        for (const BatchGroup& g : bp.groups) {
-            if (!on || !g.reg || g.n > 2048 || g.sets.size() != 1) continue;
-            const int i = g.sets[0];
-            const int64_t n_bl = (int64_t)sets[i].n_ant * (sets[i].n_ant + 1) / 2;
-            if (n_bl > 64 * (int64_t)(g.n / 2 + 1)) continue;
-            cp_off[i] = cp_bytes;
-            cp_bytes += align16(sizeof(float2) * (size_t)g.n_img * (size_t)n_bl);
+        for (BatchGroup& g : bp.groups) {
+            if (!on || !g.reg || g.n > 2048) continue;
+            for (int i : g.sets) {
+                const int64_t n_bl = (int64_t)sets[i].n_ant * (sets[i].n_ant + 1) / 2;
+                if (n_bl > 64 * (int64_t)(g.n / 2 + 1)) continue;
+                cp_off[i] = cp_bytes;
+                cp_bytes += align16(sizeof(float2) * (size_t)sets[i].n_ch * sets[i].n_t *
+                                    (size_t)n_bl);
+                g.compact_max_bl = std::max(g.compact_max_bl, (int)n_bl);
+            }
         }
     }
     void *inter, *rs, *en, *is, *pa, *cl, *co, *cp = nullptr;
@@ -1159,7 +1163,11 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
     const int ns = (int)g.sets.size();
     FftPlan plan;
     make_fft_plan(n, plan);
-    if (first.compact) LAUNCH("vis_compact", s, launch_vis_compact(first, g.n_img, s));
+    if (g.compact_max_bl)
+        LAUNCH("vis_compact", s,
+               g.sets.size() == 1
+                   ? launch_vis_compact(first, g.n_img, s)
+                   : launch_vis_compact_sets(sd, is, g.n_img, g.compact_max_bl, s));
     for (int64_t gi0 = 0; gi0 < g.n_img; gi0 += g.chunk) {
         const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img - gi0);
         if (g.reg) {
@@ -1216,8 +1224,11 @@ static int autotune_batch(blink_ctx* ctx, const BatchPlan& bp, const BatchTables
     for (const BatchGroup& g : bp.groups) {
         if (!g.reg || ctx->tiles.count(g.n)) continue;
         // the timed row passes gather what the call will gather
-        const SetDesc& first = bp.host_desc(g);
-        if (first.compact) HIP_TRY(launch_vis_compact(first, g.n_img, s));
+        if (g.compact_max_bl)
+            HIP_TRY(g.sets.size() == 1
+                        ? launch_vis_compact(bp.host_desc(g), g.n_img, s)
+                        : launch_vis_compact_sets(t.group_sets(g), t.img_set(g), g.n_img,
+                                                  g.compact_max_bl, s));
         if ((rc = autotune_group(ctx, bp, g, t, s))) return rc;
     }
     HIP_TRY(hipStreamSynchronize(s));

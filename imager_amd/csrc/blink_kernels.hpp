@@ -224,5 +224,9 @@ hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int*
 // Fills sd.compact for all n_img images of the set (after the plan, which
 // writes the phase coefficients).
 hipError_t launch_vis_compact(const SetDesc& sd, int64_t n_img, hipStream_t stream);
+// The same for the n_img images of a group of several sets (device set table
+// and image -> set table; sets without a compact buffer are skipped).
+hipError_t launch_vis_compact_sets(const SetDesc* sets, const int* img_set, int64_t n_img,
+                                   int max_bl, hipStream_t stream);
 
 }  // namespace blink
