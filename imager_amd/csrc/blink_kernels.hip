@@ -443,7 +443,7 @@ hipError_t launch_plan_rows(const float* u_m, const float* v_m, const int* flags
 // Entry order inside a row is not fixed; the sums it feeds are FP32 sums in
 // any order either way.
 
-constexpr int kUnitThreads = 256;
+constexpr int kUnitThreads = 512;
 constexpr uint32_t kNoCell = 0xffffffffu;
 // baselines per thread in a plan part (parts hold at most kPlanPartMax)
 constexpr int kPlanPerThread = kPlanPartMax / kUnitThreads;

@@ -177,7 +177,7 @@ hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int
 // and rows N/2+1 .. N-1 of row_start are left empty.
 // Scratch: partial is [n_units][parts][n_max] ints, cells holds one packed
 // cell per (unit, baseline) from cell_off on.
-constexpr int kPlanPartMax = 2048;
+constexpr int kPlanPartMax = 4096;
 struct PlanUnit {
     int set, ch;
     int64_t cell_off;
