@@ -555,6 +555,42 @@ def test_image_batch_min_uv(ctx, n, dft):
     assert np.array_equal(_host(sums), c_ref.reshape(o.n_images, -1).sum(1).astype(np.float64))
 
 
+def test_image_batch_compacted_and_record_sets_in_one_group(ctx):
+    """One 128^2 group holding a dense set (100 antennas, 5050 baselines over
+    65 plan rows: its row tiles gather the records) and a sparse one (40
+    antennas: compacted visibilities), so that one row launch takes both
+    paths set by set: images and exact count sums against the oracle."""
+    import torch
+    from imager_amd import capi
+    obs, keep, sets = [], [], []
+    for k, n_ant in enumerate((100, 40)):
+        o = synth.small(n_ant=n_ant, n_ch=2, n_t=2, n_pixels=128, seed=950 + k,
+                        height_sigma_m=1.0, bl_flag_fraction=0.1)
+        d = _obs_dev(o)
+        imgs = torch.empty((o.n_t, o.n_ch, 128, 128), dtype=torch.complex64, device="cuda")
+        sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
+        sets.append(capi.make_set(d["vis"], n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"],
+                                  d["flags"], d["freq"], d["cable"], 3, o.delta_u, o.delta_v,
+                                  o.min_uv, 128, "XX", imgs, sums))
+        obs.append(o)
+        keep.append((d, imgs, sums))
+    for rep in range(2):   # capture, then replay
+        for _, imgs, _ in keep:
+            imgs.fill_(float("nan"))
+        _say(f"compacted + record sets, rep {rep}")
+        ctx.image_batch(sets)
+        for o, (_, imgs, sums) in zip(obs, keep):
+            ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True,
+                                    True, o.delta_u, o.delta_v, o.min_uv, 128)
+            assert oracle.peak_normalised_error(_host(imgs), ref)[0] < IMAGE_TOL, (o.n_ant, rep)
+            _, c_ref, _ = oracle.gridding(oracle.cable_corrections(
+                oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq), o.n_ant, o.cable,
+                o.freq), o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v, o.min_uv,
+                128, "XX")
+            assert np.array_equal(_host(sums),
+                                  c_ref.reshape(o.n_images, -1).sum(1).astype(np.float64))
+
+
 def test_image_batch_grouped_sets(ctx):
     """Several sets per grid side in one call (one launch pair per side):
     different arrays, channel and timestep counts, flags and polarisations,
