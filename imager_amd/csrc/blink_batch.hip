@@ -312,8 +312,11 @@ __device__ __forceinline__ void vis_compact_one(const SetDesc& sd, int64_t img, 
     const float2 x1 = g.pol >= 2 ? rec[p1] : make_float2(0.f, 0.f);
     float2 v = select_pol2(x0, x1, g.pol);
     if (g.corrections) {
+        // the coefficient the plan writes for the row tiles, formed here, so
+        // that the compaction need not wait for the plan
+        const double cf = phase_coefficient(g.corrections, sd.w_m, sd.cable_m, (unsigned)bl);
         const double nu = g.freq_hz[img % g.n_ch];
-        v = cmul(v, unit_phasor(((g.phase_coef[bl] * nu) / kVelLight)));
+        v = cmul(v, unit_phasor(((cf * nu) / kVelLight)));
     }
     if (!vis_ok(v)) v = make_float2(__builtin_nanf(""), __builtin_nanf(""));
     sd.compact[i] = v;

@@ -53,6 +53,8 @@ struct blink_ctx {
     // call's stream and the others beside it on aux_stream
     hipStream_t aux_stream = nullptr;
     hipEvent_t fork_event = nullptr, join_event = nullptr;
+    // the visibility compaction runs on aux_stream beside the plan
+    hipEvent_t compact_fork = nullptr, compact_join = nullptr;
     bool use_graphs = true;
     bool autotune = true;
     // tile choice of the batched kernels per grid side: {row tile, column block}
@@ -332,10 +334,16 @@ BLINK_API int blink_create(int device, blink_ctx** out) {
     if (hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->fork_event, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->compact_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->compact_join, hipEventDisableTiming) != hipSuccess) {
         if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
         if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
         if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
+        if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
+    if (ctx->compact_fork) (void)hipEventDestroy(ctx->compact_fork);
+    if (ctx->compact_join) (void)hipEventDestroy(ctx->compact_join);
+        if (ctx->compact_fork) (void)hipEventDestroy(ctx->compact_fork);
         delete ctx;
         return BLINK_ERR_HIP;
     }
@@ -1163,11 +1171,6 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
     const int ns = (int)g.sets.size();
     FftPlan plan;
     make_fft_plan(n, plan);
-    if (g.compact_max_bl)
-        LAUNCH("vis_compact", s,
-               g.sets.size() == 1
-                   ? launch_vis_compact(first, g.n_img, s)
-                   : launch_vis_compact_sets(sd, is, g.n_img, g.compact_max_bl, s));
     for (int64_t gi0 = 0; gi0 < g.n_img; gi0 += g.chunk) {
         const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img - gi0);
         if (g.reg) {
@@ -1196,10 +1199,28 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
 // launches overlap the large side's bandwidth-bound ones.
 static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const BatchPlan& bp,
                          const BatchTables& t, hipStream_t s) {
+    hipStream_t a = ctx->aux_stream;
+    // the visibility compaction reads only the visibilities and the sets' own
+    // inputs, so it runs on the aux stream beside the plan; the row passes
+    // (on s, or on the aux stream after it) come after both
+    bool compact = false;
+    for (const BatchGroup& g : bp.groups) compact |= g.compact_max_bl != 0;
+    if (compact) {
+        HIP_TRY(hipEventRecord(ctx->compact_fork, s));
+        HIP_TRY(hipStreamWaitEvent(a, ctx->compact_fork, 0));
+        for (const BatchGroup& g : bp.groups)
+            if (g.compact_max_bl)
+                LAUNCH("vis_compact", a,
+                       g.sets.size() == 1
+                           ? launch_vis_compact(bp.host_desc(g), g.n_img, a)
+                           : launch_vis_compact_sets(t.group_sets(g), t.img_set(g), g.n_img,
+                                                     g.compact_max_bl, a));
+        HIP_TRY(hipEventRecord(ctx->compact_join, a));
+    }
     int rc = enqueue_plans(ctx, bp, t, s);
     if (rc) return rc;
+    if (compact) HIP_TRY(hipStreamWaitEvent(s, ctx->compact_join, 0));
     if (bp.groups.size() == 1) return enqueue_group(ctx, sets, bp, t, bp.groups[0], bp.inter, s);
-    hipStream_t a = ctx->aux_stream;
     HIP_TRY(hipEventRecord(ctx->fork_event, s));
     HIP_TRY(hipStreamWaitEvent(a, ctx->fork_event, 0));
     for (size_t k = 0; k < bp.groups.size(); k++)

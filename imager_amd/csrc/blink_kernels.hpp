@@ -221,8 +221,8 @@ hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const
 hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
                            int n_sets, int n, int tile, int64_t gi0, int n_img,
                            const float* inter, const float* tw, hipStream_t stream);
-// Fills sd.compact for all n_img images of the set (after the plan, which
-// writes the phase coefficients).
+// Fills sd.compact for all n_img images of the set (from the visibilities and
+// the set's own inputs only: it may run beside the plan).
 hipError_t launch_vis_compact(const SetDesc& sd, int64_t n_img, hipStream_t stream);
 // The same for the n_img images of a group of several sets (device set table
 // and image -> set table; sets without a compact buffer are skipped).
