@@ -78,11 +78,12 @@ def make_workload(name: str):
 
 
 def algorithmic_bytes(kernel: str, n_bl: int, n_pix: int, n_img: int) -> float | None:
-    """Per-launch algorithmic HBM bytes of the batched-path kernels, from the
-    per-image figures: gridding = 8 n_bl (one pol) + 8 n_bl (u, v) + 12 N^2
-    (grid + count write), 2-D FFT = 16 N^2 (read grid + write image), shift and
-    normalisation fused = 0.  The row kernel grids and does the row FFTs, the
-    column kernel does the column FFTs with the fused shift+norm store."""
+    """Per-launch bytes of the unfused pipeline's stages (the figure the
+    gridding + FFT roofline target is quoted in): gridding = 8 n_bl (one pol)
+    + 8 n_bl (u, v) + 12 N^2 (grid + count write), 2-D FFT = 16 N^2 (read
+    grid + write image), shift and normalisation fused = 0.  The fused
+    kernels move fewer bytes than this (the grid never reaches HBM), so it is
+    reported as an effective bandwidth, not as a roofline fraction."""
     g = 16.0 * n_bl + 12.0 * n_pix * n_pix
     f = 16.0 * n_pix * n_pix
     if kernel in ("grid_rows_fft", "grid_tiles"):
@@ -91,6 +92,70 @@ def algorithmic_bytes(kernel: str, n_bl: int, n_pix: int, n_img: int) -> float |
         return (16.0 * n_bl + 8.0 * (n_pix // 2 + 1) * n_pix) * n_img
     if kernel == "fft_cols":
         return f * n_img
+    return None
+
+
+def plan_stats(o, transposed=True):
+    """Per channel of observation o: the plan entries in plan rows 0 .. N/2
+    (each gridded baseline's cell and conjugate cell that fall there) and the
+    number of those rows that hold any entry.  The FFT sides' plan is
+    transposed (plan rows = grid columns u), the DFT sides' is not."""
+    import oracle
+    n = o.n_pixels
+    cells = oracle.cells(o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v, o.min_uv, n)
+    k = 0 if transposed else 1
+    rows = np.concatenate([cells[:, :, k], cells[:, :, k + 2]], axis=1)
+    half = (rows >= 0) & (rows <= n // 2)
+    entries = half.sum(1).astype(np.float64)
+    nonempty = np.array([len(np.unique(rows[c][half[c]])) for c in range(rows.shape[0])],
+                        np.float64)
+    return entries, nonempty
+
+
+def compacted(n_bl: int, n_pix: int) -> bool:
+    """Whether the batched call gathers compacted visibilities for a set
+    (register sides up to 2048 with at most 64 baselines per plan row)."""
+    return (n_pix in (128, 256, 512, 1024, 2048) and n_bl <= 64 * (n_pix // 2 + 1)
+            and os.environ.get("BLINK_COMPACT", "1") != "0")
+
+
+def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft") -> float | None:
+    """HBM bytes a fused kernel must move per launch over n_img images of
+    observation o (images cycle over its channels), given the plan: what it
+    has to read and write, with every byte counted once.
+
+      grid_rows_fft  plan entries (4 B each) + the visibilities once (8 B per
+                     baseline compacted, else the 32 B records) + the
+                     transformed non-empty plan rows (8 N B each)
+      fft_cols       the non-empty plan rows (8 N B each) + the image (8 N^2 B)
+      vis_compact    the records (32 B) + the compacted value (8 B) per baseline
+      plan           per channel: u, v, flags (12 B) read, the packed cell (4 B)
+                     written and read back per baseline, the entries written
+      grid_half_rows plan entries + records + the half plane (8 (N/2+1) N B)
+      dft_rows       the half plane + the stage-1 product (8 N pad16(N/2+1) B)
+      dft_cols       the stage-1 product + the image"""
+    n, n_bl = o.n_pixels, o.n_bl
+    entries, nonempty = stats
+    ch = np.arange(n_img) % o.n_ch
+    e_sum = float(entries[ch].sum())
+    r_sum = float(nonempty[ch].sum())
+    h1 = n // 2 + 1
+    pad = -(-h1 // 16) * 16
+    if kernel == "grid_rows_fft":
+        vis_b = 8.0 if compacted(n_bl, n) else 32.0
+        return 4.0 * e_sum + vis_b * n_bl * n_img + 8.0 * n * r_sum
+    if kernel == "fft_cols":
+        return 8.0 * n * r_sum + 8.0 * n * n * n_img
+    if kernel == "vis_compact":
+        return 40.0 * n_bl * n_img
+    if kernel == "plan":
+        return 20.0 * n_bl * o.n_ch + 4.0 * float(entries.sum())
+    if kernel == "grid_half_rows":
+        return 4.0 * e_sum + 32.0 * n_bl * n_img + 8.0 * h1 * n * n_img
+    if kernel == "dft_rows":
+        return (8.0 * h1 * n + 8.0 * n * pad) * n_img
+    if kernel == "dft_cols":
+        return (8.0 * n * pad + 8.0 * n * n) * n_img
     return None
 
 
@@ -130,48 +195,19 @@ def read_traffic(kernel: str, workload: str):
         return None
 
 
-def cpu_baseline(obs, budget_s: float):
-    """Oracle CPU pipeline (reference FP32 semantics, OpenMP over images and FFT
-    rows/columns) on the first images of the workload."""
+def cpu_baseline(obs, budget_s: float, max_images: int = 128, reps: int = 5):
+    """The reference CPU pipeline (the oracle's restatement, reference FP32
+    semantics, OpenMP over images as src/pacer_imager.cpp:158,263) timed
+    stage by stage on this host's cores, on the first images of the workload:
+    corrections (geometric + cable), gridding, the 2-D FFT, and normalisation
+    + fft_shift, all in C; the FFT stage also through torch's CPU FFT (MKL)
+    with as many threads.  Each stage: one warm-up, then the median of `reps`
+    runs.  The value charges the faster FFT, as an FFTW build of the
+    reference would run (FFTW itself is not installed)."""
+    import torch
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-
-    def run(n_t, n_ch):
-        t0 = time.perf_counter()
-        oracle.run_f32(obs.vis[:n_t, :n_ch], obs.n_ant, obs.u, obs.v, obs.w, obs.flags,
-                       obs.freq[:n_ch], obs.cable, True, True, obs.delta_u, obs.delta_v,
-                       obs.min_uv, obs.n_pixels, "XX", threads)
-        return time.perf_counter() - t0
-
-    per_img = run(1, 1)
-    want = int(max(1, budget_s / max(per_img, 1e-4)))
-    n_ch = min(obs.n_ch, want)
-    n_t = min(obs.n_t, max(1, want // n_ch))
-    dt = run(n_t, n_ch)
-    k = n_t * n_ch
-    # a workload smaller than the budget is imaged again, so that the figure
-    # rests on at least two thirds of the budget of CPU work
-    reps = 1
-    while dt < 0.67 * budget_s and reps < 8:
-        dt += run(n_t, n_ch)
-        reps += 1
-    return {"value": reps * k / dt, "unit": "images/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} pass(es) over {n_t} timestep(s) x {n_ch} channel(s) = {k} images "
-                      f"of the workload ({obs.n_ant} antennas, {obs.n_pixels}^2): geometric+cable "
-                      f"corrections, gridding, FFT, shift+norm in oracle/blink_oracle.c with "
-                      f"{threads} OpenMP threads, {dt:.1f} s"}
-
-
-def cpu_stages(obs, max_images: int = 16, reps: int = 5):
-    """The oracle's stages timed one by one on the first images of the workload
-    (one warm-up, then the median of `reps` runs each), in images/s:
-    corrections (geometric + cable), gridding, FFT, and shift+norm (each image
-    scaled by (float)(1 / sum counts), then the oracle's fft_shift)."""
-    import oracle
-    n_ch = min(obs.n_ch, max_images)
-    n_t = min(obs.n_t, max(1, max_images // n_ch))
-    k = n_t * n_ch
-    vis, freq = obs.vis[:n_t, :n_ch], obs.freq[:n_ch]
+    torch.set_num_threads(threads)
 
     def median_s(fn):
         fn()
@@ -182,38 +218,50 @@ def cpu_stages(obs, max_images: int = 16, reps: int = 5):
             ts.append(time.perf_counter() - t0)
         return float(np.median(ts))
 
-    def corrections():
-        v = oracle.geometric_corrections(vis, obs.n_ant, obs.w, freq)
-        return oracle.cable_corrections(v, obs.n_ant, obs.cable, freq)
+    def stages(n_t, n_ch):
+        vis, freq = obs.vis[:n_t, :n_ch], obs.freq[:n_ch]
+        k = n_t * n_ch
 
-    vis_c = corrections()
-    grid = lambda: oracle.gridding(vis_c, obs.n_ant, obs.u, obs.v, obs.flags, freq, obs.delta_u,
-                                   obs.delta_v, obs.min_uv, obs.n_pixels, "XX")
-    g, c, _ = grid()
-    t_corr = median_s(corrections)
-    t_grid = median_s(grid)
-    img = oracle.fft2d(g, +1)
-    t_fft = median_s(lambda: oracle.fft2d(g, +1))
-    flat = img.reshape(k, obs.n_pixels, obs.n_pixels)
-    scale = (1.0 / c.reshape(k, -1).sum(1, dtype=np.float64)).astype(np.float32)
-    t_shift = median_s(lambda: [oracle.fft_shift(flat[i] * scale[i]) for i in range(k)])
-    return {"images": k, "reps": reps,
-            "corrections_images_per_s": k / t_corr, "gridding_images_per_s": k / t_grid,
-            "fft_images_per_s": k / t_fft, "shift_norm_images_per_s": k / t_shift}
+        def corrections():
+            v = oracle.geometric_corrections(vis, obs.n_ant, obs.w, freq)
+            return oracle.cable_corrections(v, obs.n_ant, obs.cable, freq)
 
+        vis_c = corrections()
+        grid = lambda: oracle.gridding(vis_c, obs.n_ant, obs.u, obs.v, obs.flags, freq,
+                                       obs.delta_u, obs.delta_v, obs.min_uv, obs.n_pixels, "XX")
+        g, c, _ = grid()
+        t = {"corrections": median_s(corrections), "gridding": median_s(grid),
+             "fft_c": median_s(lambda: oracle.fft2d(g, +1))}
+        gt = torch.from_numpy(g)
+        # unnormalised backward transform, as FFTW_BACKWARD
+        t["fft_mkl"] = median_s(lambda: torch.fft.ifft2(gt, norm="forward"))
+        img = oracle.fft2d(g, +1)
+        t["shift_norm"] = median_s(lambda: oracle.norm_shift(img, c))
+        return k, t
 
-def fft_mkl_rate(n_pix: int, seconds: float = 3.0):
-    """Batched 2-D FFT images/s on this host's CPU through torch (MKL), an
-    FFTW-class bound for the FFT stage of the CPU baseline."""
-    import torch
-    x = torch.randn(8, n_pix, n_pix, dtype=torch.complex64)
-    torch.fft.ifft2(x)
-    n = 0
+    # one image first, to size the sample to the budget
+    _, t1 = stages(1, 1)
+    per_img = (1 + reps) * sum(t1.values())
+    want = int(min(max_images, max(1, budget_s / max(per_img, 1e-4))))
+    n_ch = min(obs.n_ch, want)
+    n_t = min(obs.n_t, max(1, want // n_ch))
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        torch.fft.ifft2(x)
-        n += x.shape[0]
-    return n / (time.perf_counter() - t0), torch.get_num_threads()
+    k, t = stages(n_t, n_ch)
+    wall = time.perf_counter() - t0
+    fft_backend = "MKL (torch CPU FFT)" if t["fft_mkl"] < t["fft_c"] else "oracle C FFT"
+    total = t["corrections"] + t["gridding"] + min(t["fft_c"], t["fft_mkl"]) + t["shift_norm"]
+    rate = {name: k / sec for name, sec in t.items()}
+    return {"value": k / total, "unit": "images/s", "cores": threads, "kind": "port",
+            "fft_backend": fft_backend,
+            "sample": f"{n_t} timestep(s) x {n_ch} channel(s) = {k} images of the workload "
+                      f"({obs.n_ant} antennas, {obs.n_pixels}^2), every stage 1 + {reps} runs "
+                      f"on {threads} threads ({wall:.1f} s): corrections, gridding, norm+shift "
+                      f"in oracle/blink_oracle.c; FFT charged with {fft_backend}",
+            "stages_images_per_s": {
+                "corrections": rate["corrections"], "gridding": rate["gridding"],
+                "fft_oracle_c": rate["fft_c"], "fft_mkl": rate["fft_mkl"],
+                "shift_norm": rate["shift_norm"]},
+            "value_with_oracle_c_fft": k / (total - min(t["fft_c"], t["fft_mkl"]) + t["fft_c"])}
 
 
 def memory_reference(mib: int = 2048):
@@ -268,10 +316,10 @@ def build_job(name, wl, transform):
             keep.append((d, img))
             sets.append(st)
             per_size.setdefault(o.n_pixels, []).append(st)
-        seen = set()
-        for i, o in enumerate(wl):     # one parity image per grid side
-            if o.n_pixels not in seen:
-                seen.add(o.n_pixels)
+        seen = {}
+        for i, o in enumerate(wl):     # two parity images per grid side
+            if seen.get(o.n_pixels, 0) < 2:
+                seen[o.n_pixels] = seen.get(o.n_pixels, 0) + 1
                 parity.append((o, 0, 0, keep[i][1][0, 0], f"image{i}_{o.n_ant}ant_{o.n_pixels}"))
         counts = {n: len(v) for n, v in sorted(per_size.items())}
         return {"obs": wl[0], "n_pixels": None, "calls": [sets], "images_per_step": len(wl),
@@ -299,8 +347,9 @@ def build_job(name, wl, transform):
                                         o.delta_v, o.min_uv, N, "XX", images,
                                         transform=transform)])
         t_last = o.n_t - 1
-        parity = [(o, t_last, 0, images[0, 0], f"t{t_last}_ch0"),
-                  (o, t_last, o.n_ch - 1, images[0, o.n_ch - 1], f"t{t_last}_ch{o.n_ch - 1}")]
+        # the buffer holds the last timestep's images after a step
+        parity = [(o, t_last, int(c), images[0, int(c)], f"t{t_last}_ch{int(c)}")
+                  for c in np.linspace(0, o.n_ch - 1, 8).round().astype(int)]
         return {"obs": o, "n_pixels": N, "calls": calls, "images_per_step": o.n_images,
                 "pipeline_obs": [o], "pipeline_repeat": o.n_images, "parity": parity,
                 "plain_ok": True, "keep": (d, images, sums), "sizes": [N], "dev": d,
@@ -315,9 +364,10 @@ def build_job(name, wl, transform):
     st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
                        d["freq"], d["cable"], corr, o.delta_u, o.delta_v, o.min_uv, N, "XX",
                        images, sums, transform=transform)
-    parity = [(o, 0, 0, images[0, 0], "t0_ch0"),
-              (o, o.n_t - 1, o.n_ch - 1, images[o.n_t - 1, o.n_ch - 1],
-               f"t{o.n_t - 1}_ch{o.n_ch - 1}")]
+    parity = []
+    for i in np.unique(np.linspace(0, o.n_images - 1, 8).round().astype(int)):
+        t, c = divmod(int(i), o.n_ch)
+        parity.append((o, t, c, images[t, c], f"t{t}_ch{c}"))
     return {"obs": o, "n_pixels": N, "calls": [[st]], "images_per_step": o.n_images,
             "pipeline_obs": [o], "pipeline_repeat": o.n_images, "parity": parity,
             "plain_ok": True, "keep": (d, images, sums), "sizes": [N], "dev": d,
@@ -424,6 +474,9 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = job["images_per_step"] * args.steps / elapsed
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} images/s")
+    # the parity sample is read back from what the last timed replay wrote
+    parity_images = None if args.no_parity else [
+        (o, t, ch, view.cpu().numpy(), label) for (o, t, ch, view, label) in job["parity"]]
     if args.profile_only:
         print(json.dumps({"metric": METRIC, "value": value, "unit": "images/s",
                           "ms_per_step": ms_per_step, "profile_only": True}), flush=True)
@@ -448,18 +501,18 @@ def main():
     log("kernels: " + ", ".join(f"{k} {v['ms_per_step']:.3f} ms/step ({v['avg_ms']*1e3:.1f} us x "
                                f"{v['launches_per_step']:.0f})" for k, v in kernels.items()))
     roof = None
-    pipeline = None
+    stats = None
     if N is not None:
+        stats = plan_stats(obs, transposed=args.transform == "fft")
         n_launch = max(prof.get("grid_rows_fft", (0, 0))[0], prof.get("grid_tiles", (0, 0))[0],
                        prof.get("grid_half_rows", (0, 0))[0]) // args.steps
         chunk = -(-job["images_per_step"] // max(n_launch, 1))
         imgs_per_launch = chunk if dom in ("grid_rows_fft", "fft_cols", "grid_tiles",
                                            "grid_half_rows", "dft_rows", "dft_cols") \
             else job["images_per_step"]
-        alg = algorithmic_bytes(dom, obs.n_bl, N, imgs_per_launch)
+        avg_s = kernels[dom]["avg_ms"] * 1e-3
         if dom in ("dft_rows", "dft_cols"):
             useful, executed = dft_flops(dom, N, imgs_per_launch)
-            avg_s = kernels[dom]["avg_ms"] * 1e-3
             ach = executed / avg_s / 1e12
             prec = "fp16" if args.transform == "dft-fp16" else "bf16"
             roof = {"bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s",
@@ -469,20 +522,25 @@ def main():
                     "useful_tflops": useful / avg_s / 1e12,
                     "note": f"dense {prec} MFMA peak; executed = 3 split terms x 4 (dft_rows) "
                             "or 2 (dft_cols) real products over padded tiles"}
-        elif alg is not None:
-            avg_s = kernels[dom]["avg_ms"] * 1e-3
-            ach = alg / avg_s / 1e9
-            per_image = read_traffic(dom, args.workload)
-            traffic = per_image * imgs_per_launch if per_image else None
-            roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                    "images_per_launch": imgs_per_launch, "alg_bytes_per_launch": alg}
-            if traffic:
-                # the bytes the kernel actually moves (PMC) over the same launch time: the
-                # physical bandwidth, below the algorithmic figure when the fused kernels
-                # skip empty rows and the Hermitian half
-                roof["traffic_gbs"] = traffic / avg_s / 1e9
-                roof["traffic_frac"] = traffic / avg_s / 1e9 / HBM_PEAK_GBS
+        else:
+            phys = physical_bytes(dom, obs, stats, imgs_per_launch, args.transform)
+            if phys is not None:
+                ach = phys / avg_s / 1e9
+                per_image = read_traffic(dom, args.workload)
+                traffic = per_image * imgs_per_launch if per_image else None
+                roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                        "images_per_launch": imgs_per_launch, "bytes_per_launch": phys,
+                        "bytes_model": "what the fused kernel must move (plan rows, image)"}
+                alg = algorithmic_bytes(dom, obs.n_bl, N, imgs_per_launch)
+                if alg is not None:
+                    # the unfused stages' byte figure over the same time
+                    roof["effective_gbs_unfused_figure"] = alg / avg_s / 1e9
+                if traffic:
+                    # the bytes the PMC counters saw for this launch, over the same time
+                    roof["traffic_gbs"] = traffic / avg_s / 1e9
+                    roof["traffic_frac"] = traffic / avg_s / 1e9 / HBM_PEAK_GBS
+                    roof["traffic_over_model"] = traffic / phys
     # the DFT-as-GEMM stages against the dense MFMA peak, whichever kernel dominates
     dft_mfma = None
     if N is not None and any(k in kernels for k in ("dft_rows", "dft_cols")):
@@ -498,11 +556,22 @@ def main():
                            "useful_tflops": useful / avg_s / 1e12,
                            "frac": executed / avg_s / 1e12 / 2500.0, "peak": 2500.0,
                            "images_per_launch": n_img, "avg_us": avg_s * 1e6}
-    pipe_bytes = sum(16.0 * o.n_bl + 28.0 * o.n_pixels ** 2 for o in job["pipeline_obs"])
-    pipe_bytes *= job["pipeline_repeat"]
-    pipeline = {"alg_bytes_per_step": pipe_bytes,
-                "achieved_gbs": pipe_bytes / (ms_per_step * 1e-3) / 1e9,
-                "frac": pipe_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    # whole step: every kernel's bytes (the same model) over the step time, and
+    # the unfused gridding + FFT figure as an effective rate
+    eff_bytes = sum(16.0 * o.n_bl + 28.0 * o.n_pixels ** 2 for o in job["pipeline_obs"])
+    eff_bytes *= job["pipeline_repeat"]
+    pipeline = {"effective_gbs_unfused_figure": eff_bytes / (ms_per_step * 1e-3) / 1e9}
+    if stats is not None:
+        step_bytes = 0.0
+        for k, v in kernels.items():
+            if k == "plan":
+                b = v["launches_per_step"] * physical_bytes(k, obs, stats, obs.n_ch)
+            else:
+                b = physical_bytes(k, obs, stats, job["images_per_step"], args.transform)
+            step_bytes += b or 0.0
+        pipeline.update({"bytes_per_step": step_bytes,
+                         "achieved_gbs": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                         "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS})
 
     # per-size throughput of a mixed batch: one call per grid side
     per_size = None
@@ -557,12 +626,11 @@ def main():
         log(f"plain path: {plain['value']:.1f} images/s; batched is {plain['batched_speedup']:.1f}x")
         del vis_s, grid, cnt, img
 
-    # parity of sample images (left in the outputs by the last call) against the oracle
+    # parity of the sample images of the last timed replay against the oracle
     parity = None
-    if not args.no_parity:
+    if parity_images is not None:
         e32s, e64s, names = [], [], []
-        for (o, t, ch, view, label) in job["parity"]:
-            got = view.cpu().numpy()
+        for (o, t, ch, got, label) in parity_images:
             v1 = o.vis[t:t + 1, ch:ch + 1]
             a = (o.n_ant, o.u, o.v, o.w, o.flags, o.freq[ch:ch + 1], o.cable, True,
                  True, o.delta_u, o.delta_v, o.min_uv, o.n_pixels, "XX")
@@ -571,7 +639,7 @@ def main():
             e32s.append(oracle.peak_normalised_error(got, r32[0, 0]))
             e64s.append(oracle.peak_normalised_error(got, r64[0, 0]))
             names.append(label)
-        parity = {"images": names,
+        parity = {"images": names, "from": "the last timed graph replay",
                   "max_err_vs_fp32_oracle": max(e[0] for e in e32s),
                   "max_err_vs_fp64_oracle": max(e[0] for e in e64s),
                   "rms_err_vs_fp64_oracle": max(e[1] for e in e64s),
@@ -581,13 +649,6 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(obs, args.cpu_seconds)
-        cpu["stages"] = cpu_stages(obs)
-        try:
-            rate, thr = fft_mkl_rate(obs.n_pixels)
-            cpu["fft_stage_mkl_images_per_s"] = rate
-            cpu["fft_stage_mkl_threads"] = thr
-        except Exception as e:  # torch CPU FFT is optional context
-            cpu["fft_stage_mkl_error"] = str(e)
         log(f"cpu baseline: {cpu}")
 
     config = dict(job["config"])

@@ -93,6 +93,7 @@ def lib():
     L.oracle_fft_shift_f64.argtypes = [_f64p, ctypes.c_size_t, ctypes.c_size_t]
     L.oracle_image_f32.argtypes = [_f32p, _f32p, _c_int, _c_int, _f32p, _c_int]
     L.oracle_image_f64.argtypes = [_f64p, _f64p, _c_int, _c_int, _f64p, _c_int]
+    L.oracle_norm_shift_f32.argtypes = [_f32p, _f32p, _c_int, _c_int, _c_int]
     L.oracle_run_f32.argtypes = [_f32p, _c_int, _c_int, _c_int, _f32p, _f32p, _f32p, _i32p,
                                  _f64p, _f64p, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_int,
                                  _c_int, _f32p, _f32p, _f32p, _c_int]
@@ -271,6 +272,16 @@ def image(grids: np.ndarray, counts: np.ndarray, n_threads: int = 0) -> np.ndarr
         out = np.zeros_like(g)
         lib().oracle_image_f64(_as_float_view(g), c, N, n_img, _as_float_view(out), n_threads)
     return out
+
+
+def norm_shift(images: np.ndarray, counts: np.ndarray, n_threads: int = 0) -> np.ndarray:
+    """In place on transformed complex64 images [..., N, N]: multiply each by
+    (float)(1/sum(counts)) and fft_shift (the steps of image() after the FFT)."""
+    assert images.dtype == np.complex64 and images.flags.c_contiguous
+    N = images.shape[-1]
+    n_img = int(np.prod(images.shape[:-2])) if images.ndim > 2 else 1
+    lib().oracle_norm_shift_f32(_as_float_view(images), _f32(counts), N, n_img, n_threads)
+    return images
 
 
 def run_f32(vis, n_ant, u_m, v_m, w_m, flags, freq_hz, cable_m, apply_geom, apply_cable,

@@ -601,6 +601,14 @@ void oracle_image_f32(const float* grids, const float* counts, int n_pixels, int
                       float* images, int n_threads) {
     set_threads(n_threads);
     oracle_fft2d_f32(grids, images, n_pixels, n_images, +1);
+    oracle_norm_shift_f32(images, counts, n_pixels, n_images, n_threads);
+}
+
+/* The part of src/pacer_imager.cpp:158-181 after the FFT, on transformed
+ * images in place: sum(counts) in double, multiply by (float)(1/sum), shift. */
+void oracle_norm_shift_f32(float* images, const float* counts, int n_pixels, int n_images,
+                           int n_threads) {
+    set_threads(n_threads);
     const int64_t gs = (int64_t)n_pixels * n_pixels;
     #pragma omp parallel for schedule(static)
     for (int img = 0; img < n_images; img++) {

@@ -152,6 +152,10 @@ void oracle_image_f32(const float* grids, const float* counts, int n_pixels,
                       int n_images, float* images, int n_threads);
 void oracle_image_f64(const double* grids, const double* counts, int n_pixels,
                       int n_images, double* images, int n_threads);
+/* The normalisation and shift of oracle_image_f32 alone, in place on images
+ * the FFT has already transformed (src/pacer_imager.cpp:158-181). */
+void oracle_norm_shift_f32(float* images, const float* counts, int n_pixels, int n_images,
+                           int n_threads);
 
 /* Reference CPU pipeline end to end (CPacerImager::run with the corrections
  * switched by the flags), FP32 semantics, OpenMP over images as in

@@ -395,7 +395,12 @@ def _batch(ctx, o, corrections=3, pol="XX"):
                                          (1024, 128, "XX"), (2048, 64, "YY"), (4096, 32, "XX"),
                                          (90, 24, "XX"), (150, 32, "YY"), (128, 100, "I")])
 def test_image_batch_matches_reference_pipeline(ctx, n, n_ant, pol):
-    o = synth.small(n_ant=n_ant, n_ch=3, n_t=2, n_pixels=n, seed=n + n_ant, height_sigma_m=2.0)
+    if (n, n_ant) == (1024, 128):
+        # the 128-tile Gaussian array of the 1024^2 benchmark (1.5 km)
+        o = synth.mwa_array(n_ch=3, n_t=2, n_pixels=1024, seed=1002, height_sigma_m=2.0)
+    else:
+        o = synth.small(n_ant=n_ant, n_ch=3, n_t=2, n_pixels=n, seed=n + n_ant,
+                        height_sigma_m=2.0)
     _say("batch", n, n_ant, pol)
     ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
                             o.delta_u, o.delta_v, o.min_uv, n, pol)
@@ -507,6 +512,43 @@ def test_image_batch_graph_replays_are_identical(ctx):
         got = _host(imgs)
         assert not np.isnan(got).any()
         assert oracle.peak_normalised_error(got, first)[0] < 1e-6, k
+
+
+@pytest.mark.parametrize("n,dft", [(1024, False), (180, False), (128, True)])
+def test_image_batch_new_buffers_update_the_graph(ctx, n, dft):
+    """A call repeated with other vis / images / count_sums buffers (the
+    fields that do not key the graph cache) updates the captured graph: every
+    call's outputs land in that call's buffers and match the oracle, and
+    going back to the first buffers gives the first images again."""
+    import torch
+    from imager_amd import capi
+    o = synth.small(n_ant=48, n_ch=2, n_t=2, n_pixels=n, seed=300 + n, height_sigma_m=1.0)
+    o2 = synth.small(n_ant=48, n_ch=2, n_t=2, n_pixels=n, seed=300 + n, height_sigma_m=1.0,
+                     n_src=5)
+    assert np.array_equal(o.u, o2.u) and not np.array_equal(o.vis, o2.vis)
+    d = _obs_dev(o)
+    vis2 = _dev(o2.vis)
+    tr = capi.TRANSFORM_DFT_BF16X3 if dft else capi.TRANSFORM_FFT
+    bufs = [(d["vis"], torch.empty((o.n_t, o.n_ch, n, n), dtype=torch.complex64, device="cuda"),
+             torch.empty(o.n_images, dtype=torch.float64, device="cuda")) for _ in range(2)]
+    bufs[1] = (vis2, bufs[1][1], bufs[1][2])
+    refs = []
+    for obs in (o, o2):
+        r, _ = oracle.run_f32(obs.vis, obs.n_ant, obs.u, obs.v, obs.w, obs.flags, obs.freq,
+                              obs.cable, True, True, obs.delta_u, obs.delta_v, obs.min_uv, n)
+        refs.append(r)
+    for k in (0, 1, 0, 1):
+        vis, imgs, sums = bufs[k]
+        imgs.fill_(float("nan"))
+        st = capi.make_set(vis, o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                           d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, n, "XX",
+                           imgs, sums, transform=tr)
+        _say(f"buffers {k}, N={n}, dft={dft}")
+        ctx.image_batch([st])
+        err = oracle.peak_normalised_error(_host(imgs), refs[k])[0]
+        assert err < IMAGE_TOL, (k, err)
+    # the other buffer pair still holds its own images (not overwritten)
+    assert oracle.peak_normalised_error(_host(bufs[0][1]), refs[0])[0] < IMAGE_TOL
 
 
 def test_image_batch_two_sets(ctx):

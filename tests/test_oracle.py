@@ -366,6 +366,48 @@ def test_frequency_cotter():
     assert abs((g0 - f0) - 20e3) < 1e-3
 
 
+def test_zenith_uvw_hand_kat():
+    """Three antennas with answers worked out by hand.  Baselines are
+    bl = a1 (a1 + 1) / 2 + a2, a2 <= a1: 0 = (0,0), 1 = (1,0), 2 = (1,1),
+    3 = (2,0), 4 = (2,1), 5 = (2,2); u = -(x[a1] - x[a2]) formed in double
+    and stored as FP32 (src/antenna_positions.cpp:81-89), v, w likewise."""
+    x = np.array([1.5, -2.25, 0.1])
+    y = np.array([16777217.0, 16777216.0, 0.0])   # 2^24 + 1 has no FP32 value
+    z = np.array([0.0, 0.5, -0.25])
+    u, v, w = oracle.zenith_uvw(x, y, z)
+    assert u.dtype == np.float32 and len(u) == 6
+    for a in (u, v, w):                       # autos
+        assert a[0] == 0 and a[2] == 0 and a[5] == 0
+    # -(x1 - x0) = 3.75 exactly; a swapped sign or operand order gives -3.75
+    assert u[1] == 3.75
+    # -(0.1 - 1.5) = 1.4 in double, stored as 11744051 / 2^23
+    assert float(u[3]) == 11744051 / 2 ** 23
+    # -(0.1 + 2.25) = -2.35, stored as -4928307 / 2^21
+    assert float(u[4]) == -4928307 / 2 ** 21
+    # the difference is formed in double: FP32 operands would both be 2^24
+    # and give 0
+    assert v[1] == 1.0
+    # 2^24 + 1 rounds to even in FP32
+    assert v[3] == 16777216.0 and v[4] == 16777216.0
+    assert w[1] == -0.5 and w[3] == 0.25 and w[4] == 0.75
+
+
+def test_delta_uv_closed_forms():
+    """delta_u = oversampling * u_max / N with the oversampling factor a float
+    member promoted to double (src/pacer_imager.cpp:497-499, src/pacer_imager.h:65)."""
+    du, dv = oracle.delta_uv(2.0, 1024.0, 512.0, 1024)
+    assert du == 2.0 and dv == 1.0            # u_max feeds delta_u, v_max delta_v
+    du, dv = oracle.delta_uv(2.0, 90.0, 45.0, 180)
+    assert du == 1.0 and dv == 0.5
+    # 1.9 is not an FP32 value: the factor enters as 15938355 / 2^23
+    du, dv = oracle.delta_uv(1.9, 1024.0, 2048.0, 1024)
+    assert du == 15938355 / 2 ** 23 and dv == 2 * 15938355 / 2 ** 23
+    assert du != 1.9
+    # every baseline flagged: u_max = v_max = -inf
+    du, dv = oracle.delta_uv(2.0, -np.inf, -np.inf, 180)
+    assert du == -np.inf and dv == -np.inf
+
+
 # ------------------------------------------ convolutional (Kaiser-Bessel) mode
 
 def _kb_1d(table):
