@@ -28,6 +28,7 @@ BLINK_ERR_UNSUPPORTED = 4
 
 POL = {"XX": 0, "YY": 1, "I": 2, "V": 3}
 WEIGHTING_NATURAL, WEIGHTING_UNIFORM = 0, 1
+WEIGHTING = {"N": WEIGHTING_NATURAL, "": WEIGHTING_NATURAL, "U": WEIGHTING_UNIFORM}
 CORR_GEOMETRIC, CORR_CABLE = 1, 2
 FFT_FORWARD, FFT_BACKWARD = -1, 1
 GRID_TILED, GRID_ATOMIC = 0, 1
@@ -74,7 +75,8 @@ class ImageSet(ctypes.Structure):
         ("delta_u", ctypes.c_double), ("delta_v", ctypes.c_double), ("min_uv", ctypes.c_double),
         ("n_pixels", ctypes.c_int), ("pol", ctypes.c_int),
         ("images", ctypes.c_void_p), ("count_sums", ctypes.c_void_p),
-        ("transform", ctypes.c_int),
+        ("transform", ctypes.c_int), ("weighting", ctypes.c_int),
+        ("average", ctypes.c_void_p),
     ]
 
 
@@ -318,9 +320,12 @@ class Context:
 
 def make_set(vis, n_ant, n_ch, n_t, u_m, v_m, w_m, bl_flags, freq_hz, cable_m, corrections,
              delta_u, delta_v, min_uv, n_pixels, pol, images, count_sums=None,
-             transform=TRANSFORM_FFT) -> ImageSet:
+             transform=TRANSFORM_FFT, weighting=WEIGHTING_NATURAL, average=None) -> ImageSet:
+    """One struct blink_image_set; `images` may be None when `average` is given."""
     return ImageSet(_ptr(vis), n_ant, n_ch, n_t, _ptr(u_m), _ptr(v_m), _ptr(w_m), _ptr(bl_flags),
                     _ptr(freq_hz), _ptr(cable_m), int(corrections), float(delta_u),
                     float(delta_v), float(min_uv), int(n_pixels),
                     POL[pol] if isinstance(pol, str) else int(pol), _ptr(images),
-                    _ptr(count_sums), int(transform))
+                    _ptr(count_sums), int(transform),
+                    WEIGHTING[weighting] if isinstance(weighting, str) else int(weighting),
+                    _ptr(average))

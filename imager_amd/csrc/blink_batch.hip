@@ -71,10 +71,11 @@ struct BatchShape {
 };
 
 // LDS of one workgroup: RB tile rows (row-tile rows, or column teams) and a
-// copy of the twiddle table.
-template <int N, int RB>
+// copy of the twiddle table; with uniform weighting, a count per tile cell.
+template <int N, int RB, bool kUniform = false>
 constexpr size_t batch_lds_bytes() {
-    return sizeof(float2) * ((size_t)RB * tile_stride(N, RB) + N);
+    return sizeof(float2) * ((size_t)RB * tile_stride(N, RB) + N) +
+           (kUniform ? sizeof(unsigned) * (size_t)RB * N : 0);
 }
 
 // Threads of a row-tile workgroup: the RB teams of L lanes that run the row
@@ -86,7 +87,11 @@ constexpr int rows_reg_threads() {
     return RB * BatchLanes<N>::L > 256 ? RB * BatchLanes<N>::L : 256;
 }
 
-template <int N, int RB_, bool kMulti>
+// kUniform: the "U" weighting of src/pacer_imager.cpp:421-428 -- each cell is
+// divided by its count before the transform, so an empty cell becomes
+// 0 / 0 = NaN, as in the reference (the transform then spreads it over the
+// row, and the second pass over the image).
+template <int N, int RB_, bool kMulti, bool kUniform>
 __global__ void __launch_bounds__((rows_reg_threads<N, RB_>()))
 k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
                 int64_t gi0, float2* __restrict__ inter, const float2* __restrict__ tw) {
@@ -158,6 +163,9 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     if (tid <= RB) rs[tid] = rsv;
     if (tid < RB) rrow[tid] = rrv;
     for (int i = tid; i < RB * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
+    unsigned* cnt = (unsigned*)(twl + N);  // [RB][N] cell counts (kUniform)
+    if constexpr (kUniform)
+        for (int i = tid; i < RB * N; i += NT) cnt[i] = 0u;
     __syncthreads();
     auto sink = [&](int e, int u_idx, float2 v) {
         // tile rows are consecutive non-empty plan rows (rows past the last one
@@ -169,6 +177,7 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         float2* c = &tile[r * TS + pad_index(u_idx)];
         atomicAdd(&c->x, v.x);
         atomicAdd(&c->y, v.y);
+        if constexpr (kUniform) atomicAdd(&cnt[r * N + u_idx], 1u);
         return ((edge >> r) & 1u) ? 1 : 2;
     };
     int n_ok = cimg ? gb.consume_compact(sink) : gb.consume(g, nu, sink);
@@ -186,6 +195,14 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     const int tot = block_sum_int(n_ok, red);
     if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();
+    if constexpr (kUniform) {
+        for (int i = tid; i < RB * N; i += NT) {
+            float2* c = &tile[(i / N) * TS + pad_index(i % N)];
+            const float k = (float)cnt[i];
+            *c = make_float2(c->x / k, c->y / k);
+        }
+        __syncthreads();
+    }
 
     // each team transforms its row and stores it from registers, row-major
     // (consecutive lanes write consecutive elements: whole 128-byte lines);
@@ -228,11 +245,15 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     const int li = blockIdx.x / n_cb;
     const int64_t gi = gi0 + li;
     const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    if (!sd.images) return;  // a set imaged for its average only (uniform per workgroup)
     const int img = (int)(gi - sd.img_base);  // image within its set
     const int tid = threadIdx.x;
     const uint32_t count = sd.isums[img];
     if (cb == 0 && tid == 0 && sd.count_sums) sd.count_sums[img] = (double)count;
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (N + 1);
+    // plan rows without entries hold zeros, or under uniform weighting empty
+    // cells, 0 / 0 = NaN
+    const float empty = sd.uniform ? __builtin_nanf("") : 0.f;
     float2* twl = tile + TEAMS * TS;  // twiddle table copy [N]
     for (int i = tid; i < N; i += NT) twl[i] = tw[i];
     for (int i = tid; i <= H; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
@@ -250,7 +271,7 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     for (int k = 0; k < ITER; k++) {
         const int i = tid + k * NT;
         const int vr = i / TEAMS, t = i % TEAMS;
-        p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        p[k] = make_float4(empty, empty, empty, empty);
         if (i < STAGE && nonempty[vr])
             p[k] = *(const float4*)&inter[inter_index(N, li, vr, c0 + 2 * t)];
     }
@@ -361,6 +382,67 @@ hipError_t launch_vis_compact_sets(const SetDesc* sets, const int* img_set, int6
     return hipGetLastError();
 }
 
+// Mean of a set's images without forming them.  The second pass is linear in
+// the row-pass buffer, so the mean of the normalised images is the second
+// pass of A = sum_i s_i R_i with the scale 1 / n_img, where R_i is image i's
+// row-pass buffer (zero in plan rows without entries, NaN there under uniform
+// weighting) and s_i = (float)(1 / sum counts of image i), the scale the
+// second pass would apply (so an image with no gridded visibility makes the
+// mean NaN, as 0 * (1/0) makes its image).  k_average_rows adds the set's
+// images of one chunk into A, in image order (deterministic); the mean is then
+// one second-pass launch over A with the set's descriptor pointed at the
+// average (meta[0] = n_img as its count, meta[1 ..] an all-non-empty row start).
+// kBlocked: the column-blocked buffer of the register kernels, else row-major.
+template <bool kBlocked>
+__global__ void __launch_bounds__(256)
+k_average_rows(SetDesc sd, int64_t img0, int n_img, int64_t li0, int n,
+               const float4* __restrict__ inter, float4* __restrict__ acc, int accumulate,
+               unsigned* __restrict__ meta, int n_img_total) {
+    const int h1 = n / 2 + 1;
+    const int64_t per = (int64_t)h1 * n / 2;  // float4 per image
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0) {
+        if (!accumulate) {
+            for (int i = tid; i <= n; i += 256) meta[1 + i] = (unsigned)i;
+            if (tid == 0) meta[0] = (unsigned)n_img_total;
+        }
+        // the count sums of images the second pass does not form
+        if (!sd.images && sd.count_sums)
+            for (int k = tid; k < n_img; k += 256)
+                sd.count_sums[img0 + k] = (double)sd.isums[img0 + k];
+    }
+    const int64_t j = (int64_t)blockIdx.x * 256 + tid;
+    if (j >= per) return;
+    const int e = (int)(2 * j);  // first of the two float2 (one plan row)
+    const int p = kBlocked ? (e % (h1 * kInterBlock)) / kInterBlock : e / n;
+    const float z = sd.uniform ? __builtin_nanf("") : 0.f;
+    float4 sum = accumulate ? acc[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < n_img; k++) {
+        const int64_t img = img0 + k;
+        const int* rsg = sd.row_start + (img % sd.g.n_ch) * (int64_t)(n + 1);
+        const float sc = (float)(1.0 / (double)sd.isums[img]);
+        const float4 v = rsg[p + 1] != rsg[p] ? inter[(li0 + k) * per + j] : make_float4(z, z, z, z);
+        sum.x += sc * v.x;
+        sum.y += sc * v.y;
+        sum.z += sc * v.z;
+        sum.w += sc * v.w;
+    }
+    acc[j] = sum;
+}
+
+hipError_t launch_average_rows(const SetDesc& sd, int64_t img0, int n_img, int64_t li0,
+                               const float* inter, float* acc, bool accumulate, unsigned* meta,
+                               hipStream_t stream) {
+    if (n_img <= 0) return hipSuccess;
+    const int n = sd.n_pixels;
+    const int64_t per = (int64_t)(n / 2 + 1) * n / 2;
+    const int64_t blocks = (per + 255) / 256;
+    auto k = batch_reg_supported(n) && n <= 2048 ? k_average_rows<true> : k_average_rows<false>;
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, stream, sd, img0, n_img, li0, n,
+                       (const float4*)inter, (float4*)acc, accumulate ? 1 : 0, meta, sd.n_img);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------- N = 4096
 //
 // At N = 4096 one row needs 4 wavefronts (256 lanes x 16 values) and 34 KiB
@@ -378,16 +460,17 @@ constexpr int kWideLanes = 256;
 
 __host__ __device__ constexpr int wide_tile_stride(int n, int rows) { return tile_stride(n, rows); }
 
-template <int N, int RB>
+template <int N, int RB, bool kUniform = false>
 constexpr size_t wide_rows_lds_bytes() {
-    return sizeof(float2) * ((size_t)RB * padded_len(N) + 128);
+    return sizeof(float2) * ((size_t)RB * padded_len(N) + 128) +
+           (kUniform ? sizeof(unsigned) * (size_t)RB * N : 0);
 }
 template <int N, int CB>
 constexpr size_t wide_cols_lds_bytes() {
     return sizeof(float2) * ((size_t)(CB / 2) * wide_tile_stride(N, CB / 2) + 128) + N;
 }
 
-template <int N, int RB, bool kMulti>
+template <int N, int RB, bool kMulti, bool kUniform>
 __global__ void __launch_bounds__(RB * kWideLanes)
 k_grid_rows_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
                  int64_t gi0, float2* __restrict__ inter, const float2* __restrict__ tws) {
@@ -396,6 +479,7 @@ k_grid_rows_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __r
     extern __shared__ float4 rows_wide_lds4[];
     float2* tile = (float2*)rows_wide_lds4;  // [RB][TS]
     float2* twl = tile + RB * TS;             // lo[64], hi[64]
+    unsigned* cnt = (unsigned*)(twl + 128);   // [RB][N] cell counts (kUniform)
     __shared__ int rs[RB + 1];
     __shared__ int red[NT / 64 + 1];
 
@@ -422,6 +506,8 @@ k_grid_rows_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __r
 
     if (tid < 128) twl[tid] = tws[tid];
     for (int i = tid; i < RB * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
+    if constexpr (kUniform)
+        for (int i = tid; i < RB * N; i += NT) cnt[i] = 0u;
     __syncthreads();
     const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
     const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[RB], tid, NT,
@@ -431,12 +517,21 @@ k_grid_rows_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __r
         float2* c = &tile[r * TS + pad_index(u_idx)];
         atomicAdd(&c->x, v.x);
         atomicAdd(&c->y, v.y);
+        if constexpr (kUniform) atomicAdd(&cnt[r * N + u_idx], 1u);
         const int row = r0 + r;
         return (row == 0 || row == H) ? 1 : 2;
     });
     const int tot = block_sum_int(n_ok, red);
     if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();
+    if constexpr (kUniform) {  // grid / count per cell, 0 / 0 = NaN in empty cells
+        for (int i = tid; i < RB * N; i += NT) {
+            float2* c = &tile[(i / N) * TS + pad_index(i % N)];
+            const float k = (float)cnt[i];
+            *c = make_float2(c->x / k, c->y / k);
+        }
+        __syncthreads();
+    }
 
     // every team transforms its row (the exchanges synchronise the workgroup);
     // rows with entries are stored
@@ -475,11 +570,13 @@ k_cols_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     const int li = (int)(bid / n_cb);
     const int64_t gi = gi0 + li;
     const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    if (!sd.images) return;  // a set imaged for its average only (uniform per workgroup)
     const int img = (int)(gi - sd.img_base);
     const int tid = threadIdx.x;
     const uint32_t count = sd.isums[img];
     if (cb == 0 && tid == 0 && sd.count_sums) sd.count_sums[img] = (double)count;
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (N + 1);
+    const float empty = sd.uniform ? __builtin_nanf("") : 0.f;  // rows without entries
     for (int i = tid; i <= H; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
     if (tid < 128) twl[tid] = tws[tid];
     __syncthreads();
@@ -489,7 +586,7 @@ k_cols_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     const float2* src = inter + (int64_t)li * (H + 1) * N + c0;
     for (int i = tid; i < (H + 1) * TEAMS; i += NT) {
         const int vr = i / TEAMS, t = i % TEAMS;
-        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 p = make_float4(empty, empty, empty, empty);
         if (nonempty[vr]) p = *(const float4*)&src[(int64_t)vr * N + 2 * t];
         // a = c_x[v] = (p.x, p.y), b = c_{x+1}[v] = (p.z, p.w); row N - v holds conjugates
         float2* zr = tile + t * TS;
@@ -525,16 +622,17 @@ k_cols_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     }
 }
 
-template <int N, int RB>
+template <int N, int RB, bool kUniform>
 static hipError_t launch_rows_wide_t(const SetDesc& first, const SetDesc* sets,
                                      const int* img_set, int n_sets, int64_t gi0, int n_img,
                                      float* inter, const float* tws, hipStream_t stream) {
     constexpr int NT = RB * kWideLanes;
-    constexpr size_t lds = wide_rows_lds_bytes<N, RB>();
+    constexpr size_t lds = wide_rows_lds_bytes<N, RB, kUniform>();
     if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
     const int64_t blocks = (int64_t)((N / 2 + 1 + RB - 1) / RB) * n_img;
     if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
-    auto k = n_sets == 1 ? k_grid_rows_wide<N, RB, false> : k_grid_rows_wide<N, RB, true>;
+    auto k = n_sets == 1 ? k_grid_rows_wide<N, RB, false, kUniform>
+                         : k_grid_rows_wide<N, RB, true, kUniform>;
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(NT), lds, stream, first, sets, img_set,
                        gi0, (float2*)inter, (const float2*)tws);
     return hipGetLastError();
@@ -555,21 +653,23 @@ static hipError_t launch_cols_wide_t(const SetDesc& first, const SetDesc* sets,
     return hipGetLastError();
 }
 
-template <int N, int RB>
+template <int N, int RB, bool kUniform>
 static hipError_t launch_rows_reg_t(const SetDesc& first, const SetDesc* sets,
                                     const int* img_set, int n_sets, int64_t gi0, int n_img,
                                     float* inter, const float* tw, hipStream_t stream) {
     constexpr int NT = rows_reg_threads<N, RB>();
-    constexpr size_t lds = batch_lds_bytes<N, RB>();
+    constexpr size_t lds = batch_lds_bytes<N, RB, kUniform>();
     if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
     const int64_t blocks = (int64_t)((N / 2 + 1 + RB - 1) / RB) * n_img;
     if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
     if (n_sets == 1)
-        hipLaunchKernelGGL((k_grid_rows_reg<N, RB, false>), dim3((unsigned)blocks), dim3(NT), lds,
-                           stream, first, sets, img_set, gi0, (float2*)inter, (const float2*)tw);
+        hipLaunchKernelGGL((k_grid_rows_reg<N, RB, false, kUniform>), dim3((unsigned)blocks),
+                           dim3(NT), lds, stream, first, sets, img_set, gi0, (float2*)inter,
+                           (const float2*)tw);
     else
-        hipLaunchKernelGGL((k_grid_rows_reg<N, RB, true>), dim3((unsigned)blocks), dim3(NT), lds,
-                           stream, first, sets, img_set, gi0, (float2*)inter, (const float2*)tw);
+        hipLaunchKernelGGL((k_grid_rows_reg<N, RB, true, kUniform>), dim3((unsigned)blocks),
+                           dim3(NT), lds, stream, first, sets, img_set, gi0, (float2*)inter,
+                           (const float2*)tw);
     return hipGetLastError();
 }
 
@@ -602,6 +702,10 @@ static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
     X(1024, 4) X(1024, 8) X(1024, 16) \
     X(2048, 4) X(2048, 8)
 #define BLINK_ROW_WIDE_VARIANTS(X) X(4096, 1) X(4096, 2) X(4096, 4)
+// Row tiles of uniform weighting (the smallest built tile per side: the cell
+// counts take LDS beside the tile)
+#define BLINK_ROW_UNIFORM_VARIANTS(X) X(128, 8) X(256, 4) X(512, 2) X(1024, 4) X(2048, 4)
+#define BLINK_ROW_WIDE_UNIFORM_VARIANTS(X) X(4096, 1)
 #define BLINK_COL_WIDE_VARIANTS(X) X(4096, 2) X(4096, 4) X(4096, 8)
 #define BLINK_COL_VARIANTS(X) \
     X(128, 16) X(128, 32) X(128, 64) \
@@ -643,20 +747,44 @@ int batch_col_variants(int n, int* tiles, int max_tiles) {
     return k;
 }
 
+int batch_uniform_row_tile(int n) {
+#define X(NN, T) \
+    if (n == NN) return T;
+    BLINK_ROW_UNIFORM_VARIANTS(X)
+    BLINK_ROW_WIDE_UNIFORM_VARIANTS(X)
+#undef X
+    return 0;
+}
+
 hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                 int n_sets, int n, int tile, int64_t gi0, int n_img, float* inter,
-                                const float* tw, hipStream_t stream) {
+                                const float* tw, hipStream_t stream, bool uniform) {
     if (n_img == 0) return hipSuccess;
+    if (uniform) {
+#define X(NN, T) \
+    if (n == NN) \
+        return launch_rows_reg_t<NN, T, true>(first, sets, img_set, n_sets, gi0, n_img, inter, \
+                                              tw, stream);
+        BLINK_ROW_UNIFORM_VARIANTS(X)
+#undef X
+#define X(NN, T) \
+    if (n == NN) \
+        return launch_rows_wide_t<NN, T, true>(first, sets, img_set, n_sets, gi0, n_img, inter, \
+                                               tw + 2 * NN, stream);
+        BLINK_ROW_WIDE_UNIFORM_VARIANTS(X)
+#undef X
+        return hipErrorNotSupported;
+    }
 #define X(NN, T) \
     if (n == NN && tile == T) \
-        return launch_rows_reg_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, tw, \
-                                        stream);
+        return launch_rows_reg_t<NN, T, false>(first, sets, img_set, n_sets, gi0, n_img, inter, \
+                                               tw, stream);
     BLINK_ROW_VARIANTS(X)
 #undef X
 #define X(NN, T) \
     if (n == NN && tile == T) \
-        return launch_rows_wide_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, \
-                                         tw + 2 * NN, stream);
+        return launch_rows_wide_t<NN, T, false>(first, sets, img_set, n_sets, gi0, n_img, inter, \
+                                                tw + 2 * NN, stream);
     BLINK_ROW_WIDE_VARIANTS(X)
 #undef X
     return hipErrorNotSupported;
@@ -688,20 +816,30 @@ hipError_t set_batch_kernel_attributes() {
         hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (r != hipSuccess && e == hipSuccess) e = r;
     };
-#define X(NN, T)                                       \
-    set((const void*)k_grid_rows_reg<NN, T, false>);   \
-    set((const void*)k_grid_rows_reg<NN, T, true>);
+#define X(NN, T)                                              \
+    set((const void*)k_grid_rows_reg<NN, T, false, false>);   \
+    set((const void*)k_grid_rows_reg<NN, T, true, false>);
     BLINK_ROW_VARIANTS(X)
+#undef X
+#define X(NN, T)                                             \
+    set((const void*)k_grid_rows_reg<NN, T, false, true>);   \
+    set((const void*)k_grid_rows_reg<NN, T, true, true>);
+    BLINK_ROW_UNIFORM_VARIANTS(X)
 #undef X
 #define X(NN, T)                                  \
     set((const void*)k_cols_reg<NN, T, false>);   \
     set((const void*)k_cols_reg<NN, T, true>);
     BLINK_COL_VARIANTS(X)
 #undef X
-#define X(NN, T)                                        \
-    set((const void*)k_grid_rows_wide<NN, T, false>);   \
-    set((const void*)k_grid_rows_wide<NN, T, true>);
+#define X(NN, T)                                               \
+    set((const void*)k_grid_rows_wide<NN, T, false, false>);   \
+    set((const void*)k_grid_rows_wide<NN, T, true, false>);
     BLINK_ROW_WIDE_VARIANTS(X)
+#undef X
+#define X(NN, T)                                              \
+    set((const void*)k_grid_rows_wide<NN, T, false, true>);   \
+    set((const void*)k_grid_rows_wide<NN, T, true, true>);
+    BLINK_ROW_WIDE_UNIFORM_VARIANTS(X)
 #undef X
 #define X(NN, T)                                   \
     set((const void*)k_cols_wide<NN, T, false>);   \

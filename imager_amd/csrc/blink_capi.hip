@@ -29,8 +29,19 @@ struct GraphEntry {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void* blob = nullptr;  // device copy of the call's descriptor tables
+    hipEvent_t done = nullptr;  // recorded after every replay
+    std::vector<const void*> ptrs;  // vis, images, count_sums, average of every set at capture
+    std::vector<unsigned char> host_blob;  // source of the last descriptor refresh
     uint64_t last_use = 0;
 };
+
+void release_graph(GraphEntry& ge) {
+    if (ge.exec) (void)hipGraphExecDestroy(ge.exec);
+    if (ge.graph) (void)hipGraphDestroy(ge.graph);
+    if (ge.blob) (void)hipFree(ge.blob);
+    if (ge.done) (void)hipEventDestroy(ge.done);
+    ge = GraphEntry{};
+}
 
 // Captured graphs kept per context; the least recently used is dropped
 // beyond this.
@@ -160,20 +171,30 @@ int set_device(blink_ctx* ctx) {
     return BLINK_OK;
 }
 
+// Streams and events of a context, each destroyed once (creation failures
+// leave the later handles null).
+void destroy_streams(blink_ctx* ctx) {
+    for (hipStream_t* s : {&ctx->capture_stream, &ctx->aux_stream}) {
+        if (*s) (void)hipStreamDestroy(*s);
+        *s = nullptr;
+    }
+    for (hipEvent_t* e : {&ctx->fork_event, &ctx->join_event, &ctx->compact_fork,
+                          &ctx->compact_join}) {
+        if (*e) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
+}
+
 void drop_graphs(blink_ctx* ctx) {
     if (ctx->graphs.empty()) return;
     (void)hipDeviceSynchronize();  // a queued replay may still read a descriptor blob
-    for (auto& kv : ctx->graphs) {
-        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
-        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
-        if (kv.second.blob) (void)hipFree(kv.second.blob);
-    }
+    for (auto& kv : ctx->graphs) release_graph(kv.second);
     ctx->graphs.clear();
 }
 
 // Grow-only named scratch buffer.  Growing one invalidates every captured
 // graph (they hold the old address), so those are dropped first.
-int scratch(blink_ctx* ctx, const char* name, size_t bytes, void** out) {
+int ctx_scratch(blink_ctx* ctx, const char* name, size_t bytes, void** out) {
     DeviceBuffer& b = ctx->scratch[name];
     if (b.bytes < bytes) {
         if (b.ptr) {
@@ -337,13 +358,7 @@ BLINK_API int blink_create(int device, blink_ctx** out) {
         hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->compact_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->compact_join, hipEventDisableTiming) != hipSuccess) {
-        if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
-        if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
-        if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
-        if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
-    if (ctx->compact_fork) (void)hipEventDestroy(ctx->compact_fork);
-    if (ctx->compact_join) (void)hipEventDestroy(ctx->compact_join);
-        if (ctx->compact_fork) (void)hipEventDestroy(ctx->compact_fork);
+        destroy_streams(ctx);
         delete ctx;
         return BLINK_ERR_HIP;
     }
@@ -375,10 +390,7 @@ BLINK_API void blink_destroy(blink_ctx* ctx) {
     for (auto& kv : ctx->kb_tables) (void)hipFree(kv.second);
     prof_collect(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
-    if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
-    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
-    if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
-    if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
+    destroy_streams(ctx);
     delete ctx;
 }
 
@@ -452,8 +464,8 @@ static int build_plan(blink_ctx* ctx, const float* u_m, const float* v_m, const 
     void *rs, *en;
     int rc;
     std::string a = std::string("plan_rows") + tag, b = std::string("plan_entries") + tag;
-    if ((rc = scratch(ctx, a.c_str(), sizeof(int) * (size_t)n_ch * (n_pixels + 1), &rs))) return rc;
-    if ((rc = scratch(ctx, b.c_str(), sizeof(uint32_t) * 2 * (size_t)n_ch * std::max(n_bl, 1), &en)))
+    if ((rc = ctx_scratch(ctx, a.c_str(), sizeof(int) * (size_t)n_ch * (n_pixels + 1), &rs))) return rc;
+    if ((rc = ctx_scratch(ctx, b.c_str(), sizeof(uint32_t) * 2 * (size_t)n_ch * std::max(n_bl, 1), &en)))
         return rc;
     LAUNCH("plan_rows", stream,
            launch_plan_rows(u_m, v_m, flags, n_bl, freq_hz, n_ch, du, dv, min_uv, n_pixels,
@@ -688,7 +700,7 @@ BLINK_API int blink_image(blink_ctx* ctx, float* grids, float* counts, int n_pix
     if (n_images == 0) return BLINK_OK;
     hipStream_t s = (hipStream_t)stream;
     void* sums;
-    if ((rc = scratch(ctx, "image_sums", sizeof(double) * (size_t)n_images, &sums))) return rc;
+    if ((rc = ctx_scratch(ctx, "image_sums", sizeof(double) * (size_t)n_images, &sums))) return rc;
     const int64_t gs = (int64_t)n_pixels * n_pixels;
     LAUNCH("vector_sum", s, launch_vector_sum(counts, gs, n_images, (double*)sums, s));
     if ((rc = fft2d(ctx, grids, images, n_pixels, n_images, BLINK_FFT_BACKWARD, true,
@@ -732,11 +744,16 @@ struct SetScratch {
     double* coef;  // [n_bl] phase coefficients
     int chunk;  // DFT sets: images per launch triple
     float2* compact = nullptr;  // [n_img][n_bl] compacted visibilities, or none
+    // FFT sets with an average: the weighted sum of the row-pass buffers and
+    // the average pass's count and row start (launch_average_rows)
+    float* avg_acc = nullptr;
+    unsigned* avg_meta = nullptr;
 };
 
 struct BatchGroup {
     int n = 0;
     int transform = BLINK_TRANSFORM_FFT;
+    bool uniform = false;        // "U" weighting
     bool reg = false;            // register-FFT kernels (power-of-two sides 128..2048)
     std::vector<int> sets;       // call indices of the sets, in descriptor order
     int first_desc = 0;          // index of the group's first SetDesc in the table
@@ -759,6 +776,11 @@ struct BatchPlan {
     // on the call's stream, the others inter_aux on the aux stream
     float* inter_aux = nullptr;
     int main = 0;
+    // environment switches read for this call (part of the graph key)
+    int64_t inter_budget = 0;
+    bool compact_on = true;
+    // device copy of the blob, when build_batch placed it (arena)
+    void* blob_dev = nullptr;
     // host copy of a group's first set descriptor (a kernel argument)
     const SetDesc& host_desc(const BatchGroup& g) const {
         return ((const SetDesc*)blob.data())[g.first_desc];
@@ -769,9 +791,37 @@ size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 }  // namespace
 
+// Where a batched call's scratch lives: the context's grow-only named buffers,
+// or, for blink_image_batch_noalloc, consecutive 256-byte aligned pieces of
+// one fixed device region.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static int batch_buffer(blink_ctx* ctx, Arena* arena, const char* name, size_t bytes,
+                        void** out) {
+    bytes = std::max<size_t>(bytes, 16);
+    if (!arena) return ctx_scratch(ctx, name, bytes, out);
+    const size_t off = align256(arena->used);
+    if (off + bytes > arena->cap)
+        return ctx->fail(BLINK_ERR_INVALID_ARGUMENT,
+                         "the call needs more than the %zu MiB of the fixed scratch region "
+                         "(%s: %zu bytes at offset %zu); split it into smaller calls",
+                         arena->cap >> 20, name, bytes, off);
+    *out = arena->base + off;
+    arena->used = off + bytes;
+    return BLINK_OK;
+}
+
 // Groups, scratch and descriptor tables of one call.  Every scratch buffer is
-// reserved here, before any capture.
-static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, BatchPlan& bp) {
+// reserved here, before any capture.  With an arena, the buffers and the
+// descriptor blob (bp.blob_dev) are cut from it and the row-pass chunks are
+// sized to what remains.
+static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, BatchPlan& bp,
+                       Arena* arena = nullptr) {
     int rc;
     // Row-pass buffer: up to 4 GiB of row-transformed grids per chunk.  Fewer,
     // larger launches win over chunks small enough to stay in the 256 MiB
@@ -783,16 +833,18 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         const long long mib = atoll(e);
         if (mib > 0) inter_budget = mib << 20;
     }
-    // groups in order of first appearance
-    std::map<std::pair<int, int>, int> gidx;
+    // groups (grid side, transform, weighting) in order of first appearance
+    std::map<std::tuple<int, int, int>, int> gidx;
     for (int i = 0; i < n_sets; i++) {
-        const std::pair<int, int> key(sets[i].n_pixels, sets[i].transform);
+        const std::tuple<int, int, int> key(sets[i].n_pixels, sets[i].transform,
+                                            sets[i].weighting);
         auto it = gidx.find(key);
         if (it == gidx.end()) {
             it = gidx.emplace(key, (int)bp.groups.size()).first;
             BatchGroup g;
             g.n = sets[i].n_pixels;
             g.transform = sets[i].transform;
+            g.uniform = sets[i].weighting == BLINK_WEIGHTING_UNIFORM;
             g.reg = g.transform == BLINK_TRANSFORM_FFT && batch_reg_supported(g.n);
             bp.groups.push_back(g);
         }
@@ -800,12 +852,10 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         g.sets.push_back(i);
         g.n_img += (int64_t)sets[i].n_ch * sets[i].n_t;
     }
-    // scratch sizes
-    size_t rs_bytes = 0, en_bytes = 0, inter_bytes = 16;
-    std::vector<size_t> group_inter(bp.groups.size(), 16);
-    std::vector<size_t> rs_off(n_sets), en_off(n_sets), is_off(n_sets);
+    // scratch sizes of the plans
+    size_t rs_bytes = 0, en_bytes = 0;
+    std::vector<size_t> rs_off(n_sets), en_off(n_sets), is_off(n_sets), co_off(n_sets);
     int64_t n_img_all = 0, n_units = 0, min_bl = INT64_MAX, max_bl = 0, n_cells = 0, n_bl_all = 0;
-    std::vector<size_t> co_off(n_sets);
     int n_max = 0;
     for (int i = 0; i < n_sets; i++) {
         const blink_image_set& st = sets[i];
@@ -825,35 +875,6 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         min_bl = std::min<int64_t>(min_bl, n_bl);
         max_bl = std::max<int64_t>(max_bl, n_bl);
         n_max = std::max(n_max, st.n_pixels);
-    }
-    for (BatchGroup& g : bp.groups) {
-        const int64_t n = g.n;
-        if (is_dft(g.transform)) {
-            for (int i : g.sets) {
-                const int n_img = sets[i].n_ch * sets[i].n_t;
-                const int64_t per = dft_inter_bytes(n);
-                const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
-                group_inter[&g - bp.groups.data()] =
-                    std::max(group_inter[&g - bp.groups.data()], (size_t)(per * chunk));
-            }
-            continue;
-        }
-        const int64_t per = g.reg ? batch_inter_bytes(g.n) : 8 * (n / 2 + 1) * n;
-        g.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(g.n_img, inter_budget / per));
-        group_inter[&g - bp.groups.data()] = (size_t)(per * g.chunk);
-    }
-    // the main group (most image bytes) keeps its own row-pass buffer; the
-    // others share a second one on the aux stream
-    {
-        int64_t best = -1;
-        for (size_t k = 0; k < bp.groups.size(); k++) {
-            const int64_t bytes = bp.groups[k].n_img * (int64_t)bp.groups[k].n * bp.groups[k].n;
-            if (bytes > best) best = bytes, bp.main = (int)k;
-        }
-        size_t aux = 0;
-        for (size_t k = 0; k < bp.groups.size(); k++)
-            if ((int)k != bp.main) aux = std::max(aux, group_inter[k]);
-        inter_bytes = align16(group_inter[bp.main]) + aux;
     }
     // plan parts: about 1024 workgroups over the units with at least 256
     // baselines per part, and never more than kPlanPartMax baselines per part
@@ -880,6 +901,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     {
         const char* env = getenv("BLINK_COMPACT");
         const bool on = !(env && env[0] == '0');
+        bp.compact_on = on;
         for (BatchGroup& g : bp.groups) {
             if (!on || !g.reg || g.n > 2048) continue;
             for (int i : g.sets) {
@@ -892,23 +914,82 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             }
         }
     }
-    void *inter, *rs, *en, *is, *pa, *cl, *co, *cp = nullptr;
-    if ((rc = scratch(ctx, "batch_inter", inter_bytes, &inter))) return rc;
-    if (cp_bytes && (rc = scratch(ctx, "batch_compact", cp_bytes, &cp))) return rc;
-    if ((rc = scratch(ctx, "batch_rows", std::max<size_t>(rs_bytes, 16), &rs))) return rc;
-    if ((rc = scratch(ctx, "batch_entries", std::max<size_t>(en_bytes, 16), &en))) return rc;
-    if ((rc = scratch(ctx, "batch_isums", std::max<size_t>(4 * (size_t)n_img_all, 16), &is)))
+    // descriptor blob layout: SetDesc table (group order), units, image -> set tables
+    size_t blob_bytes = align16(sizeof(SetDesc) * (size_t)n_sets);
+    bp.units_off = blob_bytes;
+    blob_bytes = align16(blob_bytes + sizeof(PlanUnit) * (size_t)n_units);
+    for (BatchGroup& g : bp.groups) {
+        if (g.sets.size() < 2 || g.transform != BLINK_TRANSFORM_FFT) continue;
+        g.img_set_off = blob_bytes;
+        blob_bytes = align16(blob_bytes + sizeof(int) * (size_t)g.n_img);
+    }
+
+    // averages of FFT sets: the weighted sum of one image's row-pass buffer and
+    // the average pass's metadata
+    std::vector<size_t> av_off(n_sets, SIZE_MAX);
+    size_t av_bytes = 0;
+    for (int i = 0; i < n_sets; i++) {
+        if (!sets[i].average || is_dft(sets[i].transform)) continue;
+        const size_t n = (size_t)sets[i].n_pixels;
+        av_off[i] = av_bytes;
+        av_bytes += align256(sizeof(float2) * (n / 2 + 1) * n) + align256(4 * (n + 2));
+    }
+    void *rs, *en, *is, *pa, *cl, *co, *cp = nullptr, *av = nullptr;
+    if (av_bytes && (rc = batch_buffer(ctx, arena, "batch_average", av_bytes, &av))) return rc;
+    if (cp_bytes && (rc = batch_buffer(ctx, arena, "batch_compact", cp_bytes, &cp))) return rc;
+    if ((rc = batch_buffer(ctx, arena, "batch_rows", rs_bytes, &rs))) return rc;
+    if ((rc = batch_buffer(ctx, arena, "batch_entries", en_bytes, &en))) return rc;
+    if ((rc = batch_buffer(ctx, arena, "batch_isums", 4 * (size_t)n_img_all, &is))) return rc;
+    if ((rc = batch_buffer(ctx, arena, "batch_plan_parts", partial_bytes, &pa))) return rc;
+    if ((rc = batch_buffer(ctx, arena, "batch_plan_cells", 4 * (size_t)n_cells, &cl))) return rc;
+    if ((rc = batch_buffer(ctx, arena, "batch_bl_coef", sizeof(double) * (size_t)n_bl_all, &co)))
         return rc;
-    if ((rc = scratch(ctx, "batch_plan_parts", std::max<size_t>(partial_bytes, 16), &pa)))
-        return rc;
-    if ((rc = scratch(ctx, "batch_plan_cells", std::max<size_t>(4 * (size_t)n_cells, 16), &cl)))
-        return rc;
+    if (arena) {
+        if ((rc = batch_buffer(ctx, arena, "batch_desc", blob_bytes, &bp.blob_dev))) return rc;
+        // the row-pass buffer takes what is left, split between the call's
+        // stream and the second stream when there are several groups
+        const int64_t left = (int64_t)arena->cap - (int64_t)align256(arena->used) - 256;
+        inter_budget = std::max<int64_t>(0, bp.groups.size() > 1 ? left / 2 - 256 : left);
+    }
+    bp.inter_budget = inter_budget;
+
+    // row-pass chunks: images per row / column launch pair
+    std::vector<size_t> group_inter(bp.groups.size(), 16);
+    for (BatchGroup& g : bp.groups) {
+        const int64_t n = g.n;
+        const size_t gk = &g - bp.groups.data();
+        if (is_dft(g.transform)) {
+            for (int i : g.sets) {
+                const int n_img = sets[i].n_ch * sets[i].n_t;
+                const int64_t per = dft_inter_bytes(n);
+                const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
+                group_inter[gk] = std::max(group_inter[gk], (size_t)(per * chunk));
+            }
+            continue;
+        }
+        const int64_t per = g.reg ? batch_inter_bytes(g.n) : 8 * (n / 2 + 1) * n;
+        g.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(g.n_img, inter_budget / per));
+        group_inter[gk] = (size_t)(per * g.chunk);
+    }
+    // the main group (most image bytes) keeps its own row-pass buffer; the
+    // others share a second one on the aux stream
+    size_t inter_bytes = 16;
+    {
+        int64_t best = -1;
+        for (size_t k = 0; k < bp.groups.size(); k++) {
+            const int64_t bytes = bp.groups[k].n_img * (int64_t)bp.groups[k].n * bp.groups[k].n;
+            if (bytes > best) best = bytes, bp.main = (int)k;
+        }
+        size_t aux = 0;
+        for (size_t k = 0; k < bp.groups.size(); k++)
+            if ((int)k != bp.main) aux = std::max(aux, group_inter[k]);
+        inter_bytes = align256(group_inter[bp.main]) + aux;
+    }
+    void* inter;
+    if ((rc = batch_buffer(ctx, arena, "batch_inter", inter_bytes, &inter))) return rc;
     bp.cells = (uint32_t*)cl;
-    if ((rc = scratch(ctx, "batch_bl_coef", std::max<size_t>(sizeof(double) * (size_t)n_bl_all, 16),
-                      &co)))
-        return rc;
     bp.inter = (float*)inter;
-    bp.inter_aux = (float*)((char*)inter + align16(group_inter[bp.main]));
+    bp.inter_aux = (float*)((char*)inter + align256(group_inter[bp.main]));
     bp.partial = (int*)pa;
     bp.sc.resize(n_sets);
     for (int i = 0; i < n_sets; i++) {
@@ -920,17 +1001,33 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         bp.sc[i].isums = (uint32_t*)is + is_off[i];
         bp.sc[i].coef = (double*)co + co_off[i];
         if (cp_off[i] != SIZE_MAX) bp.sc[i].compact = (float2*)((char*)cp + cp_off[i]);
+        if (av_off[i] != SIZE_MAX) {
+            const size_t n = (size_t)sets[i].n_pixels;
+            bp.sc[i].avg_acc = (float*)((char*)av + av_off[i]);
+            bp.sc[i].avg_meta = (unsigned*)((char*)av + av_off[i] +
+                                            align256(sizeof(float2) * (n / 2 + 1) * n));
+        }
         const int64_t per = dft_inter_bytes(sets[i].n_pixels);
         const int n_img = sets[i].n_ch * sets[i].n_t;
         bp.sc[i].chunk = (int)std::max<int64_t>(1, std::min<int64_t>(n_img, inter_budget / per));
     }
     for (int i = 0; i < n_sets; i++) {
+        const int n = sets[i].n_pixels;
+        if (arena) {
+            // tables are never built here: only those the context already holds
+            if (!ctx->twiddles.count(n))
+                return ctx->fail(BLINK_ERR_UNSUPPORTED,
+                                 "set %d: grid side %d has no FFT table in this context (the "
+                                 "no-allocation call images the sides blink_create prepares: "
+                                 "128, 180, 256, 512, 1024, 2048, 4096)", i, n);
+            continue;
+        }
         const float* tw;
-        if ((rc = twiddles(ctx, sets[i].n_pixels, &tw))) return rc;
+        if ((rc = twiddles(ctx, n, &tw))) return rc;
         if (is_dft(sets[i].transform)) {
             const uint16_t* pl;
             for (int stage = 1; stage <= 2; stage++)
-                if ((rc = dft_planes(ctx, sets[i].n_pixels, stage, dft_prec(sets[i].transform), &pl)))
+                if ((rc = dft_planes(ctx, n, stage, dft_prec(sets[i].transform), &pl)))
                     return rc;
         }
     }
@@ -973,20 +1070,13 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             // pass writes whole image rows (the DFT stages keep rows v)
             d.uv_swap = g.transform == BLINK_TRANSFORM_FFT;
             d.compact = bp.sc[i].compact;
+            d.uniform = st.weighting == BLINK_WEIGHTING_UNIFORM ? 1 : 0;
             base += d.n_img;
             desc_of[i] = (int)desc.size();
             desc.push_back(d);
         }
     }
-    size_t off = align16(sizeof(SetDesc) * desc.size());
-    bp.units_off = off;
-    off = align16(off + sizeof(PlanUnit) * (size_t)n_units);
-    for (BatchGroup& g : bp.groups) {
-        if (g.sets.size() < 2 || g.transform != BLINK_TRANSFORM_FFT) continue;
-        g.img_set_off = off;
-        off = align16(off + sizeof(int) * (size_t)g.n_img);
-    }
-    bp.blob.assign(off, 0);
+    bp.blob.assign(blob_bytes, 0);
     memcpy(bp.blob.data(), desc.data(), sizeof(SetDesc) * desc.size());
     PlanUnit* units = (PlanUnit*)(bp.blob.data() + bp.units_off);
     int64_t u = 0, cell_off = 0;
@@ -1156,12 +1246,16 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
                 const int cnt = std::min(sc.chunk, n_img - img0);
                 LAUNCH("grid_half_rows", s,
                        launch_grid_half_rows(ga, sc.row_start, sc.entries, n, cnt, img0, half,
-                                             sc.isums, s));
+                                             sc.isums, s, g.uniform));
                 LAUNCH("dft_rows", s, launch_dft_rows(half, pl1, tbuf, n, cnt, prec, s));
                 LAUNCH("dft_cols", s,
                        launch_dft_cols(tbuf, pl2, st.images, n, cnt, img0, sc.isums,
                                        st.count_sums, prec, s));
             }
+            // the mean of the formed images (image_averaging_gpu's arithmetic)
+            if (st.average)
+                LAUNCH("average", s,
+                       launch_average(st.images, (int64_t)n * n, n_img, 0, st.average, s));
         }
         return BLINK_OK;
     }
@@ -1171,24 +1265,101 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
     const int ns = (int)g.sets.size();
     FftPlan plan;
     make_fft_plan(n, plan);
+    auto it = ctx->tiles.find(n);
+    const std::pair<int, int> tl = it != ctx->tiles.end() ? it->second : default_tiles(n);
+    // the second pass is skipped when no set of the group forms its images
+    bool any_images = false;
+    for (int i : g.sets) any_images |= sets[i].images != nullptr;
+    auto second_pass = [&](const SetDesc& f, const SetDesc* tab, const int* img_set, int n_sets,
+                           int64_t gi0, int cnt, const float* src) {
+        return g.reg ? launch_cols_reg(f, tab, img_set, n_sets, n, tl.second, gi0, cnt, src, tw, s)
+                     : launch_cols_half(f, tab, img_set, n_sets, plan, gi0, cnt, src, tw, s);
+    };
     for (int64_t gi0 = 0; gi0 < g.n_img; gi0 += g.chunk) {
         const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img - gi0);
-        if (g.reg) {
-            auto it = ctx->tiles.find(n);
-            const std::pair<int, int> tl = it != ctx->tiles.end() ? it->second : default_tiles(n);
+        if (g.reg)
             LAUNCH("grid_rows_fft", s,
-                   launch_grid_rows_reg(first, sd, is, ns, n, tl.first, gi0, cnt, inter,
-                                        tw, s));
-            LAUNCH("fft_cols", s,
-                   launch_cols_reg(first, sd, is, ns, n, tl.second, gi0, cnt, inter, tw,
-                                   s));
-            continue;
+                   launch_grid_rows_reg(first, sd, is, ns, n, tl.first, gi0, cnt, inter, tw, s,
+                                        g.uniform));
+        else
+            LAUNCH("grid_rows_fft", s,
+                   launch_grid_rows_fft(first, sd, is, ns, plan, gi0, cnt, inter, tw, s));
+        if (any_images) LAUNCH("fft_cols", s, second_pass(first, sd, is, ns, gi0, cnt, inter));
+        // sets with an average: add this chunk's images of the set
+        for (size_t j = 0; j < g.sets.size(); j++) {
+            const SetScratch& sc = bp.sc[g.sets[j]];
+            if (!sc.avg_acc) continue;
+            const SetDesc& d = ((const SetDesc*)bp.blob.data())[g.first_desc + j];
+            const int64_t a = std::max<int64_t>(gi0, d.img_base);
+            const int64_t b = std::min<int64_t>(gi0 + cnt, d.img_base + d.n_img);
+            if (a >= b) continue;
+            LAUNCH("average_rows", s,
+                   launch_average_rows(d, a - d.img_base, (int)(b - a), a - gi0, inter,
+                                       sc.avg_acc, a > d.img_base, sc.avg_meta, s));
         }
-        LAUNCH("grid_rows_fft", s,
-               launch_grid_rows_fft(first, sd, is, ns, plan, gi0, cnt, inter, tw, s));
-        LAUNCH("fft_cols", s,
-               launch_cols_half(first, sd, is, ns, plan, gi0, cnt, inter, tw, s));
     }
+    // the averages: one second pass each over the summed rows, scaled by 1/n_img
+    for (size_t j = 0; j < g.sets.size(); j++) {
+        const SetScratch& sc = bp.sc[g.sets[j]];
+        if (!sc.avg_acc) continue;
+        SetDesc d = ((const SetDesc*)bp.blob.data())[g.first_desc + j];
+        d.images = (float2*)sets[g.sets[j]].average;
+        d.isums = sc.avg_meta;
+        d.row_start = (int*)(sc.avg_meta + 1);
+        d.count_sums = nullptr;
+        d.g.n_ch = 1;
+        d.img_base = 0;
+        d.n_img = 1;
+        d.uniform = 0;
+        LAUNCH("fft_cols", s, second_pass(d, nullptr, nullptr, 1, 0, 1, sc.avg_acc));
+    }
+    return BLINK_OK;
+}
+
+// Work forked from s onto the aux stream: fork() makes the aux stream wait
+// for s; join() (also on every error path after a successful fork) makes s
+// wait for everything enqueued on the aux stream since, so that a capture on
+// s never ends with the aux stream still part of it.
+struct AuxFork {
+    blink_ctx* ctx;
+    hipStream_t s, a;
+    hipEvent_t fork_ev, join_ev;
+    bool forked = false;
+    int fork() {
+        hipError_t e = hipEventRecord(fork_ev, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(a, fork_ev, 0);
+        if (e != hipSuccess)
+            return ctx->fail(BLINK_ERR_HIP, "fork to the second stream failed: %s",
+                             hipGetErrorString(e));
+        forked = true;
+        return BLINK_OK;
+    }
+    int join() {
+        if (!forked) return BLINK_OK;
+        forked = false;
+        hipError_t e = hipEventRecord(join_ev, a);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, join_ev, 0);
+        if (e != hipSuccess)
+            return ctx->fail(BLINK_ERR_HIP, "join of the second stream failed: %s",
+                             hipGetErrorString(e));
+        return BLINK_OK;
+    }
+    // keeps the first error: a failure inside the forked work, else the join's
+    int join_after(int rc) {
+        const int rj = join();
+        return rc ? rc : rj;
+    }
+};
+
+static int enqueue_compaction(blink_ctx* ctx, const BatchPlan& bp, const BatchTables& t,
+                              hipStream_t a) {
+    for (const BatchGroup& g : bp.groups)
+        if (g.compact_max_bl)
+            LAUNCH("vis_compact", a,
+                   g.sets.size() == 1
+                       ? launch_vis_compact(bp.host_desc(g), g.n_img, a)
+                       : launch_vis_compact_sets(t.group_sets(g), t.img_set(g), g.n_img,
+                                                 g.compact_max_bl, a));
     return BLINK_OK;
 }
 
@@ -1205,32 +1376,20 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const Batc
     // (on s, or on the aux stream after it) come after both
     bool compact = false;
     for (const BatchGroup& g : bp.groups) compact |= g.compact_max_bl != 0;
-    if (compact) {
-        HIP_TRY(hipEventRecord(ctx->compact_fork, s));
-        HIP_TRY(hipStreamWaitEvent(a, ctx->compact_fork, 0));
-        for (const BatchGroup& g : bp.groups)
-            if (g.compact_max_bl)
-                LAUNCH("vis_compact", a,
-                       g.sets.size() == 1
-                           ? launch_vis_compact(bp.host_desc(g), g.n_img, a)
-                           : launch_vis_compact_sets(t.group_sets(g), t.img_set(g), g.n_img,
-                                                     g.compact_max_bl, a));
-        HIP_TRY(hipEventRecord(ctx->compact_join, a));
-    }
-    int rc = enqueue_plans(ctx, bp, t, s);
-    if (rc) return rc;
-    if (compact) HIP_TRY(hipStreamWaitEvent(s, ctx->compact_join, 0));
+    AuxFork cf{ctx, s, a, ctx->compact_fork, ctx->compact_join};
+    int rc = BLINK_OK;
+    if (compact && !(rc = cf.fork())) rc = enqueue_compaction(ctx, bp, t, a);
+    if (!rc) rc = enqueue_plans(ctx, bp, t, s);
+    if ((rc = cf.join_after(rc))) return rc;
     if (bp.groups.size() == 1) return enqueue_group(ctx, sets, bp, t, bp.groups[0], bp.inter, s);
-    HIP_TRY(hipEventRecord(ctx->fork_event, s));
-    HIP_TRY(hipStreamWaitEvent(a, ctx->fork_event, 0));
-    for (size_t k = 0; k < bp.groups.size(); k++)
-        if ((int)k != bp.main &&
-            (rc = enqueue_group(ctx, sets, bp, t, bp.groups[k], bp.inter_aux, a)))
-            return rc;
-    HIP_TRY(hipEventRecord(ctx->join_event, a));
-    if ((rc = enqueue_group(ctx, sets, bp, t, bp.groups[bp.main], bp.inter, s))) return rc;
-    HIP_TRY(hipStreamWaitEvent(s, ctx->join_event, 0));
-    return BLINK_OK;
+    AuxFork gf{ctx, s, a, ctx->fork_event, ctx->join_event};
+    if ((rc = gf.fork())) return rc;
+    for (size_t k = 0; k < bp.groups.size() && !rc; k++)
+        if ((int)k != bp.main) rc = enqueue_group(ctx, sets, bp, t, bp.groups[k], bp.inter_aux, a);
+    if (rc) return gf.join_after(rc);
+    // the main group runs on s while the aux stream works; join at the end
+    rc = enqueue_group(ctx, sets, bp, t, bp.groups[bp.main], bp.inter, s);
+    return gf.join_after(rc);
 }
 
 // Autotunes the register kernels of every group whose side has no tiles yet.
@@ -1256,21 +1415,79 @@ static int autotune_batch(blink_ctx* ctx, const BatchPlan& bp, const BatchTables
     return BLINK_OK;
 }
 
-BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
-                                void* stream) {
-    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+// Cache key of a batched call: every field of every set (by value, so that
+// struct padding never enters it) except the visibility, image, count-sum and
+// average addresses (their presence is part of the key), plus the environment
+// switches that shape the call.  Calls that differ only in those addresses
+// replay one instantiated graph.
+static std::string graph_key(const blink_image_set* sets, int n_sets, const BatchPlan& bp) {
+    std::string k;
+    auto put = [&](const auto& v) { k.append((const char*)&v, sizeof v); };
+    put(n_sets);
+    put(bp.inter_budget);
+    put(bp.compact_on);
+    for (int i = 0; i < n_sets; i++) {
+        const blink_image_set& st = sets[i];
+        put(st.n_ant), put(st.n_ch), put(st.n_t);
+        put(st.u_m), put(st.v_m), put(st.w_m), put(st.bl_flags), put(st.freq_hz), put(st.cable_m);
+        put(st.corrections), put(st.delta_u), put(st.delta_v), put(st.min_uv);
+        put(st.n_pixels), put(st.pol), put(st.transform), put(st.weighting);
+        const bool has_sums = st.count_sums != nullptr, has_images = st.images != nullptr,
+                   has_average = st.average != nullptr;
+        put(has_sums), put(has_images), put(has_average);
+    }
+    return k;
+}
+
+static std::vector<const void*> io_pointers(const blink_image_set* sets, int n_sets) {
+    std::vector<const void*> p;
+    p.reserve(3 * (size_t)n_sets);
+    for (int i = 0; i < n_sets; i++) {
+        p.push_back(sets[i].vis);
+        p.push_back(sets[i].images);
+        p.push_back(sets[i].count_sums);
+        p.push_back(sets[i].average);
+    }
+    return p;
+}
+
+// Captures the call's launches on the context's capture stream.
+static int capture_batch(blink_ctx* ctx, const blink_image_set* sets, const BatchPlan& bp,
+                         const BatchTables& t, hipGraph_t* out) {
+    hipStream_t cs = ctx->capture_stream;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess)
+        return ctx->fail(BLINK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    const int rc = enqueue_batch(ctx, sets, bp, t, cs);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(cs, &graph);
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
+        return ctx->fail(BLINK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    *out = graph;
+    return BLINK_OK;
+}
+
+// Argument checks shared by the batched entry points.
+static int check_sets(blink_ctx* ctx, const blink_image_set* sets, int n_sets) {
     CHECK_ARG(sets && n_sets >= 1, "need at least one image set");
     for (int i = 0; i < n_sets; i++) {
         const blink_image_set& st = sets[i];
         int rc = check_vis_shape(ctx, st.n_ant, st.n_ch, st.n_t);
         if (rc) return rc;
-        CHECK_ARG(st.vis && st.u_m && st.v_m && st.bl_flags && st.freq_hz && st.images,
+        CHECK_ARG(st.vis && st.u_m && st.v_m && st.bl_flags && st.freq_hz,
                   "set %d: null pointer argument", i);
+        CHECK_ARG(st.images || st.average, "set %d: neither images nor average is given", i);
         CHECK_ARG(!(st.corrections & BLINK_CORR_GEOMETRIC) || st.w_m, "set %d: w_m is null", i);
         CHECK_ARG(!(st.corrections & BLINK_CORR_CABLE) || st.cable_m, "set %d: cable_m is null", i);
         CHECK_ARG(st.pol >= 0 && st.pol <= 3, "set %d: pol must be 0..3", i);
-        CHECK_ARG(((uintptr_t)st.vis & 15) == 0 && ((uintptr_t)st.images & 15) == 0,
-                  "set %d: vis and images must be 16-byte aligned", i);
+        CHECK_ARG(st.weighting == BLINK_WEIGHTING_NATURAL || st.weighting == BLINK_WEIGHTING_UNIFORM,
+                  "set %d: weighting must be NATURAL or UNIFORM", i);
+        CHECK_ARG(((uintptr_t)st.vis & 15) == 0 && ((uintptr_t)st.images & 15) == 0 &&
+                      ((uintptr_t)st.average & 15) == 0,
+                  "set %d: vis, images and average must be 16-byte aligned", i);
         CHECK_ARG(st.transform == BLINK_TRANSFORM_FFT || is_dft(st.transform),
                   "set %d: unknown transform %d", i, st.transform);
         CHECK_ARG(st.n_pixels % 2 == 0, "set %d: the batched path needs an even grid side", i);
@@ -1279,11 +1496,20 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
                 return ctx->fail(BLINK_ERR_UNSUPPORTED,
                                  "set %d: the DFT transform needs 2 <= N <= %d, got %d", i,
                                  BLINK_MAX_DFT_PIXELS, st.n_pixels);
+            CHECK_ARG(st.images, "set %d: the DFT transforms need `images` (also with an average)",
+                      i);
             continue;
         }
         FftPlan plan;
         if ((rc = check_fft_size(ctx, st.n_pixels, plan))) return rc;
     }
+    return BLINK_OK;
+}
+
+BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
+                                void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    if (int rc = check_sets(ctx, sets, n_sets)) return rc;
     int rc = set_device(ctx);
     if (rc) return rc;
     BatchPlan bp;
@@ -1294,7 +1520,7 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
         // eager: the descriptor blob goes to a context buffer that no queued
         // work may still read
         void* dblob;
-        if ((rc = scratch(ctx, "batch_desc", bp.blob.size(), &dblob))) return rc;
+        if ((rc = ctx_scratch(ctx, "batch_desc", bp.blob.size(), &dblob))) return rc;
         HIP_TRY(hipDeviceSynchronize());
         HIP_TRY(hipMemcpy(dblob, bp.blob.data(), bp.blob.size(), hipMemcpyHostToDevice));
         const BatchTables t = tables_at(bp, dblob);
@@ -1302,58 +1528,83 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
         return enqueue_batch(ctx, sets, bp, t, s);
     }
 
-    std::string key((const char*)sets, sizeof(blink_image_set) * (size_t)n_sets);
+    const std::string key = graph_key(sets, n_sets, bp);
     auto it = ctx->graphs.find(key);
-    if (it == ctx->graphs.end()) {
+    if (it != ctx->graphs.end()) {
+        GraphEntry& ge = it->second;
+        if (ge.ptrs != io_pointers(sets, n_sets)) {
+            // same call with other visibility / image / count-sum buffers:
+            // refresh the descriptor blob in stream order, capture the call
+            // again and update the instantiated graph's kernel arguments
+            // (no new instantiation).  The graph's previous replay, and the
+            // copy before it, must have finished with the blob and its source.
+            HIP_TRY(hipEventSynchronize(ge.done));
+            ge.host_blob = bp.blob;
+            HIP_TRY(hipMemcpyAsync(ge.blob, ge.host_blob.data(), ge.host_blob.size(),
+                                   hipMemcpyHostToDevice, s));
+            hipGraph_t graph = nullptr;
+            if ((rc = capture_batch(ctx, sets, bp, tables_at(bp, ge.blob), &graph))) {
+                // the blob no longer matches the graph: drop both
+                release_graph(ge);
+                ctx->graphs.erase(it);
+                return rc;
+            }
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            hipError_t e = hipGraphExecUpdate(ge.exec, graph, &err_node, &res);
+            if (e != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(ge.exec);
+                ge.exec = nullptr;
+                e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+                if (e != hipSuccess) {
+                    (void)hipGraphDestroy(graph);
+                    release_graph(ge);
+                    ctx->graphs.erase(it);
+                    return ctx->fail(BLINK_ERR_HIP, "graph instantiation failed: %s",
+                                     hipGetErrorString(e));
+                }
+            }
+            (void)hipGraphDestroy(ge.graph);
+            ge.graph = graph;
+            ge.ptrs = io_pointers(sets, n_sets);
+        }
+    } else {
         GraphEntry ge;
-        HIP_TRY(hipMalloc(&ge.blob, bp.blob.size()));
-        hipError_t e = hipMemcpy(ge.blob, bp.blob.data(), bp.blob.size(), hipMemcpyHostToDevice);
+        HIP_TRY(hipEventCreateWithFlags(&ge.done, hipEventDisableTiming));
+        hipError_t e = hipMalloc(&ge.blob, bp.blob.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(ge.blob, bp.blob.data(), bp.blob.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
-            (void)hipFree(ge.blob);
+            release_graph(ge);
             return ctx->fail(BLINK_ERR_HIP, "descriptor upload failed: %s", hipGetErrorString(e));
         }
         const BatchTables t = tables_at(bp, ge.blob);
-        if ((rc = autotune_batch(ctx, bp, t, s))) {
-            (void)hipFree(ge.blob);
+        if ((rc = autotune_batch(ctx, bp, t, s)) || (rc = capture_batch(ctx, sets, bp, t, &ge.graph))) {
+            release_graph(ge);
             return rc;
         }
-        hipStream_t cs = ctx->capture_stream;
-        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        e = hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0);
         if (e != hipSuccess) {
-            (void)hipFree(ge.blob);
-            return ctx->fail(BLINK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-        }
-        rc = enqueue_batch(ctx, sets, bp, t, cs);
-        hipGraph_t graph = nullptr;
-        e = hipStreamEndCapture(cs, &graph);
-        if (rc || e != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            (void)hipFree(ge.blob);
-            if (rc) return rc;
-            return ctx->fail(BLINK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-        }
-        ge.graph = graph;
-        e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            (void)hipFree(ge.blob);
+            release_graph(ge);
             return ctx->fail(BLINK_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         }
+        ge.ptrs = io_pointers(sets, n_sets);
         if (ctx->graphs.size() >= kMaxGraphs) {
             auto lru = ctx->graphs.begin();
             for (auto g = ctx->graphs.begin(); g != ctx->graphs.end(); ++g)
                 if (g->second.last_use < lru->second.last_use) lru = g;
-            // a graph still executing is destroyed only after it completes
-            HIP_TRY(hipDeviceSynchronize());
-            (void)hipGraphExecDestroy(lru->second.exec);
-            (void)hipGraphDestroy(lru->second.graph);
-            if (lru->second.blob) (void)hipFree(lru->second.blob);
+            // a graph still executing is destroyed only after its last replay
+            // completes
+            HIP_TRY(hipEventSynchronize(lru->second.done));
+            release_graph(lru->second);
             ctx->graphs.erase(lru);
         }
         it = ctx->graphs.emplace(key, ge).first;
     }
     it->second.last_use = ++ctx->graph_clock;
     HIP_TRY(hipGraphLaunch(it->second.exec, s));
+    HIP_TRY(hipEventRecord(it->second.done, s));
     return BLINK_OK;
 }
 

@@ -837,10 +837,11 @@ hipError_t launch_grid_tiles(const GatherArgs& g, const int* row_start, const ui
 __global__ void __launch_bounds__(kTileThreads)
 k_grid_half_rows(GatherArgs g, const int* __restrict__ row_start,
                  const uint32_t* __restrict__ entries, int n, int rb, int64_t img0,
-                 float2* __restrict__ half, uint32_t* __restrict__ isums) {
+                 float2* __restrict__ half, uint32_t* __restrict__ isums, int uniform) {
     extern __shared__ float4 ghr_lds4[];
     float4* tile4 = ghr_lds4;  // [rb][n] complex, as n/2 float4 per row
     float2* tile = (float2*)ghr_lds4;
+    unsigned* cnt = (unsigned*)(tile + (size_t)rb * n);  // [rb][n] counts ("U" weighting)
     __shared__ int rs[kMaxTileRows + 1];
     __shared__ int red[kTileThreads / 64];
     const int h = n / 2;
@@ -859,11 +860,14 @@ k_grid_half_rows(GatherArgs g, const int* __restrict__ row_start,
     __syncthreads();
     float4* dst = (float4*)(half + (li * (h + 1) + r0) * (int64_t)n);
     const int q = nr * n / 2;
-    if (rs[nr] == rs[0]) {  // no entries: zero rows
-        for (int i = threadIdx.x; i < q; i += blockDim.x) dst[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (rs[nr] == rs[0]) {  // no entries: zero rows (empty cells, NaN, under "U")
+        const float z = uniform ? __builtin_nanf("") : 0.f;
+        for (int i = threadIdx.x; i < q; i += blockDim.x) dst[i] = make_float4(z, z, z, z);
         return;
     }
     for (int i = threadIdx.x; i < q; i += blockDim.x) tile4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (uniform)
+        for (int i = threadIdx.x; i < nr * n; i += blockDim.x) cnt[i] = 0u;
     __syncthreads();
     const uint32_t* ent = entries + (int64_t)ch * 2 * g.n_bl;
     const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[nr], (int)threadIdx.x,
@@ -871,18 +875,26 @@ k_grid_half_rows(GatherArgs g, const int* __restrict__ row_start,
         const int r = row_of_entry(rs, nr, e);
         atomicAdd(&tile[r * n + u_idx].x, v.x);
         atomicAdd(&tile[r * n + u_idx].y, v.y);
+        if (uniform) atomicAdd(&cnt[r * n + u_idx], 1u);
         const int row = r0 + r;
         return (row == 0 || row == h) ? 1 : 2;
     });
     const int tot = block_sum_int(n_ok, red);
     if (threadIdx.x == 0 && tot) atomicAdd(&isums[img], (uint32_t)tot);
     __syncthreads();
+    if (uniform) {  // grid / count per cell, 0 / 0 = NaN in empty cells
+        for (int i = threadIdx.x; i < nr * n; i += blockDim.x) {
+            const float k = (float)cnt[i];
+            tile[i] = make_float2(tile[i].x / k, tile[i].y / k);
+        }
+        __syncthreads();
+    }
     for (int i = threadIdx.x; i < q; i += blockDim.x) dst[i] = tile4[i];
 }
 
 hipError_t launch_grid_half_rows(const GatherArgs& g, const int* row_start, const uint32_t* entries,
                                  int n, int64_t n_img, int64_t img0, float* half,
-                                 uint32_t* isums, hipStream_t stream) {
+                                 uint32_t* isums, hipStream_t stream, bool uniform) {
     if (n_img == 0) return hipSuccess;
     if (n % 2) return hipErrorInvalidValue;
     // Two rows per workgroup: the gather of plan entries is latency bound, and
@@ -891,10 +903,10 @@ hipError_t launch_grid_half_rows(const GatherArgs& g, const int* row_start, cons
     const int rb = 2;
     const int64_t n_rb = (n / 2 + 1 + rb - 1) / rb;
     const int64_t blocks = n_rb * n_img;
-    const size_t lds = sizeof(float2) * (size_t)rb * n;
+    const size_t lds = (sizeof(float2) + (uniform ? sizeof(unsigned) : 0)) * (size_t)rb * n;
     if (!lds_fits(lds) || blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_grid_half_rows, dim3((unsigned)blocks), dim3(kTileThreads), lds, stream, g,
-                       row_start, entries, n, rb, img0, (float2*)half, isums);
+                       row_start, entries, n, rb, img0, (float2*)half, isums, uniform ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -1159,19 +1171,32 @@ k_grid_rows_fft(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     if ((int)threadIdx.x <= nr) rs[threadIdx.x] = rsg[r0 + threadIdx.x];
     __syncthreads();
     if (rs[nr] == rs[0]) return;  // uniform: no entries in this tile
+    // "U" weighting: a count per tile cell after the team scratch
+    unsigned* cnt = (unsigned*)(tile + (size_t)rb * n + (size_t)teams * n);
+    const bool uniform = sd.uniform != 0;
     for (int i = threadIdx.x; i < nr * n; i += blockDim.x) tile[i] = make_float2(0.f, 0.f);
+    if (uniform)
+        for (int i = threadIdx.x; i < nr * n; i += blockDim.x) cnt[i] = 0u;
     __syncthreads();
     const int n_ok = gather_range<4>(g, img, ch, ent, rs[0], rs[nr], (int)threadIdx.x,
                                      (int)blockDim.x, [&](int e, int u_idx, float2 v) {
         const int r = row_of_entry(rs, nr, e);
         atomicAdd(&tile[r * n + u_idx].x, v.x);
         atomicAdd(&tile[r * n + u_idx].y, v.y);
+        if (uniform) atomicAdd(&cnt[r * n + u_idx], 1u);
         const int row = r0 + r;  // an entry of rows 1 .. N/2-1 stands for its mirror too
         return (row == 0 || row == h) ? 1 : 2;
     });
     const int tot = block_sum_int(n_ok, red);
     if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();
+    if (uniform) {  // grid / count per cell (src/pacer_imager.cpp:421-428), 0 / 0 = NaN
+        for (int i = threadIdx.x; i < nr * n; i += blockDim.x) {
+            const float k = (float)cnt[i];
+            tile[i] = make_float2(tile[i].x / k, tile[i].y / k);
+        }
+        __syncthreads();
+    }
     const int rounds = kBlockTeam ? nr : (nr + teams - 1) / teams;
     for (int k = 0; k < rounds; k++) {
         const int r = kBlockTeam ? k : k * teams + t_id;
@@ -1213,10 +1238,12 @@ k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     const int li = (int)(blockIdx.x / n_cb);
     const int64_t gi = gi0 + li;
     const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    if (!sd.images) return;  // a set imaged for its average only (uniform per workgroup)
     const int img = (int)(gi - sd.img_base);
     const uint32_t count = sd.isums[img];
     if (cbk == 0 && threadIdx.x == 0 && sd.count_sums) sd.count_sums[img] = (double)count;
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (n + 1);
+    const float empty = sd.uniform ? __builtin_nanf("") : 0.f;  // rows without entries
     for (int i = threadIdx.x; i <= h; i += blockDim.x) nonempty[i] = rsg[i + 1] != rsg[i];
     __syncthreads();
 
@@ -1230,7 +1257,7 @@ k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
         for (int k = 0; k < 4; k++) {
             const int i = i0 + k * nt;
             const int v = i / pairs, t = i % pairs;
-            p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            p[k] = make_float4(empty, empty, empty, empty);
             if (i < stage && nonempty[v]) p[k] = *(const float4*)&src[(int64_t)v * n + 2 * t];
         }
 #pragma unroll
@@ -1408,10 +1435,17 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
     // gather of plan entries is latency bound: small tiles keep more
     // workgroups in flight (4 rows measured fastest against 2 to 16 at 180^2).
     const int64_t want = (int64_t)(n / 2 + 1) * n_img / 2048;
-    const int rb = (int)std::max<int64_t>(
+    int rb = (int)std::max<int64_t>(
         1, std::min<int64_t>({(int64_t)grid_rows_tile_rows(n), want, 4}));
+    // uniform weighting keeps a count per tile cell beside the tile
+    const bool uniform = first.uniform != 0;
+    auto lds_of = [&](int r) {
+        return sizeof(float2) * ((size_t)r * n + (size_t)teams * n) +
+               (uniform ? sizeof(unsigned) * (size_t)r * n : 0);
+    };
+    while (rb > 1 && !lds_fits(lds_of(rb))) rb--;
     const int n_rb = (n / 2 + 1 + rb - 1) / rb;
-    const size_t lds = sizeof(float2) * ((size_t)rb * n + (size_t)teams * n);
+    const size_t lds = lds_of(rb);
     if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)((int64_t)n_rb * n_img));
     const bool multi = n_sets > 1;

@@ -76,6 +76,8 @@ struct SetDesc {
                               // after the pol combination and the phase rotation,
                               // NaN where it is not gridded (k_vis_compact); the row
                               // tiles then gather these instead of the records
+    int uniform;              // "U" weighting: cells divided by their counts, so
+                              // empty cells (and plan rows without entries) are NaN
 };
 
 // The set of group image gi, by value.  Kernels are built twice: for a
@@ -194,7 +196,7 @@ enum { kDftBf16 = 0, kDftFp16 = 1 };
 int dft_inter_stride(int n);
 hipError_t launch_grid_half_rows(const GatherArgs& g, const int* row_start, const uint32_t* entries,
                                  int n, int64_t n_img, int64_t img0, float* half,
-                                 uint32_t* isums, hipStream_t stream);
+                                 uint32_t* isums, hipStream_t stream, bool uniform = false);
 hipError_t launch_dft_rows(const float* half, const uint16_t* planes1, float* inter, int n,
                            int64_t n_img, int prec, hipStream_t stream);
 hipError_t launch_dft_cols(const float* inter, const uint16_t* planes2, float* images, int n,
@@ -215,12 +217,26 @@ int64_t batch_inter_bytes(int n);
 int batch_row_variants(int n, int* tiles, int max_tiles);
 int batch_col_variants(int n, int* tiles, int max_tiles);
 hipError_t set_batch_kernel_attributes();
+// uniform: the "U" weighting (grid / count per cell before the transform),
+// with the side's fixed uniform row tile (batch_uniform_row_tile) instead of
+// `tile`.
 hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                 int n_sets, int n, int tile, int64_t gi0, int n_img, float* inter,
-                                const float* tw, hipStream_t stream);
+                                const float* tw, hipStream_t stream, bool uniform = false);
+int batch_uniform_row_tile(int n);
 hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
                            int n_sets, int n, int tile, int64_t gi0, int n_img,
                            const float* inter, const float* tw, hipStream_t stream);
+// Mean of a set's images fused into the batched path: adds the set's images
+// img0 .. img0 + n_img - 1 (row-pass buffer slots li0 ..) into acc, each scaled
+// by (float)(1 / its count sum), in image order (accumulate = false starts
+// from zero and writes meta: [0] = sd.n_img, [1 .. N+1] a row start with every
+// plan row non-empty); then a second pass over acc with a descriptor whose
+// isums / row_start point at meta and images at the average gives the mean
+// image.  Also writes the count sums of the images when sd.images is null.
+hipError_t launch_average_rows(const SetDesc& sd, int64_t img0, int n_img, int64_t li0,
+                               const float* inter, float* acc, bool accumulate, unsigned* meta,
+                               hipStream_t stream);
 // Fills sd.compact for all n_img images of the set (from the visibilities and
 // the set's own inputs only: it may run beside the plan).
 hipError_t launch_vis_compact(const SetDesc& sd, int64_t n_img, hipStream_t stream);

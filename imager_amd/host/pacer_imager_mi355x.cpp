@@ -12,18 +12,6 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-bool file_exists(const std::string& f) {
-    std::ifstream s(f);
-    return s.good();
-}
-
-bool is_fits(const std::string& f) {
-    std::ifstream s(f, std::ios::binary);
-    char head[9] = {0};
-    s.read(head, 8);
-    return std::strncmp(head, "SIMPLE  ", 8) == 0;
-}
-
 // 80-character FITS header card.
 std::string card(const std::string& key, const std::string& value, bool quoted = false) {
     char buf[81];
@@ -46,72 +34,8 @@ std::string num(double v) {
 
 }  // namespace
 
-int CPacerImagerMI355X::m_ImagerDebugLevel = IMAGER_INFO_LEVEL;
-int CPacerImagerMI355X::m_SaveFilesLevel = 0;
-bool CPacerImagerMI355X::m_bPrintImageStatistics = false;
-
 #define PRINTF_INFO \
-    if (CPacerImagerMI355X::m_ImagerDebugLevel >= IMAGER_INFO_LEVEL) std::printf
-#define PRINTF_WARNING \
-    if (CPacerImagerMI355X::m_ImagerDebugLevel >= IMAGER_WARNING_LEVEL) std::printf
-
-// ------------------------------------------------------------- utilities
-
-void fft_shift(std::complex<float>* image, size_t image_x_side, size_t image_y_side) {
-    blink_ctx* ctx = nullptr;
-    if (blink_create(0, &ctx) != BLINK_OK) throw std::runtime_error("fft_shift: no GPU context");
-    const size_t n = image_x_side * image_y_side;
-    void* d = nullptr;
-    hipError_t e = hipMalloc(&d, sizeof(std::complex<float>) * n);
-    if (e == hipSuccess) e = hipMemcpy(d, image, sizeof(std::complex<float>) * n, hipMemcpyHostToDevice);
-    int rc = e == hipSuccess ? blink_fft_shift_and_norm(ctx, (float*)d, (int)image_x_side,
-                                                         (int)image_y_side, 1, nullptr, 1, nullptr)
-                             : BLINK_ERR_HIP;
-    if (rc == BLINK_OK) e = hipMemcpy(image, d, sizeof(std::complex<float>) * n, hipMemcpyDeviceToHost);
-    std::string msg = rc == BLINK_OK ? "" : blink_last_error(ctx);
-    if (d) (void)hipFree(d);
-    blink_destroy(ctx);
-    if (rc != BLINK_OK) throw std::runtime_error("fft_shift: " + msg);
-    hip_check(e, "fft_shift");
-}
-
-int CAntennaList::ReadAntennaPositions(const char* filename, bool /*bConvertToXYZ*/) {
-    std::ifstream f(filename ? filename : "");
-    if (!f) {
-        std::printf("ERROR: filename %s does not exist\n", filename ? filename : "(null)");
-        return -1;
-    }
-    clear();
-    std::string line;
-    while (std::getline(f, line)) {
-        if (line.empty() || line[0] == '#') continue;
-        std::istringstream is(line);
-        InputMapping in;
-        if (!(is >> in.szAntName >> in.x >> in.y >> in.z)) continue;
-        push_back(in);
-    }
-    return (int)size();
-}
-
-int CAntennaList::CalculateUVW(MemoryBuffer<float>& u, MemoryBuffer<float>& v,
-                               MemoryBuffer<float>& w, bool /*bIncludeAutos*/) const {
-    const size_t n_ant = size();
-    if (n_ant == 0) return 0;
-    const size_t n_bl = n_ant * (n_ant + 1) / 2;
-    if (u.size() != n_bl) u.allocate(n_bl);
-    if (v.size() != n_bl) v.allocate(n_bl);
-    if (w.size() != n_bl) w.allocate(n_bl);
-    for (size_t bl = 0; bl < n_bl; bl++) {
-        const unsigned a1 = (unsigned)(-0.5 + std::sqrt(0.25 + 2.0 * (double)bl));
-        const unsigned a2 = (unsigned)(bl - ((size_t)(a1 + 1) * a1) / 2);
-        const InputMapping& p = (*this)[a1];
-        const InputMapping& q = (*this)[a2];
-        u[bl] = (float)(-(p.x - q.x));
-        v[bl] = (float)(-(p.y - q.y));
-        w[bl] = (float)(-(p.z - q.z));
-    }
-    return (int)n_bl;
-}
+    if (CPacerImager::m_ImagerDebugLevel >= IMAGER_INFO_LEVEL) std::printf
 
 // ----------------------------------------------------------------- class
 
@@ -120,26 +44,10 @@ CPacerImagerMI355X::CPacerImagerMI355X(const std::string metadata_file, int n_pi
                                        bool average_images, Polarization pol_to_image,
                                        float oversampling_factor, double min_uv, double max_uv,
                                        const char* weighting, int device)
-    : average_images(average_images),
-      oversampling_factor(oversampling_factor),
-      n_pixels(n_pixels),
-      min_uv(min_uv),
-      max_uv(max_uv),
-      weighting(weighting ? weighting : ""),
-      pol_to_image(pol_to_image) {
+    : CPacerImager(metadata_file, n_pixels, flagged_antennas, average_images, pol_to_image,
+                   oversampling_factor, min_uv, max_uv, weighting) {
     const int rc = blink_create(device, &ctx_);
     if (rc != BLINK_OK) throw std::runtime_error("CPacerImagerMI355X: cannot create a GPU context");
-    if (!metadata_file.empty() && file_exists(metadata_file)) {
-        if (is_fits(metadata_file)) {
-            PRINTF_WARNING("WARNING : %s is a FITS metadata file; set the array with "
-                           "SetAntennaPositions() or SetUVW()\n", metadata_file.c_str());
-        } else {
-            PRINTF_INFO("INFO : reading antenna positions from %s\n", metadata_file.c_str());
-            m_MetaData.m_AntennaPositions.ReadAntennaPositions(metadata_file.c_str());
-        }
-    }
-    m_FlaggedAntennas = flagged_antennas;
-    UpdateFlags();
 }
 
 CPacerImagerMI355X::~CPacerImagerMI355X() {
@@ -155,89 +63,6 @@ void CPacerImagerMI355X::check(int rc) const {
     if (rc == BLINK_ERR_INVALID_ARGUMENT || rc == BLINK_ERR_UNSUPPORTED)
         throw std::invalid_argument(msg);
     throw std::runtime_error(msg);
-}
-
-void CPacerImagerMI355X::SetDebugLevel(int debug_level) { m_ImagerDebugLevel = debug_level; }
-void CPacerImagerMI355X::SetFileLevel(int filesave_level) { m_SaveFilesLevel = filesave_level; }
-
-void CPacerImagerMI355X::SetAntennaPositions(const std::vector<double>& x,
-                                             const std::vector<double>& y,
-                                             const std::vector<double>& z,
-                                             const std::vector<double>& cable_len_delta) {
-    if (x.size() != y.size() || x.size() != z.size() ||
-        (!cable_len_delta.empty() && cable_len_delta.size() != x.size()))
-        throw std::invalid_argument("SetAntennaPositions: arrays of different lengths");
-    m_MetaData.m_AntennaPositions.clear();
-    for (size_t i = 0; i < x.size(); i++) {
-        InputMapping in;
-        in.x = x[i];
-        in.y = y[i];
-        in.z = z[i];
-        in.cableLenDelta = cable_len_delta.empty() ? 0.0 : cable_len_delta[i];
-        m_MetaData.m_AntennaPositions.push_back(in);
-    }
-    uvw_given_ = false;
-    cable_lengths = MemoryBuffer<double>();
-    UpdateFlags();
-}
-
-void CPacerImagerMI355X::SetUVW(const float* u, const float* v, const float* w, size_t n_bl) {
-    u_cpu.allocate(n_bl);
-    v_cpu.allocate(n_bl);
-    w_cpu.allocate(n_bl);
-    std::memcpy(u_cpu.data(), u, sizeof(float) * n_bl);
-    std::memcpy(v_cpu.data(), v, sizeof(float) * n_bl);
-    std::memcpy(w_cpu.data(), w, sizeof(float) * n_bl);
-    uvw_given_ = true;
-    m_Baselines = (int)n_bl;
-}
-
-// src/pacer_imager.cpp:60-86: flagged = either antenna in the list, or an auto.
-int CPacerImagerMI355X::UpdateFlags() {
-    int n_ant = (int)m_MetaData.m_AntennaPositions.size();
-    if (n_ant == 0 && uvw_given_) n_ant = (int)std::lround((-1.0 + std::sqrt(1.0 + 8.0 * m_Baselines)) / 2.0);
-    const int n_baselines = (n_ant * (n_ant + 1)) / 2;
-    m_FlaggedBaselines.assign(n_baselines, 0);
-    for (int bl = 0; bl < n_baselines; ++bl) {
-        const unsigned a1 = (unsigned)(-0.5 + std::sqrt(0.25 + 2.0 * bl));
-        const unsigned a2 = bl - ((a1 + 1) * a1) / 2u;
-        bool listed = false;
-        for (int a : m_FlaggedAntennas)
-            if (a == (int)a1 || a == (int)a2) {
-                listed = true;
-                break;
-            }
-        m_FlaggedBaselines[bl] = (listed || a1 == a2) ? 1 : 0;
-    }
-    return 0;
-}
-
-// src/pacer_imager.cpp:210-229
-bool CPacerImagerMI355X::CalculateUVW() {
-    if (!uvw_given_)
-        m_Baselines = m_MetaData.m_AntennaPositions.CalculateUVW(u_cpu, v_cpu, w_cpu, m_bIncludeAutos);
-    u_max = -std::numeric_limits<double>::infinity();
-    v_max = -std::numeric_limits<double>::infinity();
-    for (size_t i = 0; i < u_cpu.size() && i < m_FlaggedBaselines.size(); i++) {
-        if (m_FlaggedBaselines[i] == 0) {
-            if (u_max < std::abs(u_cpu[i])) u_max = std::abs(u_cpu[i]);
-            if (v_max < std::abs(v_cpu[i])) v_max = std::abs(v_cpu[i]);
-        }
-    }
-    u_min = -u_max;
-    v_min = -v_max;
-    return m_Baselines > 0;
-}
-
-// src/pacer_imager.cpp:528-555
-double CPacerImagerMI355X::get_frequency_hz(const Visibilities& vis, int fine_channel,
-                                            bool cotter_compatible) {
-    const double fine_ch_bw = vis.obsInfo.frequencyResolution * vis.nAveragedChannels;
-    const double coarse_ch_bw = vis.obsInfo.coarseChannelBandwidth;
-    const double coarse_central = vis.obsInfo.coarseChannel * coarse_ch_bw;
-    double mhz = coarse_central - coarse_ch_bw / 2.00 + fine_ch_bw * fine_channel;
-    if (!cotter_compatible) mhz += fine_ch_bw / 2.00;
-    return mhz * 1e6;
 }
 
 void CPacerImagerMI355X::UpdateAntennaFlags(int n_ant) {
@@ -269,42 +94,6 @@ void CPacerImagerMI355X::upload_inputs(const Visibilities& xcorr) {
         frequencies_gpu.allocate(frequencies.size(), MemoryType::DEVICE);
     hip_check(hipMemcpy(frequencies_gpu.data(), frequencies.data(), sizeof(double) * frequencies.size(),
                         hipMemcpyHostToDevice), "frequencies H2D");
-}
-
-// src/pacer_imager.cpp:445-526 (fix_metafits and the metafits UVW are outside
-// this library; see the header).
-void CPacerImagerMI355X::grid(Visibilities& xcorr) {
-    UpdateFlags();
-    CalculateUVW();
-    if (frequencies.size() != xcorr.nFrequencies) frequencies.allocate(xcorr.nFrequencies);
-    for (size_t ch = 0; ch < xcorr.nFrequencies; ch++)
-        frequencies[ch] = m_fFrequencyMHz > 0 ? m_fFrequencyMHz * 1e6
-                                              : get_frequency_hz(xcorr, (int)ch, true);
-    if (apply_geom_correction) ApplyGeometricCorrections(xcorr, w_cpu, frequencies);
-    if (apply_cable_correction) {
-        const size_t n_ant = xcorr.obsInfo.nAntennas;
-        if (cable_lengths.size() != n_ant) {
-            cable_lengths.allocate(n_ant);
-            for (size_t a = 0; a < n_ant; a++)
-                cable_lengths[a] = a < m_MetaData.m_AntennaPositions.size()
-                                       ? m_MetaData.m_AntennaPositions[a].cableLenDelta
-                                       : 0.0;
-        }
-        ApplyCableCorrections(xcorr, cable_lengths, frequencies);
-    }
-    n_gridded_intervals = xcorr.integration_intervals();
-    n_gridded_channels = xcorr.nFrequencies;
-    delta_u = oversampling_factor * (u_max) / n_pixels;
-    delta_v = oversampling_factor * (v_max) / n_pixels;
-    pixsize_in_radians = 1.00 / (oversampling_factor * u_max);
-    PRINTF_INFO("delta_u = %.8f (u_max = %.8f), delta_v = %.8f (v_max = %.8f)\n", delta_u, u_max,
-                delta_v, v_max);
-    gridding(xcorr);
-}
-
-Images CPacerImagerMI355X::run(Visibilities& xcorr) {
-    grid(xcorr);
-    return image(xcorr.obsInfo);
 }
 
 void CPacerImagerMI355X::gridding(Visibilities& xcorr) {
@@ -403,40 +192,38 @@ void CPacerImagerMI355X::ApplyCableCorrections(Visibilities& xcorr, MemoryBuffer
 }
 
 Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
-    if (std::strcmp(weighting, "U") == 0)
-        throw std::invalid_argument("run_batched images with natural weighting; use run() for \"U\"");
     if (m_nConvolvingKernelSize > 0)
         throw std::invalid_argument("run_batched grids to the nearest cell; use run() for "
                                     "convolutional gridding");
     UpdateFlags();
     CalculateUVW();
-    if (frequencies.size() != xcorr.nFrequencies) frequencies.allocate(xcorr.nFrequencies);
-    for (size_t ch = 0; ch < xcorr.nFrequencies; ch++)
-        frequencies[ch] = m_fFrequencyMHz > 0 ? m_fFrequencyMHz * 1e6
-                                              : get_frequency_hz(xcorr, (int)ch, true);
+    fill_frequencies(xcorr);
     const size_t n_ant = xcorr.obsInfo.nAntennas;
-    if (cable_lengths.size() != n_ant) {
-        cable_lengths.allocate(n_ant);
-        for (size_t a = 0; a < n_ant; a++)
-            cable_lengths[a] = a < m_MetaData.m_AntennaPositions.size()
-                                   ? m_MetaData.m_AntennaPositions[a].cableLenDelta
-                                   : 0.0;
-    }
-    n_gridded_intervals = xcorr.integration_intervals();
-    n_gridded_channels = xcorr.nFrequencies;
-    delta_u = oversampling_factor * (u_max) / n_pixels;
-    delta_v = oversampling_factor * (v_max) / n_pixels;
-    pixsize_in_radians = 1.00 / (oversampling_factor * u_max);
+    fill_cable_lengths(n_ant);
+    set_grid_scalars(xcorr);
     upload_inputs(xcorr);
     if (cable_lengths_gpu.size() != n_ant) cable_lengths_gpu.allocate(n_ant, MemoryType::DEVICE);
     hip_check(hipMemcpy(cable_lengths_gpu.data(), cable_lengths.data(), sizeof(double) * n_ant,
                         hipMemcpyHostToDevice), "cables H2D");
-    xcorr.to_gpu();
+    // the visibilities are read in place on the device; host visibilities are
+    // copied (run() moves them to the device, as the reference's GPU path does)
+    const float* vis = (const float*)xcorr.data();
+    if (!xcorr.on_gpu()) {
+        if (vis_gpu_.size() != xcorr.size()) vis_gpu_.allocate(xcorr.size(), MemoryType::DEVICE);
+        hip_check(hipMemcpy(vis_gpu_.data(), xcorr.data(), sizeof(std::complex<float>) * xcorr.size(),
+                            hipMemcpyHostToDevice), "visibilities H2D");
+        vis = (const float*)vis_gpu_.data();
+    }
     const size_t n_img = n_gridded_channels * n_gridded_intervals;
-    MemoryBuffer<std::complex<float>> images_buffer((size_t)n_pixels * n_pixels * n_img,
+    const size_t image_size = (size_t)n_pixels * n_pixels;
+    // with average_images only the mean is formed, unless a DFT transform
+    // needs the images
+    const bool keep_images = !average_images || m_nBatchTransform != BLINK_TRANSFORM_FFT;
+    MemoryBuffer<std::complex<float>> images_buffer(keep_images ? image_size * n_img : 0,
                                                     MemoryType::DEVICE);
+    MemoryBuffer<std::complex<float>> avg(average_images ? image_size : 0, MemoryType::DEVICE);
     blink_image_set st{};
-    st.vis = (const float*)xcorr.data();
+    st.vis = vis;
     st.n_ant = (int)n_ant;
     st.n_ch = (int)n_gridded_channels;
     st.n_t = (int)n_gridded_intervals;
@@ -453,22 +240,18 @@ Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
     st.min_uv = min_uv;
     st.n_pixels = n_pixels;
     st.pol = (int)pol_to_image;
-    st.images = (float*)images_buffer.data();
+    st.images = keep_images ? (float*)images_buffer.data() : nullptr;
     st.count_sums = nullptr;
     st.transform = m_nBatchTransform;
+    st.weighting = std::strcmp(weighting, "U") == 0 ? BLINK_WEIGHTING_UNIFORM
+                                                    : BLINK_WEIGHTING_NATURAL;
+    st.average = average_images ? (float*)avg.data() : nullptr;
     check(blink_image_batch(ctx_, &st, 1, nullptr));
     const double pixscale_ra = (1.00 / (oversampling_factor * u_max)) * (180.00 / kPi);
     const double pixscale_dec = (1.00 / (oversampling_factor * v_max)) * (180.00 / kPi);
-    if (average_images) {
-        // the average of every image, as image() returns it (complex, the
-        // GPU path's averaging, src/gpu/gpu_utils.cpp:143-161)
-        MemoryBuffer<std::complex<float>> avg((size_t)n_pixels * n_pixels, MemoryType::DEVICE);
-        check(blink_image_averaging(ctx_, (const float*)images_buffer.data(),
-                                    (int64_t)n_pixels * n_pixels, (int)n_img, 0,
-                                    (float*)avg.data(), nullptr));
+    if (average_images)  // the mean image, as image() returns it (src/pacer_imager.cpp:203-205)
         return Images{std::move(avg), xcorr.obsInfo, 1, 1, (unsigned)n_pixels,
                       m_MetaData.raHrs * 15.0, m_MetaData.decDegs, pixscale_ra, pixscale_dec};
-    }
     return Images{std::move(images_buffer), xcorr.obsInfo, (unsigned)n_gridded_intervals,
                   (unsigned)n_gridded_channels, (unsigned)n_pixels, m_MetaData.raHrs * 15.0,
                   m_MetaData.decDegs, pixscale_ra, pixscale_dec};

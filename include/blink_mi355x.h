@@ -25,9 +25,11 @@
  *   - A context owns its scratch memory, FFT twiddle tables and captured
  *     graphs.  Contexts are not thread-safe: use one per host thread.  Calls
  *     on one context share its scratch, so they must not run concurrently on
- *     different streams (order them, or use one context per stream); a
- *     blink_image_batch call with several grid sides also uses the context's
- *     own second stream, forked from and joined back to `stream`.
+ *     different streams (order them, or use one context per stream).
+ *     blink_image_batch also uses the context's own second stream, forked
+ *     from and joined back to `stream` within the call: for the visibility
+ *     compaction of sparse sets on sides up to 2048 (beside the gridding
+ *     plan), and, in a call with several grid sides, for the smaller sides.
  */
 #ifndef BLINK_MI355X_H
 #define BLINK_MI355X_H
@@ -250,17 +252,32 @@ typedef struct blink_image_set {
     float* images;           /* [n_t][n_ch][N][N] complex64 output, 16-byte aligned.  The
                                 gridded plane is Hermitian, so the images are real and every
                                 transform writes imaginary parts 0 (the reference's FFT leaves
-                                FP32 rounding noise there) */
+                                FP32 rounding noise there).  May be NULL when `average` is
+                                given (FFT transform): the images are then not formed */
     double* count_sums;      /* optional [n_t*n_ch] output: sum of counts per image */
     int transform;           /* blink_transform: how the 2-D transform is computed */
+    int weighting;           /* blink_weighting: NATURAL, or UNIFORM, the reference's "U"
+                                (src/pacer_imager.cpp:421-428): every cell divided by its
+                                count before the transform, so an image with any empty cell
+                                is NaN everywhere, as the reference's FFT makes it */
+    float* average;          /* optional [N][N] complex64 output: the mean of the set's
+                                n_t x n_ch images (image_averaging_gpu,
+                                src/gpu/gpu_utils.cpp:143-161; the images being real, it is
+                                also image_averaging_cpu's real-part mean, src/utils.cpp:4-22).
+                                With the FFT transform it is formed from the row-pass buffers
+                                without the images; with a DFT transform `images` is required */
 } blink_image_set;
 
-/* Visibilities -> normalised, shifted dirty images for every set, with the
- * phase corrections, gridding, FFT, shift and normalisation fused into a
- * few kernels per set, the whole call captured once as a hipGraph and
- * replayed when it is made again with identical arguments.  Results equal
+/* Visibilities -> normalised, shifted dirty images (and / or their mean) for
+ * every set, with the phase corrections, gridding, weighting, FFT, shift and
+ * normalisation fused into a few kernels per set, the whole call captured once as a hipGraph and
+ * replayed when it is made again with the same set fields (compared field
+ * by field).  A call that differs only in the vis, images, count_sums or
+ * average addresses refreshes the captured descriptors and updates the instantiated
+ * graph (hipGraphExecUpdate) instead of instantiating a new one; it first
+ * waits for that graph's previous replay.  Results equal
  * apply_*_corrections + gridding + image of the reference (CPacerImager::run,
- * src/pacer_imager.cpp:440-526) for natural weighting. */
+ * src/pacer_imager.cpp:440-526, with average_images). */
 BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
                                 void* stream);
 

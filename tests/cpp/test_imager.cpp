@@ -1,6 +1,7 @@
 // The shape of the reference's tests/imager_test.cpp for CPacerImagerMI355X:
 // the fft_shift known-answer tests, then imaging runs checked against the CPU
 // oracle (oracle/blink_oracle.c) on identical synthetic inputs.
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -162,6 +163,65 @@ void test_imager_batched_and_pols() {
     }
 }
 
+// run_batched() with average_images (only the mean image is formed) against
+// the oracle's images and their complex mean (image_averaging_gpu,
+// src/gpu/gpu_utils.cpp:143-161), and with weighting "U" on an array that
+// leaves cells empty: the reference's 0/0 cells make every pixel NaN
+// (src/pacer_imager.cpp:421-428).
+void test_imager_batched_average_and_uniform() {
+    const int n_ch = 3, n_t = 2, N = 256;
+    Array a = make_array(48, 17.5, 41);
+    std::vector<double> freq{159.375e6, 160.0e6, 160.625e6};
+    ObservationInfo info;
+    Visibilities xcorr = make_vis(a, n_ch, n_t, freq, 8, info);
+    std::vector<float> vis0((float*)xcorr.data(), (float*)xcorr.data() + 2 * xcorr.size());
+    const size_t gs = (size_t)N * N, n_img = (size_t)n_ch * n_t;
+    const int n_bl = a.n_ant * (a.n_ant + 1) / 2;
+    auto fresh = [&]() {
+        Visibilities copy(MemoryBuffer<std::complex<float>>(xcorr.size()), info, 1, 1);
+        std::memcpy((void*)copy.data(), vis0.data(), sizeof(float) * vis0.size());
+        copy.obsInfo.frequencyResolution = 0.625;
+        copy.obsInfo.coarseChannelBandwidth = 1.25;
+        copy.obsInfo.coarseChannel = 128;  // 159.375 MHz + 0.625 ch
+        return copy;
+    };
+    for (int transform : {BLINK_TRANSFORM_FFT, BLINK_TRANSFORM_DFT_FP16X3}) {
+        CPacerImagerMI355X imager("", N, {2}, true, Polarization::XX, 2.0f);
+        imager.SetAntennaPositions(a.x, a.y, a.z, a.cable);
+        imager.m_nBatchTransform = transform;
+        Visibilities copy = fresh();
+        Images avg = imager.run_batched(copy);
+        if (avg.size() != 1 || avg.side_size != (unsigned)N)
+            throw TestFailed("run_batched with average_images must return one image");
+        std::vector<float> u(n_bl), v(n_bl), w(n_bl);
+        oracle_zenith_uvw(a.n_ant, a.x.data(), a.y.data(), a.z.data(), u.data(), v.data(), w.data());
+        std::vector<int> flags(n_bl);
+        oracle_baseline_flags(a.n_ant, imager.m_FlaggedAntennas.data(),
+                              (int)imager.m_FlaggedAntennas.size(), flags.data());
+        std::vector<float> vis(vis0), grids(2 * gs * n_img), counts(gs * n_img),
+            images(2 * gs * n_img), ref(2 * gs);
+        oracle_run_f32(vis.data(), a.n_ant, n_ch, n_t, u.data(), v.data(), w.data(), flags.data(),
+                       freq.data(), a.cable.data(), 1, 1, imager.delta_u, imager.delta_v,
+                       imager.min_uv, N, ORACLE_POL_XX, grids.data(), counts.data(),
+                       images.data(), 0);
+        oracle_image_averaging_complex(images.data(), gs, n_img, ref.data());
+        avg.to_cpu();
+        const double err = peak_normalised_error(avg.data(), (const std::complex<float>*)ref.data(), gs);
+        std::cout << "run_batched average (transform " << transform
+                  << "): peak-normalised error vs oracle " << err << std::endl;
+        if (!(err < 1e-5)) throw TestFailed("run_batched average differs from the oracle");
+    }
+    CPacerImagerMI355X uni("", N, {}, false, Polarization::XX, 2.0f, -std::numeric_limits<double>::infinity(),
+                           std::numeric_limits<double>::infinity(), "U");
+    uni.SetAntennaPositions(a.x, a.y, a.z, a.cable);
+    Visibilities copy = fresh();
+    Images images = uni.run_batched(copy);
+    images.to_cpu();
+    for (size_t i = 0; i < images.size() * gs; i++)
+        if (!std::isnan(images.data()[i].real()))
+            throw TestFailed("\"U\" with empty cells must give NaN images");
+}
+
 // src/pacer_imager.cpp:135: image() before grid() throws std::runtime_error.
 void test_error_paths() {
     CPacerImagerMI355X imager("", 64, {}, false);
@@ -262,7 +322,7 @@ void test_eda2_correlation_matrices() {
     ObservationInfo info;
     Visibilities xcorr = visibilities_from_correlation_matrices(re, im, nullptr, nullptr, info);
     std::vector<float> vis0((float*)xcorr.data(), (float*)xcorr.data() + 2 * xcorr.size());
-    CAntennaList ants;
+    CAntennaPositions ants;
     if (ants.ReadAntennaPositions((g + "antenna_locations.txt").c_str()) != 256)
         throw TestFailed("antenna_locations.txt must list 256 antennas");
     Array a;
@@ -281,8 +341,9 @@ void test_eda2_correlation_matrices() {
                            "EDA2 matrices, run_batched DFT bf16x3",
                            "EDA2 matrices, run_batched DFT fp16x3"};
     for (int k = 0; k < 4; k++) {
-        CPacerImagerMI355X imager(g + "antenna_locations.txt", 180, {}, false, Polarization::XX,
-                                  2.0f);
+        CPacerImagerMI355X imager("", 180, {}, false, Polarization::XX, 2.0f);
+        imager.m_MetaData.m_AntennaPositions.ReadAntennaPositions((g + "antenna_locations.txt").c_str());
+        imager.UpdateFlags();
         imager.m_fFrequencyMHz = 159.375;
         if (transforms[k] >= 0) imager.m_nBatchTransform = transforms[k];
         Visibilities copy(MemoryBuffer<std::complex<float>>(xcorr.size()), info, 1, 1);
@@ -293,15 +354,95 @@ void test_eda2_correlation_matrices() {
     }
 }
 
+// The command line's set-up, written against the base class as
+// apps/pacer_imager_main.cpp:653-718 writes Initialise(CPacerImager&): read
+// the antenna list, update the flags, zenith UVW when no metadata file is
+// given.
+static void initialise(CPacerImager& imager, const std::string& antenna_file) {
+    const int n = imager.m_MetaData.m_AntennaPositions.ReadAntennaPositions(antenna_file.c_str(),
+                                                                             false);
+    if (n != 256) throw TestFailed("initialise: the antenna list must hold 256 antennas");
+    imager.UpdateFlags();
+    imager.CalculateUVW();
+}
+
+// The command line's flow on the EDA2 correlation matrices
+// (apps/pacer_imager_main.cpp:509-584): the GPU class built with the command
+// line's constructor arguments, set up through a CPacerImager&, corrections
+// switched off, an ObservationInfo{VCS_OBSERVATION_INFO}, the matrices turned
+// into Visibilities, then two rounds of m_nConvolvingKernelSize / m_fFrequencyMHz
+// / run() / to_fits_files() -- every image against the oracle, every FITS file
+// read back.  The metadata readers the command line calls when a metafits
+// file is given throw (this library takes the array from the antenna list).
+void test_command_line_flow() {
+    const std::string g = golden_dir() + "/eda2_chan204/";
+    std::vector<int> flagged{12, 200};
+    CPacerImagerMI355X imager("", 180, flagged, false, Polarization::XX);
+    CPacerImager& base = imager;
+    initialise(base, g + "antenna_locations.txt");
+    if (imager.m_Baselines != 256 * 257 / 2) throw TestFailed("CalculateUVW: wrong baseline count");
+    imager.apply_geom_correction = false;
+    imager.apply_cable_correction = false;
+    FitsImage re = read_fits_image(g + "chan_204_20230205T034618_simul_vis_XX_CorrMatrix_RE.fits");
+    FitsImage im = read_fits_image(g + "chan_204_20230205T034618_simul_vis_XX_CorrMatrix_IM.fits");
+    ObservationInfo obsInfo{VCS_OBSERVATION_INFO};
+    if (obsInfo.coarseChannel != 109 || obsInfo.frequencyResolution != 0.040)
+        throw TestFailed("VCS_OBSERVATION_INFO does not carry the command line's values");
+    Visibilities xcorr = visibilities_from_correlation_matrices(re, im, nullptr, nullptr, obsInfo);
+    std::vector<float> vis0((float*)xcorr.data(), (float*)xcorr.data() + 2 * xcorr.size());
+    Array a;
+    a.n_ant = 256;
+    for (const InputMapping& m : imager.m_MetaData.m_AntennaPositions) {
+        a.x.push_back(m.x);
+        a.y.push_back(m.y);
+        a.z.push_back(m.z);
+        a.cable.push_back(m.cableLenDelta);
+    }
+    const std::string out = "/tmp/blink_cli_flow";
+    if (std::system(("mkdir -p " + out).c_str()) != 0) throw TestFailed("cannot create " + out);
+    for (int i = 0; i < 2; i++) {
+        imager.m_nConvolvingKernelSize = 0;
+        imager.m_fFrequencyMHz = 159.375;
+        Images image = base.run(xcorr);
+        check_against_oracle(imager, image, a, vis0, {159.375e6}, 1, 1, false, false,
+                             ORACLE_POL_XX, 1e-5, i == 0 ? "command line flow, image 0"
+                                                         : "command line flow, image 1");
+        image.to_fits_files(out);
+        FitsImage back = read_fits_image(out + "/dirty_image_t0000_ch0000_real.fits");
+        if (back.nx != 180 || back.ny != 180) throw TestFailed("FITS image of the wrong size");
+        for (size_t k = 0; k < back.data.size(); k += 997)
+            if ((float)back.data[k] != image.data()[k].real())
+                throw TestFailed("FITS image differs from the image");
+    }
+    bool threw = false;
+    try {
+        imager.m_MetaData.ReadMetaData((g + "antenna_locations.txt").c_str());
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    if (!threw) throw TestFailed("ReadMetaData must throw");
+    threw = false;
+    try {
+        CPacerImagerMI355X with_meta(g + "antenna_locations.txt", 180, {});
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    if (!threw) throw TestFailed("a metadata file given to the constructor must throw");
+    if (CObsMetadata::ux2gps(1534699078.0) != 1534699078.0 - 315964783.0)
+        throw TestFailed("ux2gps");
+}
+
 int main() {
     CPacerImagerMI355X::SetDebugLevel(IMAGER_WARNING_LEVEL);
     try {
         RUN_TEST(test_fft_shift_simple);
         RUN_TEST(test_imager_run);
         RUN_TEST(test_imager_batched_and_pols);
+        RUN_TEST(test_imager_batched_average_and_uniform);
         RUN_TEST(test_error_paths);
         RUN_TEST(test_imager_kaiser_bessel);
         RUN_TEST(test_eda2_correlation_matrices);
+        RUN_TEST(test_command_line_flow);
     } catch (std::exception& ex) {
         std::cerr << ex.what() << std::endl;
         return 1;

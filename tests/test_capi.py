@@ -67,9 +67,11 @@ def test_image_set_struct_matches_header():
 #include <stddef.h>
 #include "blink_mi355x.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(blink_image_set), offsetof(blink_image_set, corrections),
-         offsetof(blink_image_set, delta_u), offsetof(blink_image_set, images),
-         offsetof(blink_image_set, count_sums), offsetof(blink_image_set, transform));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(blink_image_set),
+         offsetof(blink_image_set, corrections), offsetof(blink_image_set, delta_u),
+         offsetof(blink_image_set, images), offsetof(blink_image_set, count_sums),
+         offsetof(blink_image_set, transform), offsetof(blink_image_set, weighting),
+         offsetof(blink_image_set, average));
   return 0;
 }
 '''
@@ -82,4 +84,5 @@ int main(void) {
                                         check=True).stdout.split()))
     S = capi.ImageSet
     assert vals == [ctypes.sizeof(S), S.corrections.offset, S.delta_u.offset, S.images.offset,
-                    S.count_sums.offset, S.transform.offset]
+                    S.count_sums.offset, S.transform.offset, S.weighting.offset,
+                    S.average.offset]
