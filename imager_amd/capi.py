@@ -42,7 +42,8 @@ EXPORTED_SYMBOLS = (
     "blink_apply_geometric_corrections", "blink_apply_cable_corrections", "blink_gridding",
     "blink_gridding_kernel", "blink_zenith_uvw", "blink_baseline_flags", "blink_uv_max",
     "blink_uv_cells", "blink_fft2d_c2c", "blink_vector_sum", "blink_fft_shift_and_norm",
-    "blink_image", "blink_image_averaging", "blink_image_batch", "blink_release_cache",
+    "blink_image", "blink_image_averaging", "blink_image_batch", "blink_image_batch_noalloc",
+    "blink_release_cache",
     "blink_profile_enable", "blink_profile_get", "blink_profile_kernels", "blink_profile_reset",
     "blink_batch_tiles", "blink_set_batch_tiles",
 )
@@ -113,6 +114,7 @@ def load():
         "blink_image": (i, [vp, vp, vp, i, i, vp, i, vp]),
         "blink_image_averaging": (i, [vp, vp, i64, i, i, vp, vp]),
         "blink_image_batch": (i, [vp, ctypes.POINTER(ImageSet), i, vp]),
+        "blink_image_batch_noalloc": (i, [vp, ctypes.POINTER(ImageSet), i, vp]),
         "blink_release_cache": (i, [vp]),
         "blink_profile_enable": (i, [vp, i]),
         "blink_profile_get": (i, [vp, ctypes.c_char_p, ctypes.POINTER(i64),
@@ -279,6 +281,12 @@ class Context:
     def image_batch(self, sets, stream=None):
         arr = (ImageSet * len(sets))(*sets)
         self._check(load().blink_image_batch(self._h, arr, len(sets), _stream(stream)))
+
+    def image_batch_noalloc(self, sets, stream=None):
+        """blink_image_batch_noalloc: the batched call with every scratch buffer
+        in the library's fixed device region (no allocation)."""
+        arr = (ImageSet * len(sets))(*sets)
+        self._check(load().blink_image_batch_noalloc(self._h, arr, len(sets), _stream(stream)))
 
     def release_cache(self):
         self._check(load().blink_release_cache(self._h))

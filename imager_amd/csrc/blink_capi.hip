@@ -18,6 +18,12 @@
 
 using namespace blink;
 
+// The fixed device region of blink_image_batch_noalloc.  In host code its
+// name stands for the array, not for its device address: host code uses the
+// address hipGetSymbolAddress gives each context at blink_create
+// (blink_ctx::noalloc_base), never the name.
+static __device__ __align__(256) unsigned char scratch[BLINK_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 struct DeviceBuffer {
@@ -66,6 +72,13 @@ struct blink_ctx {
     hipEvent_t fork_event = nullptr, join_event = nullptr;
     // the visibility compaction runs on aux_stream beside the plan
     hipEvent_t compact_fork = nullptr, compact_join = nullptr;
+    // blink_image_batch_noalloc: the device address of this GPU's fixed
+    // region (null when it could not be found), the host source of the last
+    // descriptor copy and an event recorded after that copy
+    char* noalloc_base = nullptr;
+    hipError_t noalloc_status = hipSuccess;
+    std::vector<unsigned char> noalloc_blob;
+    hipEvent_t noalloc_copied = nullptr;
     bool use_graphs = true;
     bool autotune = true;
     // tile choice of the batched kernels per grid side: {row tile, column block}
@@ -179,7 +192,7 @@ void destroy_streams(blink_ctx* ctx) {
         *s = nullptr;
     }
     for (hipEvent_t* e : {&ctx->fork_event, &ctx->join_event, &ctx->compact_fork,
-                          &ctx->compact_join}) {
+                          &ctx->compact_join, &ctx->noalloc_copied}) {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
     }
@@ -237,6 +250,14 @@ int twiddles(blink_ctx* ctx, int n, const float** out) {
     ctx->twiddles[n] = d;
     *out = d;
     return BLINK_OK;
+}
+
+// Grid sides whose FFT tables blink_create builds: the sides the
+// no-allocation entry point images.
+constexpr int kPreparedSides[] = {128, 180, 256, 512, 1024, 2048, 4096};
+static bool prepared_side(int n) {
+    return std::find(std::begin(kPreparedSides), std::end(kPreparedSides), n) !=
+           std::end(kPreparedSides);
 }
 
 static bool is_dft(int transform) {
@@ -357,10 +378,21 @@ BLINK_API int blink_create(int device, blink_ctx** out) {
         hipEventCreateWithFlags(&ctx->fork_event, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->compact_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->compact_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->compact_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->noalloc_copied, hipEventDisableTiming) != hipSuccess) {
         destroy_streams(ctx);
         delete ctx;
         return BLINK_ERR_HIP;
+    }
+    // this GPU's copy of the no-allocation region (its own address per device;
+    // the device is current here)
+    void* region = nullptr;
+    ctx->noalloc_status = hipGetSymbolAddress(&region, HIP_SYMBOL(scratch));
+    if (ctx->noalloc_status == hipSuccess && region) {
+        ctx->noalloc_base = (char*)region;
+    } else {
+        (void)hipGetLastError();
+        ctx->noalloc_base = nullptr;
     }
     const char* ng = getenv("BLINK_NO_GRAPH");
     ctx->use_graphs = !(ng && ng[0] == '1');
@@ -368,8 +400,7 @@ BLINK_API int blink_create(int device, blink_ctx** out) {
     ctx->autotune = !(at && at[0] == '0');
     // Twiddle tables of the common grid sides, so that later calls do not
     // allocate.
-    static const int common[] = {128, 180, 256, 512, 1024, 2048, 4096};
-    for (int n : common) {
+    for (int n : kPreparedSides) {
         const float* tw;
         if (twiddles(ctx, n, &tw) != BLINK_OK) {
             blink_destroy(ctx);
@@ -1014,8 +1045,8 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     for (int i = 0; i < n_sets; i++) {
         const int n = sets[i].n_pixels;
         if (arena) {
-            // tables are never built here: only those the context already holds
-            if (!ctx->twiddles.count(n))
+            // tables are never built here: the sides blink_create prepared
+            if (!prepared_side(n) || !ctx->twiddles.count(n))
                 return ctx->fail(BLINK_ERR_UNSUPPORTED,
                                  "set %d: grid side %d has no FFT table in this context (the "
                                  "no-allocation call images the sides blink_create prepares: "
@@ -1606,6 +1637,42 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
     HIP_TRY(hipGraphLaunch(it->second.exec, s));
     HIP_TRY(hipEventRecord(it->second.done, s));
     return BLINK_OK;
+}
+
+// ------------------------------------------------ no-allocation batched call
+
+BLINK_API int blink_image_batch_noalloc(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
+                                        void* stream) {
+    if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
+    if (int rc = check_sets(ctx, sets, n_sets)) return rc;
+    for (int i = 0; i < n_sets; i++)
+        if (is_dft(sets[i].transform))
+            return ctx->fail(BLINK_ERR_UNSUPPORTED,
+                             "set %d: the no-allocation call runs the FFT transform only", i);
+    if (!ctx->noalloc_base)
+        return ctx->fail(BLINK_ERR_HIP, "the fixed scratch region has no address on device %d: %s",
+                         ctx->device, hipGetErrorString(ctx->noalloc_status));
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    Arena arena;
+    arena.base = ctx->noalloc_base;
+    arena.cap = BLINK_NOALLOC_SCRATCH_BYTES;
+    BatchPlan bp;
+    if ((rc = build_batch(ctx, sets, n_sets, bp, &arena))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    // the previous call's descriptor copy has read its host source before
+    // that source is replaced
+    HIP_TRY(hipEventSynchronize(ctx->noalloc_copied));
+    ctx->noalloc_blob = bp.blob;
+    HIP_TRY(hipMemcpyAsync(bp.blob_dev, ctx->noalloc_blob.data(), ctx->noalloc_blob.size(),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(ctx->noalloc_copied, s));
+    // no profiling event pairs (the event pool may grow)
+    const bool prof = ctx->prof_on;
+    ctx->prof_on = false;
+    rc = enqueue_batch(ctx, sets, bp, tables_at(bp, bp.blob_dev), s);
+    ctx->prof_on = prof;
+    return rc;
 }
 
 // ------------------------------------------------------ measurement hooks

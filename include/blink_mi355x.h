@@ -281,6 +281,31 @@ typedef struct blink_image_set {
 BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
                                 void* stream);
 
+/* Bytes of the fixed device region of blink_image_batch_noalloc (256 MiB per
+ * GPU, held from library load to process exit). */
+#define BLINK_NOALLOC_SCRATCH_BYTES (256u << 20)
+
+/* blink_image_batch for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same sets, results and count sums as
+ * blink_image_batch; it allocates nothing: every scratch buffer of the call
+ * (plans, compacted visibilities, row-pass buffer, averages, descriptors) is
+ * cut from one fixed region of BLINK_NOALLOC_SCRATCH_BYTES of device memory
+ * that the library holds on each GPU it runs on, and the row-pass chunks
+ * shrink to fit it.  It launches its kernels directly on `stream` (no graph,
+ * no autotuning: the side's tuned or default tiles) and records no profiling
+ * events.  Limits: BLINK_ERR_INVALID_ARGUMENT when the call's plans do not
+ * fit the region (split the call: 64 channels x 10 timesteps of the 128-tile
+ * array at 1024^2 fit, 768 channels of a 256-antenna array do not),
+ * BLINK_ERR_UNSUPPORTED for the DFT transforms and for grid sides other than
+ * 128, 180, 256, 512, 1024, 2048 and 4096 (those whose FFT tables
+ * blink_create builds).  Every call on a GPU shares that GPU's region,
+ * whichever context makes it, and finds it as the last call left it: calls
+ * that overlap on one GPU (other streams, host threads or contexts) race.
+ * Each call copies its descriptors from host memory and waits for the
+ * previous call's copy, so it is called eagerly, not captured into a graph. */
+BLINK_API int blink_image_batch_noalloc(blink_ctx* ctx, const blink_image_set* sets, int n_sets,
+                                        void* stream);
+
 /* Releases every captured graph and scratch buffer of the context. */
 BLINK_API int blink_release_cache(blink_ctx* ctx);
 
