@@ -35,6 +35,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -119,7 +120,8 @@ def compacted(n_bl: int, n_pix: int) -> bool:
             and os.environ.get("BLINK_COMPACT", "1") != "0")
 
 
-def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft") -> float | None:
+def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
+                   kernel_size: int = 0) -> float | None:
     """HBM bytes a fused kernel must move per launch over n_img images of
     observation o (images cycle over its channels), given the plan: what it
     has to read and write, with every byte counted once.
@@ -136,6 +138,9 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft") ->
       dft_cols       the stage-1 product + the image"""
     n, n_bl = o.n_pixels, o.n_bl
     entries, nonempty = stats
+    if kernel_size:
+        # convolutional gridding writes and reads every plan row 0 .. N/2
+        nonempty = np.full_like(nonempty, n // 2 + 1)
     ch = np.arange(n_img) % o.n_ch
     e_sum = float(entries[ch].sum())
     r_sum = float(nonempty[ch].sum())
@@ -144,6 +149,8 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft") ->
     if kernel == "grid_rows_fft":
         vis_b = 8.0 if compacted(n_bl, n) else 32.0
         return 4.0 * e_sum + vis_b * n_bl * n_img + 8.0 * n * r_sum
+    if kernel == "grid_rows_kb":   # entries and records once (a lower bound), every row
+        return 4.0 * e_sum + 32.0 * n_bl * n_img + 8.0 * n * r_sum
     if kernel == "fft_cols":
         return 8.0 * n * r_sum + 8.0 * n * n * n_img
     if kernel == "vis_compact":
@@ -264,6 +271,24 @@ def cpu_baseline(obs, budget_s: float, max_images: int = 128, reps: int = 5):
             "value_with_oracle_c_fft": k / (total - min(t["fft_c"], t["fft_mkl"]) + t["fft_c"])}
 
 
+def time_run_batched(obs, steps: int, warmup: int):
+    """CPacerImagerMI355X::run_batched() (the C++ class a reference caller
+    binds) timed by tests/cpp/bin/bench_run_batched in a child process on an
+    array of the same shape: host flags / UVW / frequencies and their upload,
+    a new device image buffer per call, the captured graph updated in place."""
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", "bench_run_batched")
+    if not os.path.exists(exe):
+        return {"skipped": "tests/cpp/bin/bench_run_batched is not built"}
+    cmd = [exe, str(obs.n_ant), str(obs.n_ch), str(obs.n_t), str(obs.n_pixels), str(steps),
+           str(max(warmup, 1)), "0"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=180)
+    if r.returncode != 0:
+        return {"failed": r.returncode, "stderr": r.stderr[-400:]}
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res["program"] = "tests/cpp/bin/bench_run_batched " + " ".join(cmd[1:])
+    return res
+
+
 def memory_reference(mib: int = 2048):
     """Write-only (fill) and copy bandwidth of this GPU with torch kernels, as a
     practical reference next to the 8 TB/s peak used for the roofline."""
@@ -289,7 +314,7 @@ def memory_reference(mib: int = 2048):
     return res
 
 
-def build_job(name, wl, transform):
+def build_job(name, wl, transform, kernel_size=0):
     """Device buffers and the list of blink_image_batch calls of one step."""
     import torch
     from imager_amd import capi
@@ -312,7 +337,8 @@ def build_job(name, wl, transform):
                               device="cuda")
             st = capi.make_set(d["vis"], o.n_ant, 1, 1, d["u"], d["v"], d["w"], d["flags"],
                                d["freq"], d["cable"], corr, o.delta_u, o.delta_v, o.min_uv,
-                               o.n_pixels, "XX", img, transform=transform)
+                               o.n_pixels, "XX", img, transform=transform,
+                               kernel_size=kernel_size)
             keep.append((d, img))
             sets.append(st)
             per_size.setdefault(o.n_pixels, []).append(st)
@@ -345,7 +371,7 @@ def build_job(name, wl, transform):
             calls.append([capi.make_set(vis_t, o.n_ant, o.n_ch, 1, d["u"], d["v"], d["w"],
                                         d["flags"], d["freq"], d["cable"], corr, o.delta_u,
                                         o.delta_v, o.min_uv, N, "XX", images,
-                                        transform=transform)])
+                                        transform=transform, kernel_size=kernel_size)])
         t_last = o.n_t - 1
         # the buffer holds the last timestep's images after a step
         parity = [(o, t_last, int(c), images[0, int(c)], f"t{t_last}_ch{int(c)}")
@@ -363,7 +389,7 @@ def build_job(name, wl, transform):
     images = torch.empty((o.n_t, o.n_ch, N, N), dtype=torch.complex64, device="cuda")
     st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
                        d["freq"], d["cable"], corr, o.delta_u, o.delta_v, o.min_uv, N, "XX",
-                       images, sums, transform=transform)
+                       images, sums, transform=transform, kernel_size=kernel_size)
     parity = []
     for i in np.unique(np.linspace(0, o.n_images - 1, 8).round().astype(int)):
         t, c = divmod(int(i), o.n_ch)
@@ -388,6 +414,9 @@ def main():
                     help="2-D transform of the batched path: in-house FP32 FFT, or the "
                          "split-precision DFT-as-GEMM on MFMA (N <= 512) with bf16 (dft) "
                          "or fp16 (dft-fp16) hi+lo operands")
+    ap.add_argument("--kernel-size", type=int, default=0,
+                    help="convolutional (Kaiser-Bessel with sinc) gridding with K x K footprints "
+                         "instead of nearest-cell gridding (0)")
     ap.add_argument("--batch-tiles", action="append", default=[], metavar="N:ROWS:COLS",
                     help="pin the register kernels' row tile and column block for side N "
                          "(blink_set_batch_tiles) instead of autotuning; repeatable")
@@ -398,6 +427,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-run-batched", action="store_true",
+                    help="skip timing CPacerImagerMI355X::run_batched (C++ host class)")
     ap.add_argument("--profile-only", action="store_true",
                     help="only warmup + timed steps (for rocprofv3 runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -436,7 +467,7 @@ def main():
         ctx.set_batch_tiles(n_t, r_t, c_t)
     transform = {"fft": capi.TRANSFORM_FFT, "dft": capi.TRANSFORM_DFT_BF16X3,
                  "dft-fp16": capi.TRANSFORM_DFT_FP16X3}[args.transform]
-    job = build_job(args.workload, wl, transform)
+    job = build_job(args.workload, wl, transform, args.kernel_size)
     obs = job["obs"]            # the observation the plain path and the CPU baseline use
     N = job["n_pixels"]         # None for mixed sizes
     stream = torch.cuda.current_stream()
@@ -505,10 +536,12 @@ def main():
     if N is not None:
         stats = plan_stats(obs, transposed=args.transform == "fft")
         n_launch = max(prof.get("grid_rows_fft", (0, 0))[0], prof.get("grid_tiles", (0, 0))[0],
-                       prof.get("grid_half_rows", (0, 0))[0]) // args.steps
+                       prof.get("grid_half_rows", (0, 0))[0],
+                       prof.get("grid_rows_kb", (0, 0))[0]) // args.steps
         chunk = -(-job["images_per_step"] // max(n_launch, 1))
-        imgs_per_launch = chunk if dom in ("grid_rows_fft", "fft_cols", "grid_tiles",
-                                           "grid_half_rows", "dft_rows", "dft_cols") \
+        imgs_per_launch = chunk if dom in ("grid_rows_fft", "grid_rows_kb", "fft_cols",
+                                           "grid_tiles", "grid_half_rows", "dft_rows",
+                                           "dft_cols") \
             else job["images_per_step"]
         avg_s = kernels[dom]["avg_ms"] * 1e-3
         if dom in ("dft_rows", "dft_cols"):
@@ -523,7 +556,8 @@ def main():
                     "note": f"dense {prec} MFMA peak; executed = 3 split terms x 4 (dft_rows) "
                             "or 2 (dft_cols) real products over padded tiles"}
         else:
-            phys = physical_bytes(dom, obs, stats, imgs_per_launch, args.transform)
+            phys = physical_bytes(dom, obs, stats, imgs_per_launch, args.transform,
+                                  args.kernel_size)
             if phys is not None:
                 ach = phys / avg_s / 1e9
                 per_image = read_traffic(dom, args.workload)
@@ -567,7 +601,8 @@ def main():
             if k == "plan":
                 b = v["launches_per_step"] * physical_bytes(k, obs, stats, obs.n_ch)
             else:
-                b = physical_bytes(k, obs, stats, job["images_per_step"], args.transform)
+                b = physical_bytes(k, obs, stats, job["images_per_step"], args.transform,
+                                   args.kernel_size)
             step_bytes += b or 0.0
         pipeline.update({"bytes_per_step": step_bytes,
                          "achieved_gbs": step_bytes / (ms_per_step * 1e-3) / 1e9,
@@ -591,7 +626,7 @@ def main():
 
     # plain path: one image at a time through the stage entry points
     plain = None
-    if args.plain_images > 0 and job["plain_ok"]:
+    if args.plain_images > 0 and job["plain_ok"] and not args.kernel_size:
         n_s = min(args.plain_images, obs.n_images)
         n_bl = obs.n_bl
         d = job["dev"]
@@ -626,16 +661,36 @@ def main():
         log(f"plain path: {plain['value']:.1f} images/s; batched is {plain['batched_speedup']:.1f}x")
         del vis_s, grid, cnt, img
 
+    # the C++ class' run_batched() on the same shape, as a caller uses it
+    run_batched = None
+    if (not args.no_run_batched and not args.kernel_size and args.transform == "fft"
+            and args.workload in ("mwa128_64ch10t_1024", "mwa128_768ch_1024")):
+        run_batched = time_run_batched(obs, steps=max(args.steps, 5), warmup=args.warmup)
+        log(f"run_batched: {run_batched}")
+
     # parity of the sample images of the last timed replay against the oracle
     parity = None
     if parity_images is not None:
         e32s, e64s, names = [], [], []
         for (o, t, ch, got, label) in parity_images:
             v1 = o.vis[t:t + 1, ch:ch + 1]
-            a = (o.n_ant, o.u, o.v, o.w, o.flags, o.freq[ch:ch + 1], o.cable, True,
-                 True, o.delta_u, o.delta_v, o.min_uv, o.n_pixels, "XX")
-            r32, _ = oracle.run_f32(v1, *a)
-            r64 = oracle.run_f64(v1, *a)
+            if args.kernel_size:
+                # the oracle's convolutional gridding, reference precision and FP64
+                f1 = o.freq[ch:ch + 1]
+                vc = oracle.cable_corrections(oracle.geometric_corrections(v1, o.n_ant, o.w, f1),
+                                              o.n_ant, o.cable, f1)
+                refs = []
+                for prec in ("f32", "f64"):
+                    g, c, _ = oracle.gridding(vc, o.n_ant, o.u, o.v, o.flags, f1, o.delta_u,
+                                              o.delta_v, o.min_uv, o.n_pixels, "XX",
+                                              kernel_size=args.kernel_size, precision=prec)
+                    refs.append(oracle.image(g, c))
+                r32, r64 = refs
+            else:
+                a = (o.n_ant, o.u, o.v, o.w, o.flags, o.freq[ch:ch + 1], o.cable, True,
+                     True, o.delta_u, o.delta_v, o.min_uv, o.n_pixels, "XX")
+                r32, _ = oracle.run_f32(v1, *a)
+                r64 = oracle.run_f64(v1, *a)
             e32s.append(oracle.peak_normalised_error(got, r32[0, 0]))
             e64s.append(oracle.peak_normalised_error(got, r64[0, 0]))
             names.append(label)
@@ -647,12 +702,14 @@ def main():
         log(f"parity: {parity}")
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.kernel_size:
         cpu = cpu_baseline(obs, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
 
     config = dict(job["config"])
     config["input_copies"] = len(rotation)
+    if args.kernel_size:
+        config["gridding"] = f"Kaiser-Bessel {args.kernel_size}x{args.kernel_size}"
     config.update({"workload": args.workload, "images_per_step": job["images_per_step"],
                    "pol": "XX", "corrections": "geometric+cable",
                    "path": "blink_image_batch (hipGraph)",
@@ -671,6 +728,7 @@ def main():
         "pipeline_roofline": pipeline,
         "cpu_baseline": cpu,
         "plain_gpu": plain,
+        "run_batched": run_batched,
         "kernels": kernels,
         "eager_ms_per_step": eager_ms,
         "batch_tiles": {str(n): ctx.batch_tiles(n) for n in job["sizes"]},

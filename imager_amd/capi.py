@@ -77,7 +77,7 @@ class ImageSet(ctypes.Structure):
         ("n_pixels", ctypes.c_int), ("pol", ctypes.c_int),
         ("images", ctypes.c_void_p), ("count_sums", ctypes.c_void_p),
         ("transform", ctypes.c_int), ("weighting", ctypes.c_int),
-        ("average", ctypes.c_void_p),
+        ("average", ctypes.c_void_p), ("kernel_size", ctypes.c_int), ("kernel_mode", ctypes.c_int),
     ]
 
 
@@ -328,7 +328,8 @@ class Context:
 
 def make_set(vis, n_ant, n_ch, n_t, u_m, v_m, w_m, bl_flags, freq_hz, cable_m, corrections,
              delta_u, delta_v, min_uv, n_pixels, pol, images, count_sums=None,
-             transform=TRANSFORM_FFT, weighting=WEIGHTING_NATURAL, average=None) -> ImageSet:
+             transform=TRANSFORM_FFT, weighting=WEIGHTING_NATURAL, average=None,
+             kernel_size=0, kernel_mode=KERNEL_KAISER_BESSEL) -> ImageSet:
     """One struct blink_image_set; `images` may be None when `average` is given."""
     return ImageSet(_ptr(vis), n_ant, n_ch, n_t, _ptr(u_m), _ptr(v_m), _ptr(w_m), _ptr(bl_flags),
                     _ptr(freq_hz), _ptr(cable_m), int(corrections), float(delta_u),
@@ -336,4 +337,4 @@ def make_set(vis, n_ant, n_ch, n_t, u_m, v_m, w_m, bl_flags, freq_hz, cable_m, c
                     POL[pol] if isinstance(pol, str) else int(pol), _ptr(images),
                     _ptr(count_sums), int(transform),
                     WEIGHTING[weighting] if isinstance(weighting, str) else int(weighting),
-                    _ptr(average))
+                    _ptr(average), int(kernel_size), int(kernel_mode) if kernel_size else 0)

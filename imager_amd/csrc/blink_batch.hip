@@ -421,7 +421,8 @@ k_average_rows(SetDesc sd, int64_t img0, int n_img, int64_t li0, int n,
         const int64_t img = img0 + k;
         const int* rsg = sd.row_start + (img % sd.g.n_ch) * (int64_t)(n + 1);
         const float sc = (float)(1.0 / (double)sd.isums[img]);
-        const float4 v = rsg[p + 1] != rsg[p] ? inter[(li0 + k) * per + j] : make_float4(z, z, z, z);
+        const float4 v = (sd.all_rows || rsg[p + 1] != rsg[p]) ? inter[(li0 + k) * per + j]
+                                                                : make_float4(z, z, z, z);
         sum.x += sc * v.x;
         sum.y += sc * v.y;
         sum.z += sc * v.z;

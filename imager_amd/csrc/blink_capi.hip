@@ -779,12 +779,14 @@ struct SetScratch {
     // the average pass's count and row start (launch_average_rows)
     float* avg_acc = nullptr;
     unsigned* avg_meta = nullptr;
+    const double* kb_table = nullptr;  // convolutional gridding kernels
 };
 
 struct BatchGroup {
     int n = 0;
     int transform = BLINK_TRANSFORM_FFT;
     bool uniform = false;        // "U" weighting
+    int kb_size = 0, kb_mode = 0;  // convolutional gridding (generic kernels)
     bool reg = false;            // register-FFT kernels (power-of-two sides 128..2048)
     std::vector<int> sets;       // call indices of the sets, in descriptor order
     int first_desc = 0;          // index of the group's first SetDesc in the table
@@ -864,11 +866,14 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         const long long mib = atoll(e);
         if (mib > 0) inter_budget = mib << 20;
     }
-    // groups (grid side, transform, weighting) in order of first appearance
-    std::map<std::tuple<int, int, int>, int> gidx;
+    // groups (grid side, transform, weighting, gridding kernel) in order of
+    // first appearance
+    std::map<std::tuple<int, int, int, int, int>, int> gidx;
     for (int i = 0; i < n_sets; i++) {
-        const std::tuple<int, int, int> key(sets[i].n_pixels, sets[i].transform,
-                                            sets[i].weighting);
+        const int kb = sets[i].kernel_size;
+        const std::tuple<int, int, int, int, int> key(sets[i].n_pixels, sets[i].transform,
+                                                      sets[i].weighting, kb,
+                                                      kb ? sets[i].kernel_mode : 0);
         auto it = gidx.find(key);
         if (it == gidx.end()) {
             it = gidx.emplace(key, (int)bp.groups.size()).first;
@@ -876,7 +881,10 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             g.n = sets[i].n_pixels;
             g.transform = sets[i].transform;
             g.uniform = sets[i].weighting == BLINK_WEIGHTING_UNIFORM;
-            g.reg = g.transform == BLINK_TRANSFORM_FFT && batch_reg_supported(g.n);
+            g.kb_size = kb;
+            g.kb_mode = kb ? sets[i].kernel_mode : 0;
+            // convolutional gridding runs on the generic kernels, any side
+            g.reg = g.transform == BLINK_TRANSFORM_FFT && batch_reg_supported(g.n) && kb == 0;
             bp.groups.push_back(g);
         }
         BatchGroup& g = bp.groups[it->second];
@@ -1055,6 +1063,11 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         }
         const float* tw;
         if ((rc = twiddles(ctx, n, &tw))) return rc;
+        if (sets[i].kernel_size > 0) {
+            const double* table;
+            if ((rc = kb_table(ctx, sets[i].kernel_size, sets[i].kernel_mode, &table))) return rc;
+            bp.sc[i].kb_table = table;
+        }
         if (is_dft(sets[i].transform)) {
             const uint16_t* pl;
             for (int stage = 1; stage <= 2; stage++)
@@ -1102,6 +1115,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.uv_swap = g.transform == BLINK_TRANSFORM_FFT;
             d.compact = bp.sc[i].compact;
             d.uniform = st.weighting == BLINK_WEIGHTING_UNIFORM ? 1 : 0;
+            d.kb_size = st.kernel_size;
+            d.kb_table = bp.sc[i].kb_table;
+            d.all_rows = st.kernel_size > 0 ? 1 : 0;
             base += d.n_img;
             desc_of[i] = (int)desc.size();
             desc.push_back(d);
@@ -1312,6 +1328,9 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
             LAUNCH("grid_rows_fft", s,
                    launch_grid_rows_reg(first, sd, is, ns, n, tl.first, gi0, cnt, inter, tw, s,
                                         g.uniform));
+        else if (g.kb_size)
+            LAUNCH("grid_rows_kb", s,
+                   launch_grid_rows_kb(first, sd, is, ns, plan, gi0, cnt, inter, tw, s));
         else
             LAUNCH("grid_rows_fft", s,
                    launch_grid_rows_fft(first, sd, is, ns, plan, gi0, cnt, inter, tw, s));
@@ -1463,6 +1482,7 @@ static std::string graph_key(const blink_image_set* sets, int n_sets, const Batc
         put(st.u_m), put(st.v_m), put(st.w_m), put(st.bl_flags), put(st.freq_hz), put(st.cable_m);
         put(st.corrections), put(st.delta_u), put(st.delta_v), put(st.min_uv);
         put(st.n_pixels), put(st.pol), put(st.transform), put(st.weighting);
+        put(st.kernel_size), put(st.kernel_mode);
         const bool has_sums = st.count_sums != nullptr, has_images = st.images != nullptr,
                    has_average = st.average != nullptr;
         put(has_sums), put(has_images), put(has_average);
@@ -1516,6 +1536,19 @@ static int check_sets(blink_ctx* ctx, const blink_image_set* sets, int n_sets) {
         CHECK_ARG(st.pol >= 0 && st.pol <= 3, "set %d: pol must be 0..3", i);
         CHECK_ARG(st.weighting == BLINK_WEIGHTING_NATURAL || st.weighting == BLINK_WEIGHTING_UNIFORM,
                   "set %d: weighting must be NATURAL or UNIFORM", i);
+        CHECK_ARG(st.kernel_size >= 0 && st.kernel_size <= BLINK_MAX_KERNEL_SIZE,
+                  "set %d: kernel_size must be 0..%d, got %d", i, BLINK_MAX_KERNEL_SIZE,
+                  st.kernel_size);
+        if (st.kernel_size > 0) {
+            CHECK_ARG(st.kernel_mode == BLINK_KERNEL_KAISER_BESSEL ||
+                          st.kernel_mode == BLINK_KERNEL_KAISER_BESSEL_NO_SINC,
+                      "set %d: unknown kernel mode %d", i, st.kernel_mode);
+            if (st.transform != BLINK_TRANSFORM_FFT || st.weighting != BLINK_WEIGHTING_NATURAL)
+                return ctx->fail(BLINK_ERR_UNSUPPORTED,
+                                 "set %d: batched convolutional gridding runs with the FFT "
+                                 "transform and natural weighting (blink_gridding_kernel "
+                                 "takes the others)", i);
+        }
         CHECK_ARG(((uintptr_t)st.vis & 15) == 0 && ((uintptr_t)st.images & 15) == 0 &&
                       ((uintptr_t)st.average & 15) == 0,
                   "set %d: vis, images and average must be 16-byte aligned", i);
@@ -1645,10 +1678,14 @@ BLINK_API int blink_image_batch_noalloc(blink_ctx* ctx, const blink_image_set* s
                                         void* stream) {
     if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
     if (int rc = check_sets(ctx, sets, n_sets)) return rc;
-    for (int i = 0; i < n_sets; i++)
+    for (int i = 0; i < n_sets; i++) {
         if (is_dft(sets[i].transform))
             return ctx->fail(BLINK_ERR_UNSUPPORTED,
                              "set %d: the no-allocation call runs the FFT transform only", i);
+        if (sets[i].kernel_size > 0)
+            return ctx->fail(BLINK_ERR_UNSUPPORTED,
+                             "set %d: the no-allocation call grids to the nearest cell only", i);
+    }
     if (!ctx->noalloc_base)
         return ctx->fail(BLINK_ERR_HIP, "the fixed scratch region has no address on device %d: %s",
                          ctx->device, hipGetErrorString(ctx->noalloc_status));

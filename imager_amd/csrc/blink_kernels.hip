@@ -1244,7 +1244,8 @@ k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     if (cbk == 0 && threadIdx.x == 0 && sd.count_sums) sd.count_sums[img] = (double)count;
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (n + 1);
     const float empty = sd.uniform ? __builtin_nanf("") : 0.f;  // rows without entries
-    for (int i = threadIdx.x; i <= h; i += blockDim.x) nonempty[i] = rsg[i + 1] != rsg[i];
+    for (int i = threadIdx.x; i <= h; i += blockDim.x)
+        nonempty[i] = sd.all_rows || rsg[i + 1] != rsg[i];
     __syncthreads();
 
     const int c0 = cbk * cb;
@@ -1457,6 +1458,150 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
     return hipGetLastError();
 }
 
+// Convolutional gridding fused with the row FFT (generic kernels, transposed
+// plan: plan rows are grid columns x, positions rows y).  Workgroup = (tile of
+// rb consecutive plan rows r0 .. r1 of 0 .. N/2, image).  Every visibility
+// that is gridded adds its K x K footprint around its nearest cell (u, v) with
+// value V and, mirrored, around (-u, -v) with conj(V) (convolve_with_kernel,
+// src/pacer_imager.cpp:654-714, placed as calc_xy_grid2 places it, :716-750).
+// The plan lists, per visibility, the one of its two nearest cells that falls
+// in rows 1 .. N/2-1 (its partner is in row N - p), or both when they fall in
+// rows 0 and N/2.  A footprint around row p covers rows p - K/2 .. p + K/2,
+// so a tile gathers the entries of rows r0 - K/2 .. r1 + K/2 and, when r1 +
+// K/2 reaches N/2, of rows N - r1 - K/2 .. N/2 (their partners' footprints
+// reach back below N/2); each entry adds its own footprint and, for rows 1 ..
+// N/2-1, its partner's, keeping the cells in rows r0 .. r1.  Counts stay at
+// the nearest cells, as the reference keeps them.  Every row of the tile is
+// transformed and written.
+template <int S, bool kBlockTeam, bool kMulti>
+__global__ void __launch_bounds__(kFftThreads)
+k_grid_rows_kb(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
+               FftPlan plan, int team, int rb, int64_t gi0, float2* __restrict__ inter,
+               const float2* __restrict__ tw) {
+#pragma clang fp contract(off)
+    extern __shared__ float4 grk_lds4[];
+    const int n = plan.n;
+    float2* tile = (float2*)grk_lds4;  // [rb][n]
+    const int teams = kBlockTeam ? 1 : (int)blockDim.x / team;
+    const int t_id = kBlockTeam ? 0 : (int)threadIdx.x / team;
+    const int lane = kBlockTeam ? (int)threadIdx.x : (int)threadIdx.x % team;
+    float2* tmp = tile + (size_t)rb * n + (size_t)t_id * n;
+    __shared__ int red[kFftThreads / 64];
+
+    const int h = n / 2;
+    const int n_rb = (h + 1 + rb - 1) / rb;
+    const unsigned nwg = gridDim.x;
+    unsigned bid = blockIdx.x;
+    if ((nwg & 7u) == 0) bid = (bid & 7u) * (nwg >> 3) + (bid >> 3);
+    const int tile_id = (int)(bid % n_rb);
+    const int li = (int)(bid / n_rb);
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const GatherArgs& g = sd.g;
+    const int img = (int)(gi - sd.img_base);
+    const int ch = img % g.n_ch;
+    const int r0 = tile_id * rb;
+    const int nr = min(rb, h + 1 - r0);
+    const int r1 = r0 + nr - 1;
+    const int K = sd.kb_size, mid = K / 2;
+    int lo = max(0, r0 - mid);
+    const int hi = min(h, r1 + mid);
+    if (r1 + mid >= h) lo = max(0, min(lo, n - r1 - mid));
+    const int* rsg = sd.row_start + (int64_t)ch * (n + 1);
+    const int e0 = rsg[lo], e1 = rsg[hi + 1];
+    for (int i = threadIdx.x; i < nr * n; i += blockDim.x) tile[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+
+    const uint32_t* ent = sd.entries + (int64_t)ch * 2 * g.n_bl;
+    const float2* rec = (const float2*)g.vis + 4 * ((int64_t)img * g.n_bl);
+    const double nu = g.freq_hz[ch];
+    const double wl = wavelength_of(nu);
+    const int p0 = g.pol == 1 ? 3 : (g.pol == 3 ? 1 : 0);
+    const int p1 = g.pol == 3 ? 2 : 3;
+    int n_ok = 0;
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += (int)blockDim.x) {
+        const uint32_t ev = ent[e];
+        const unsigned bl = ev >> 14;
+        const int own = (ev >> 13) & 1u ? -1 : 1;  // sign of the listed cell
+        float2 v = select_pol2(rec[4 * (size_t)bl + p0],
+                               g.pol >= 2 ? rec[4 * (size_t)bl + p1] : make_float2(0.f, 0.f), g.pol);
+        if (g.corrections) v = cmul(v, unit_phasor(((g.phase_coef[bl] * nu) / kVelLight)));
+        if (!vis_ok(v)) continue;
+        int up, vp;
+        if (!cell_of(sd.u_m[bl], sd.v_m[bl], wl, sd.du, sd.dv, sd.min_uv, n, up, vp)) continue;
+        // the listed cell's plan row (its column x, as the plan wraps it), and
+        // the kernel rows of the fractional position
+        const int p = wrap_index(own * up, n);
+        if (p >= r0 && p <= r1) n_ok += (p == 0 || p == h) ? 1 : 2;
+        const double ue = (double)sd.u_m[bl] / wl / sd.du;
+        const double ve = (double)sd.v_m[bl] / wl / sd.dv;
+        int kx = (int)round((ue - (double)up) * 1023.0);
+        int ky = (int)round((ve - (double)vp) * 1023.0);
+        kx = (kx + 1534) % 1023;
+        ky = (ky + 1534) % 1023;
+        const double* xk = sd.kb_table + (int64_t)kx * K;
+        const double* yk = sd.kb_table + (int64_t)ky * K;
+        const int n_fp = (p == 0 || p == h) ? 1 : 2;  // own footprint, then the partner's
+        for (int f = 0; f < n_fp; f++) {
+            const int sgn = f == 0 ? own : -own;
+            for (int c = 0; c < K; c++) {
+                int x;
+                if (!kb_cell(up - mid + c, sgn, n, x) || x < r0 || x > r1) continue;
+                const double wx = xk[c];
+                float2* row = tile + (size_t)(x - r0) * n;
+                for (int j = 0; j < K; j++) {
+                    int y;
+                    if (!kb_cell(vp - mid + j, sgn, n, y)) continue;
+                    const double w = yk[j] * wx;
+                    atomicAdd(&row[y].x, (float)((double)v.x * w));
+                    atomicAdd(&row[y].y, (float)((double)sgn * (double)v.y * w));
+                }
+            }
+        }
+    }
+    const int tot = block_sum_int(n_ok, red);
+    if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
+    __syncthreads();
+    const int rounds = kBlockTeam ? nr : (nr + teams - 1) / teams;
+    for (int k = 0; k < rounds; k++) {
+        const int r = kBlockTeam ? k : k * teams + t_id;
+        if (!kBlockTeam && r >= nr) break;
+        float2* row = tile + (size_t)r * n;
+        const float2* res = fft_row<S, kBlockTeam>(row, tmp, plan, tw, lane, team);
+        float2* dst = inter + ((int64_t)li * (h + 1) + r0 + r) * n;
+        for (int i = lane; i < n; i += team) store_stream(&dst[i], res[i]);
+        team_sync<kBlockTeam>();
+    }
+}
+
+hipError_t launch_grid_rows_kb(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                               int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
+                               float* inter, const float* tw, hipStream_t stream) {
+    if (n_img == 0) return hipSuccess;
+    if (first.kb_size < 1) return hipErrorInvalidValue;
+    const int n = plan.n;
+    const int team = fft_team_for(n);
+    const bool block_team = team == kFftThreads;
+    const int teams = block_team ? 1 : kFftThreads / team;
+    // rows per tile: a footprint covers K rows, so taller tiles re-read fewer
+    // entries; as many rows as fit up to 8, and at least ~2048 workgroups
+    const int64_t want = std::max<int64_t>(1, (int64_t)(n / 2 + 1) * n_img / 2048);
+    int rb = (int)std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)grid_rows_tile_rows(n), want, 8}));
+    const size_t lds = sizeof(float2) * ((size_t)rb * n + (size_t)teams * n);
+    if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
+    const int n_rb = (n / 2 + 1 + rb - 1) / rb;
+    const int64_t blocks = (int64_t)n_rb * n_img;
+    if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+    const bool multi = n_sets > 1;
+    auto k = block_team
+                 ? (multi ? k_grid_rows_kb<1, true, true> : k_grid_rows_kb<1, true, false>)
+                 : (multi ? k_grid_rows_kb<1, false, true> : k_grid_rows_kb<1, false, false>);
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kFftThreads), lds, stream, first, sets,
+                       img_set, plan, team, rb, gi0, (float2*)inter, (const float2*)tw);
+    return hipGetLastError();
+}
+
 // Buffer columns (image rows) per workgroup of k_cols_half: the largest even divisor of
 // n, at most 64, whose LDS rows fit next to the team scratch and that still
 // gives at least 256 workgroups over the n_img images (one per CU), else the
@@ -1527,6 +1672,10 @@ hipError_t set_kernel_attributes() {
     set((const void*)k_fft_cols<1, false>);
     set((const void*)k_fft_cols<-1, true>);
     set((const void*)k_fft_cols<-1, false>);
+    set((const void*)k_grid_rows_kb<1, true, false>);
+    set((const void*)k_grid_rows_kb<1, false, false>);
+    set((const void*)k_grid_rows_kb<1, true, true>);
+    set((const void*)k_grid_rows_kb<1, false, true>);
     set((const void*)k_grid_rows_fft<1, true, false>);
     set((const void*)k_grid_rows_fft<1, false, false>);
     set((const void*)k_grid_rows_fft<1, true, true>);

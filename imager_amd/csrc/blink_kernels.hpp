@@ -78,6 +78,14 @@ struct SetDesc {
                               // tiles then gather these instead of the records
     int uniform;              // "U" weighting: cells divided by their counts, so
                               // empty cells (and plan rows without entries) are NaN
+    // convolutional gridding (kb_size > 0): every visibility and its conjugate
+    // spread over kb_size x kb_size cells with the 1023-row kernel table
+    // kb_table; the row pass then writes every plan row 0 .. N/2, and all_rows
+    // tells the second pass to read them all (rows without plan entries are
+    // not empty there)
+    const double* kb_table;
+    int kb_size;
+    int all_rows;
 };
 
 // The set of group image gi, by value.  Kernels are built twice: for a
@@ -172,6 +180,14 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
 hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int* img_set,
                             int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
                             const float* inter, const float* tw, hipStream_t stream);
+// Row pass of convolutional (Kaiser-Bessel) gridding on the generic kernels:
+// every plan row 0 .. N/2 of every image is gridded, transformed and written
+// (row-major, as launch_grid_rows_fft writes); a tile gathers the plan
+// entries of its rows +- K/2 and, near row N/2, of the rows whose Hermitian
+// partners' footprints reach it.  The sets' kb_size must be > 0.
+hipError_t launch_grid_rows_kb(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                               int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
+                               float* inter, const float* tw, hipStream_t stream);
 // Gridding plans of a batched call: one unit per (set, channel); `parts`
 // workgroups per unit split its baselines (at most kPlanPartMax each).  A plan
 // lists the cells of rows 0 .. N/2 only (each baseline's cell and conjugate

@@ -96,17 +96,19 @@ void CPacerImagerMI355X::upload_inputs(const Visibilities& xcorr) {
                         hipMemcpyHostToDevice), "frequencies H2D");
 }
 
+// blink_grid_kernel of m_nConvolvingKernelSize / _gridMode, 0 for nearest-cell
+// gridding (src/pacer_imager.cpp:378-381, :617-623: NearestNeighbourGridding
+// also selects the Kaiser-Bessel table there).
+int CPacerImagerMI355X::gridding_kernel_mode() const {
+    if (m_nConvolvingKernelSize <= 0) return 0;
+    if (_gridMode == GridMode::KaiserBesselWithoutSinc) return BLINK_KERNEL_KAISER_BESSEL_NO_SINC;
+    if (_gridMode == GridMode::KaiserBesselKernel || _gridMode == GridMode::NearestNeighbourGridding)
+        return BLINK_KERNEL_KAISER_BESSEL;
+    throw std::runtime_error("only the Kaiser-Bessel gridding kernels are implemented");
+}
+
 void CPacerImagerMI355X::gridding(Visibilities& xcorr) {
-    int kernel_mode = 0;
-    if (m_nConvolvingKernelSize > 0) {
-        if (_gridMode == GridMode::KaiserBesselWithoutSinc)
-            kernel_mode = BLINK_KERNEL_KAISER_BESSEL_NO_SINC;
-        else if (_gridMode == GridMode::KaiserBesselKernel ||
-                 _gridMode == GridMode::NearestNeighbourGridding)
-            kernel_mode = BLINK_KERNEL_KAISER_BESSEL;
-        else
-            throw std::runtime_error("only the Kaiser-Bessel gridding kernels are implemented");
-    }
+    const int kernel_mode = gridding_kernel_mode();
     upload_inputs(xcorr);
     xcorr.to_gpu();
     const size_t n_img = n_gridded_channels * n_gridded_intervals;
@@ -192,9 +194,7 @@ void CPacerImagerMI355X::ApplyCableCorrections(Visibilities& xcorr, MemoryBuffer
 }
 
 Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
-    if (m_nConvolvingKernelSize > 0)
-        throw std::invalid_argument("run_batched grids to the nearest cell; use run() for "
-                                    "convolutional gridding");
+    const int kernel_mode = gridding_kernel_mode();
     UpdateFlags();
     CalculateUVW();
     fill_frequencies(xcorr);
@@ -246,6 +246,8 @@ Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
     st.weighting = std::strcmp(weighting, "U") == 0 ? BLINK_WEIGHTING_UNIFORM
                                                     : BLINK_WEIGHTING_NATURAL;
     st.average = average_images ? (float*)avg.data() : nullptr;
+    st.kernel_size = kernel_mode ? m_nConvolvingKernelSize : 0;
+    st.kernel_mode = kernel_mode;
     check(blink_image_batch(ctx_, &st, 1, nullptr));
     const double pixscale_ra = (1.00 / (oversampling_factor * u_max)) * (180.00 / kPi);
     const double pixscale_dec = (1.00 / (oversampling_factor * v_max)) * (180.00 / kPi);

@@ -73,6 +73,7 @@ class CPacerImagerMI355X : public CPacerImager {
     MemoryBuffer<std::complex<float>> vis_gpu_;
 
     void check(int rc) const;
+    int gridding_kernel_mode() const;
     void UpdateAntennaFlags(int n_ant);
     void upload_inputs(const Visibilities& xcorr);
 
@@ -106,8 +107,9 @@ class CPacerImagerMI355X : public CPacerImager {
     // apply_*_correction, the weighting, and, with average_images, only the
     // mean image (the per-image images are not formed).  The visibilities
     // are not modified (host visibilities are copied to the device, not
-    // moved).  Images stay in device memory.  Nearest-cell gridding only:
-    // throws std::invalid_argument when m_nConvolvingKernelSize > 0.
+    // moved).  Images stay in device memory.  Convolutional gridding
+    // (m_nConvolvingKernelSize > 0) runs with natural weighting and the FFT
+    // transform; other combinations throw std::invalid_argument.
     Images run_batched(Visibilities& xcorr);
 
     blink_ctx* context() const { return ctx_; }

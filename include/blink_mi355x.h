@@ -266,6 +266,12 @@ typedef struct blink_image_set {
                                 also image_averaging_cpu's real-part mean, src/utils.cpp:4-22).
                                 With the FFT transform it is formed from the row-pass buffers
                                 without the images; with a DFT transform `images` is required */
+    int kernel_size;         /* 0: nearest-cell gridding; 1 .. BLINK_MAX_KERNEL_SIZE:
+                                convolutional gridding, each visibility and its conjugate
+                                spread over kernel_size x kernel_size cells as
+                                blink_gridding_kernel spreads them (FFT transform, natural
+                                weighting) */
+    int kernel_mode;         /* blink_grid_kernel when kernel_size > 0 */
 } blink_image_set;
 
 /* Visibilities -> normalised, shifted dirty images (and / or their mean) for
@@ -286,19 +292,21 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
 #define BLINK_NOALLOC_SCRATCH_BYTES (256u << 20)
 
 /* blink_image_batch for callers that cannot allocate device memory (serving
- * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same sets, results and count sums as
- * blink_image_batch; it allocates nothing: every scratch buffer of the call
- * (plans, compacted visibilities, row-pass buffer, averages, descriptors) is
- * cut from one fixed region of BLINK_NOALLOC_SCRATCH_BYTES of device memory
- * that the library holds on each GPU it runs on, and the row-pass chunks
- * shrink to fit it.  It launches its kernels directly on `stream` (no graph,
- * no autotuning: the side's tuned or default tiles) and records no profiling
- * events.  Limits: BLINK_ERR_INVALID_ARGUMENT when the call's plans do not
- * fit the region (split the call: 64 channels x 10 timesteps of the 128-tile
- * array at 1024^2 fit, 768 channels of a 256-antenna array do not),
- * BLINK_ERR_UNSUPPORTED for the DFT transforms and for grid sides other than
- * 128, 180, 256, 512, 1024, 2048 and 4096 (those whose FFT tables
- * blink_create builds).  Every call on a GPU shares that GPU's region,
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * Same sets, results and count sums as blink_image_batch; it allocates
+ * nothing: every scratch buffer of the call (plans, compacted visibilities,
+ * row-pass buffer, averages, descriptors) is cut from one fixed region of
+ * BLINK_NOALLOC_SCRATCH_BYTES of device memory that the library holds on
+ * each GPU it runs on, and the row-pass chunks shrink to fit it. It launches
+ * its kernels directly on `stream` (no graph, no autotuning: the side's
+ * tuned or default tiles) and records no profiling events. Limits:
+ * BLINK_ERR_INVALID_ARGUMENT when the call's plans do not fit the region
+ * (split the call: 64 channels x 10 timesteps of the 128-tile array at
+ * 1024^2 fit, 768 channels of a 256-antenna array do not),
+ * BLINK_ERR_UNSUPPORTED for the DFT transforms, for convolutional gridding
+ * (kernel_size > 0, whose tables would be allocated) and for grid sides
+ * other than 128, 180, 256, 512, 1024, 2048 and 4096 (those whose FFT tables
+ * blink_create builds). Every call on a GPU shares that GPU's region,
  * whichever context makes it, and finds it as the last call left it: calls
  * that overlap on one GPU (other streams, host threads or contexts) race.
  * Each call copies its descriptors from host memory and waits for the

@@ -238,13 +238,15 @@ void test_error_paths() {
     std::vector<double> freq{150e6};
     Visibilities xcorr = make_vis(a, 1, 1, freq, 1, info);
     imager.m_nConvolvingKernelSize = 7;
+    imager.weighting = "U";
     thrown = false;
     try {
         imager.run_batched(xcorr);
     } catch (const std::invalid_argument&) {
         thrown = true;
     }
-    if (!thrown) throw TestFailed("run_batched must refuse convolutional gridding");
+    if (!thrown) throw TestFailed("run_batched must refuse convolutional gridding with \"U\"");
+    imager.weighting = "";
     imager._gridMode = GridMode::GaussianKernel;
     thrown = false;
     try {
@@ -282,8 +284,21 @@ void test_imager_kaiser_bessel() {
             xcorr.obsInfo.frequencyResolution = 0.625;
             xcorr.obsInfo.coarseChannelBandwidth = 1.25;
             xcorr.obsInfo.coarseChannel = 121;  // 150.625 MHz + 0.625 ch
+            // the same through run_batched (fused, the per-image images never formed)
+            CPacerImagerMI355X batched("", N, {5}, true, Polarization::XX, 2.0f);
+            batched.SetAntennaPositions(a.x, a.y, a.z, a.cable);
+            batched.m_fFrequencyMHz = -1;
+            batched.m_nConvolvingKernelSize = K;
+            batched._gridMode = mode;
+            Visibilities xb(MemoryBuffer<std::complex<float>>(xcorr0.size()), info, 1, 1);
+            std::memcpy((void*)xb.data(), vis0.data(), sizeof(float) * vis0.size());
+            xb.obsInfo.frequencyResolution = 0.625;
+            xb.obsInfo.coarseChannelBandwidth = 1.25;
+            xb.obsInfo.coarseChannel = 121;
+            Images avg_b = batched.run_batched(xb);
             Images avg = imager.run(xcorr);
-            if (avg.size() != 1) throw TestFailed("average_images must return one image");
+            if (avg.size() != 1 || avg_b.size() != 1)
+                throw TestFailed("average_images must return one image");
             // oracle: corrections, convolutional gridding, image, complex average
             std::vector<float> u(n_bl), v(n_bl), w(n_bl);
             oracle_zenith_uvw(a.n_ant, a.x.data(), a.y.data(), a.z.data(), u.data(), v.data(),
@@ -303,11 +318,16 @@ void test_imager_kaiser_bessel() {
             oracle_image_f32(grids.data(), counts.data(), N, (int)n_img, images.data(), 0);
             oracle_image_averaging_complex(images.data(), gs, n_img, ref.data());
             avg.to_cpu();
+            avg_b.to_cpu();
             const double err = peak_normalised_error(avg.data(),
                                                      (const std::complex<float>*)ref.data(), gs);
+            const double err_b = peak_normalised_error(avg_b.data(),
+                                                       (const std::complex<float>*)ref.data(), gs);
             std::cout << "Kaiser-Bessel K=" << K << (mode == GridMode::KaiserBesselKernel ? " sinc" : "")
-                      << ": peak-normalised error of the average vs oracle " << err << std::endl;
+                      << ": peak-normalised error of the average vs oracle " << err
+                      << " (run), " << err_b << " (run_batched)" << std::endl;
             if (!(err < 1e-5)) throw TestFailed("Kaiser-Bessel run differs from the oracle");
+            if (!(err_b < 1e-5)) throw TestFailed("Kaiser-Bessel run_batched differs from the oracle");
         }
 }
 
