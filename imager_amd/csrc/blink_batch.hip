@@ -46,7 +46,7 @@ __host__ __device__ constexpr int tile_stride(int n, int rows) {
 //   li (N/2 + 1) N + (x / W) (N/2 + 1) W + v W + x % W,   W = kInterBlock.
 // A transformed row is written as N / W segments of W * 8 = 256 bytes (whole
 // lines), and a column block of up to W columns reads one contiguous run.
-constexpr int kInterBlock = 32;
+constexpr int kInterBlock = kInterBlockCols;
 __device__ __forceinline__ int64_t inter_index(int n, int64_t li, int v, int x) {
     const int64_t h1 = n / 2 + 1;
     return li * h1 * n + (int64_t)(x / kInterBlock) * h1 * kInterBlock + (int64_t)v * kInterBlock +
@@ -215,7 +215,10 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         team_sync<false>();
         F::template run<1>(v, row, lane, TwTable{twl});
         const int vrow = rrow[team];
-        F::store(v, lane, [&](int i, float2 x) { store_stream(&inter[inter_index(N, li, vrow, i)], x); });
+        if (sd.cached_inter)
+            F::store(v, lane, [&](int i, float2 x) { inter[inter_index(N, li, vrow, i)] = x; });
+        else
+            F::store(v, lane, [&](int i, float2 x) { store_stream(&inter[inter_index(N, li, vrow, i)], x); });
     }
 }
 
@@ -256,7 +259,7 @@ k_cols_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __restric
     const float empty = sd.uniform ? __builtin_nanf("") : 0.f;
     float2* twl = tile + TEAMS * TS;  // twiddle table copy [N]
     for (int i = tid; i < N; i += NT) twl[i] = tw[i];
-    for (int i = tid; i <= H; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
+    for (int i = tid; i <= H; i += NT) nonempty[i] = sd.all_rows || rsg[i + 1] != rsg[i];
     __syncthreads();
 
     // stage z_t[p] and z_t[N - p] from buffer columns c0 + 2t, c0 + 2t + 1 of
@@ -578,7 +581,7 @@ k_cols_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     if (cb == 0 && tid == 0 && sd.count_sums) sd.count_sums[img] = (double)count;
     const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (N + 1);
     const float empty = sd.uniform ? __builtin_nanf("") : 0.f;  // rows without entries
-    for (int i = tid; i <= H; i += NT) nonempty[i] = rsg[i + 1] != rsg[i];
+    for (int i = tid; i <= H; i += NT) nonempty[i] = sd.all_rows || rsg[i + 1] != rsg[i];
     if (tid < 128) twl[tid] = tws[tid];
     __syncthreads();
 

@@ -55,6 +55,8 @@ constexpr size_t kMaxGraphs = 16;
 
 }  // namespace
 
+static constexpr int kPassEvents = 16;
+
 struct blink_ctx {
     int device = 0;
     hipDeviceProp_t props{};
@@ -72,6 +74,12 @@ struct blink_ctx {
     hipEvent_t fork_event = nullptr, join_event = nullptr;
     // the visibility compaction runs on aux_stream beside the plan
     hipEvent_t compact_fork = nullptr, compact_join = nullptr;
+    // chunked groups: the second passes run on pass_stream, one chunk behind
+    // the row passes; pass_ev[2k] marks chunk k's row pass done, pass_ev[2k+1]
+    // its second pass (k modulo kPassEvents)
+    hipStream_t pass_stream = nullptr;
+    hipEvent_t pass_fork = nullptr, pass_join = nullptr;
+    hipEvent_t pass_ev[2 * kPassEvents] = {};
     // blink_image_batch_noalloc: the device address of this GPU's fixed
     // region (null when it could not be found), the host source of the last
     // descriptor copy and an event recorded after that copy
@@ -187,14 +195,19 @@ int set_device(blink_ctx* ctx) {
 // Streams and events of a context, each destroyed once (creation failures
 // leave the later handles null).
 void destroy_streams(blink_ctx* ctx) {
-    for (hipStream_t* s : {&ctx->capture_stream, &ctx->aux_stream}) {
+    for (hipStream_t* s : {&ctx->capture_stream, &ctx->aux_stream, &ctx->pass_stream}) {
         if (*s) (void)hipStreamDestroy(*s);
         *s = nullptr;
     }
     for (hipEvent_t* e : {&ctx->fork_event, &ctx->join_event, &ctx->compact_fork,
-                          &ctx->compact_join, &ctx->noalloc_copied}) {
+                          &ctx->compact_join, &ctx->noalloc_copied, &ctx->pass_fork,
+                          &ctx->pass_join}) {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
+    }
+    for (hipEvent_t& e : ctx->pass_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
     }
 }
 
@@ -379,7 +392,13 @@ BLINK_API int blink_create(int device, blink_ctx** out) {
         hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->compact_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->compact_join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->noalloc_copied, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->noalloc_copied, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->pass_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->pass_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->pass_join, hipEventDisableTiming) != hipSuccess ||
+        !std::all_of(std::begin(ctx->pass_ev), std::end(ctx->pass_ev), [](hipEvent_t& e) {
+            return hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        })) {
         destroy_streams(ctx);
         delete ctx;
         return BLINK_ERR_HIP;
@@ -787,13 +806,19 @@ struct BatchGroup {
     int transform = BLINK_TRANSFORM_FFT;
     bool uniform = false;        // "U" weighting
     int kb_size = 0, kb_mode = 0;  // convolutional gridding (generic kernels)
-    bool reg = false;            // register-FFT kernels (power-of-two sides 128..2048)
+    bool reg = false;            // register-FFT kernels (power-of-two sides 128..4096)
+    bool kb_blocked = false;     // convolutional rows into the register second pass
     std::vector<int> sets;       // call indices of the sets, in descriptor order
     int first_desc = 0;          // index of the group's first SetDesc in the table
     int64_t n_img = 0;
     int chunk = 1;               // FFT groups: images per row/column launch pair
     size_t img_set_off = 0;      // blob offset of the image -> set table (2+ sets)
     int compact_max_bl = 0;      // largest n_bl of its sets with compacted visibilities
+    // second passes on the context's pass stream, a chunk behind the row
+    // passes, with two row-pass buffers (inter, inter2) used in turn
+    bool overlap = false;
+    // the register row pass stores its chunks with plain (cached) stores
+    bool cached_inter = false;
 };
 
 struct BatchPlan {
@@ -808,10 +833,14 @@ struct BatchPlan {
     // several groups: the group of index `main` (most image bytes) uses inter
     // on the call's stream, the others inter_aux on the aux stream
     float* inter_aux = nullptr;
+    float* inter2 = nullptr;  // the main group's second buffer (overlap)
     int main = 0;
     // environment switches read for this call (part of the graph key)
     int64_t inter_budget = 0;
-    bool compact_on = true;
+    bool compact_on = true, compact_dense = false;
+    int cached_mode = -1;       // BLINK_INTER_CACHED: 0 never, 1 always, -1 by chunk size
+    int overlap_chunks = -1;    // BLINK_OVERLAP_CHUNKS: chunks of the main group (-1: by size)
+    int64_t overlap_bytes = 0;  // BLINK_OVERLAP_MIB: target chunk bytes of an overlapped group
     // device copy of the blob, when build_batch placed it (arena)
     void* blob_dev = nullptr;
     // host copy of a group's first set descriptor (a kernel argument)
@@ -853,19 +882,40 @@ static int batch_buffer(blink_ctx* ctx, Arena* arena, const char* name, size_t b
 // reserved here, before any capture.  With an arena, the buffers and the
 // descriptor blob (bp.blob_dev) are cut from it and the row-pass chunks are
 // sized to what remains.
+// Row-pass chunks of a register side up to 2048 (the main group of a call)
+// that the Infinity Cache (256 MiB) can hold between the two passes: the row
+// pass stores them with plain stores, the second pass reads them back from
+// the cache, and each chunk's second pass runs beside the next chunk's row
+// pass.  kOverlapChunkBytes is the target chunk size.  Only groups whose
+// images share their plans (two or more timesteps per channel on average)
+// are overlapped: where every image has a plan of its own, the row pass
+// gathers from cold plan entries and the overlapped passes measured slower
+// than one pass after the other (768 channels x 1 timestep at 1024^2: 336k
+// against 356k images/s; 64 x 10 at 1024^2: 416k against 376k).
+static constexpr int64_t kOverlapChunkBytes = 168ll << 20;
+static constexpr int64_t kCachedChunkBytes = 256ll << 20;
+
 static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, BatchPlan& bp,
                        Arena* arena = nullptr) {
     int rc;
-    // Row-pass buffer: up to 4 GiB of row-transformed grids per chunk.  Fewer,
-    // larger launches win over chunks small enough to stay in the 256 MiB
-    // Infinity Cache between the passes: at 1024^2 one 640-image chunk runs
-    // 241k images/s, 12-image chunks 181k (launch ramp and tail of the row
-    // kernel).  BLINK_INTER_MIB overrides the budget, for tuning.
+    // Row-pass buffer: up to 4 GiB of row-transformed grids per chunk
+    // (BLINK_INTER_MIB overrides the budget, for tuning).  Chunks run one
+    // after the other unless they are overlapped (below): small chunks alone
+    // lose to one large launch (row-kernel ramp and tail per launch).
     int64_t inter_budget = 4096ll << 20;
     if (const char* e = getenv("BLINK_INTER_MIB")) {
         const long long mib = atoll(e);
         if (mib > 0) inter_budget = mib << 20;
     }
+    // Overlapped chunks (see kOverlapChunkBytes); the switches are for tuning:
+    // BLINK_OVERLAP_MIB (target chunk MiB, 0: no overlap), BLINK_OVERLAP_CHUNKS
+    // (a fixed chunk count for the main FFT group, any side; 0 or 1: none),
+    // BLINK_INTER_CACHED (0 / 1: streaming / plain row-pass stores always).
+    bp.overlap_bytes = kOverlapChunkBytes;
+    if (const char* e = getenv("BLINK_OVERLAP_MIB")) bp.overlap_bytes = (int64_t)atoll(e) << 20;
+    if (const char* e = getenv("BLINK_OVERLAP_CHUNKS")) bp.overlap_chunks = std::max(0, atoi(e));
+    if (const char* e = getenv("BLINK_INTER_CACHED")) bp.cached_mode = e[0] == '1' ? 1 : 0;
+    if (arena) bp.overlap_chunks = 0, bp.overlap_bytes = 0;  // one row-pass buffer
     // groups (grid side, transform, weighting, gridding kernel) in order of
     // first appearance
     std::map<std::tuple<int, int, int, int, int>, int> gidx;
@@ -883,8 +933,12 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             g.uniform = sets[i].weighting == BLINK_WEIGHTING_UNIFORM;
             g.kb_size = kb;
             g.kb_mode = kb ? sets[i].kernel_mode : 0;
-            // convolutional gridding runs on the generic kernels, any side
+            // convolutional gridding runs its row pass on the generic kernels,
+            // any side; at the register sides up to 2048 it writes their
+            // column-blocked buffer, and the register second pass reads it
             g.reg = g.transform == BLINK_TRANSFORM_FFT && batch_reg_supported(g.n) && kb == 0;
+            g.kb_blocked = g.transform == BLINK_TRANSFORM_FFT && kb > 0 &&
+                           batch_reg_supported(g.n) && g.n <= 2048;
             bp.groups.push_back(g);
         }
         BatchGroup& g = bp.groups[it->second];
@@ -940,12 +994,14 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     {
         const char* env = getenv("BLINK_COMPACT");
         const bool on = !(env && env[0] == '0');
+        const bool dense = env && env[0] == '2';  // every set of a register side
         bp.compact_on = on;
+        bp.compact_dense = dense;
         for (BatchGroup& g : bp.groups) {
             if (!on || !g.reg || g.n > 2048) continue;
             for (int i : g.sets) {
                 const int64_t n_bl = (int64_t)sets[i].n_ant * (sets[i].n_ant + 1) / 2;
-                if (n_bl > 64 * (int64_t)(g.n / 2 + 1)) continue;
+                if (!dense && n_bl > 64 * (int64_t)(g.n / 2 + 1)) continue;
                 cp_off[i] = cp_bytes;
                 cp_bytes += align16(sizeof(float2) * (size_t)sets[i].n_ch * sets[i].n_t *
                                     (size_t)n_bl);
@@ -992,6 +1048,21 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     }
     bp.inter_budget = inter_budget;
 
+    // the main group (most image bytes) keeps its own row-pass buffer (two
+    // with overlap); the others share a second one on the aux stream
+    {
+        int64_t best = -1;
+        for (size_t k = 0; k < bp.groups.size(); k++) {
+            const int64_t bytes = bp.groups[k].n_img * (int64_t)bp.groups[k].n * bp.groups[k].n;
+            if (bytes > best) best = bytes, bp.main = (int)k;
+        }
+    }
+    // plans of a group: one per (set, channel)
+    auto group_channels = [&](const BatchGroup& g) {
+        int64_t c = 0;
+        for (int i : g.sets) c += sets[i].n_ch;
+        return c;
+    };
     // row-pass chunks: images per row / column launch pair
     std::vector<size_t> group_inter(bp.groups.size(), 16);
     for (BatchGroup& g : bp.groups) {
@@ -1007,28 +1078,41 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             continue;
         }
         const int64_t per = g.reg ? batch_inter_bytes(g.n) : 8 * (n / 2 + 1) * n;
-        g.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(g.n_img, inter_budget / per));
+        // chunks of the main group: a fixed count, or (register sides up to
+        // 2048) the target size when the images need more than one chunk of it
+        int chunks = 0;
+        if ((int)gk == bp.main && g.n_img >= 2) {
+            if (bp.overlap_chunks >= 0)
+                chunks = bp.overlap_chunks;
+            else if (g.reg && g.n <= 2048 && bp.overlap_bytes > 0 &&
+                     per * g.n_img > bp.overlap_bytes && g.n_img >= 2 * group_channels(g))
+                chunks = (int)((per * g.n_img + bp.overlap_bytes - 1) / bp.overlap_bytes);
+        }
+        const bool overlap = chunks >= 2;
+        // with overlap the budget holds both buffers
+        const int64_t budget = overlap ? inter_budget / 2 : inter_budget;
+        g.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(g.n_img, budget / per));
+        if (overlap) {
+            g.chunk = (int)std::min<int64_t>(g.chunk, (g.n_img + chunks - 1) / chunks);
+            g.overlap = g.chunk < g.n_img;
+        }
+        g.cached_inter = bp.cached_mode >= 0 ? bp.cached_mode == 1
+                                             : per * g.chunk <= kCachedChunkBytes;
         group_inter[gk] = (size_t)(per * g.chunk);
     }
-    // the main group (most image bytes) keeps its own row-pass buffer; the
-    // others share a second one on the aux stream
-    size_t inter_bytes = 16;
+    size_t inter_bytes = 16, main_bytes = align256(group_inter[bp.main]);
     {
-        int64_t best = -1;
-        for (size_t k = 0; k < bp.groups.size(); k++) {
-            const int64_t bytes = bp.groups[k].n_img * (int64_t)bp.groups[k].n * bp.groups[k].n;
-            if (bytes > best) best = bytes, bp.main = (int)k;
-        }
         size_t aux = 0;
         for (size_t k = 0; k < bp.groups.size(); k++)
             if ((int)k != bp.main) aux = std::max(aux, group_inter[k]);
-        inter_bytes = align256(group_inter[bp.main]) + aux;
+        inter_bytes = main_bytes * (bp.groups[bp.main].overlap ? 2 : 1) + aux;
     }
     void* inter;
     if ((rc = batch_buffer(ctx, arena, "batch_inter", inter_bytes, &inter))) return rc;
     bp.cells = (uint32_t*)cl;
     bp.inter = (float*)inter;
-    bp.inter_aux = (float*)((char*)inter + align256(group_inter[bp.main]));
+    bp.inter2 = bp.groups[bp.main].overlap ? (float*)((char*)inter + main_bytes) : nullptr;
+    bp.inter_aux = (float*)((char*)inter + main_bytes * (bp.inter2 ? 2 : 1));
     bp.partial = (int*)pa;
     bp.sc.resize(n_sets);
     for (int i = 0; i < n_sets; i++) {
@@ -1118,6 +1202,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.kb_size = st.kernel_size;
             d.kb_table = bp.sc[i].kb_table;
             d.all_rows = st.kernel_size > 0 ? 1 : 0;
+            d.cached_inter = g.cached_inter ? 1 : 0;
             base += d.n_img;
             desc_of[i] = (int)desc.size();
             desc.push_back(d);
@@ -1264,12 +1349,51 @@ static int enqueue_plans(blink_ctx* ctx, const BatchPlan& bp, const BatchTables&
     return BLINK_OK;
 }
 
-// Enqueues one group's launches on s, with row-pass buffer `inter`.
+// Work forked from s onto the aux stream: fork() makes the aux stream wait
+// for s; join() (also on every error path after a successful fork) makes s
+// wait for everything enqueued on the aux stream since, so that a capture on
+// s never ends with the aux stream still part of it.
+struct AuxFork {
+    blink_ctx* ctx;
+    hipStream_t s, a;
+    hipEvent_t fork_ev, join_ev;
+    bool forked = false;
+    int fork() {
+        hipError_t e = hipEventRecord(fork_ev, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(a, fork_ev, 0);
+        if (e != hipSuccess)
+            return ctx->fail(BLINK_ERR_HIP, "fork to the second stream failed: %s",
+                             hipGetErrorString(e));
+        forked = true;
+        return BLINK_OK;
+    }
+    int join() {
+        if (!forked) return BLINK_OK;
+        forked = false;
+        hipError_t e = hipEventRecord(join_ev, a);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, join_ev, 0);
+        if (e != hipSuccess)
+            return ctx->fail(BLINK_ERR_HIP, "join of the second stream failed: %s",
+                             hipGetErrorString(e));
+        return BLINK_OK;
+    }
+    // keeps the first error: a failure inside the forked work, else the join's
+    int join_after(int rc) {
+        const int rj = join();
+        return rc ? rc : rj;
+    }
+};
+
+static int enqueue_chunks(blink_ctx* ctx, const blink_image_set* sets, const BatchPlan& bp,
+                          const BatchTables& t, const BatchGroup& g, float* inter,
+                          float* inter2, hipStream_t s, hipStream_t p);
+
+// Enqueues one group's launches on s, with row-pass buffer `inter` (and, for
+// an overlapped group, `inter2` and the context's pass stream).
 static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const BatchPlan& bp,
                          const BatchTables& t, const BatchGroup& g, float* inter,
-                         hipStream_t s) {
+                         hipStream_t s, float* inter2 = nullptr) {
     const int n = g.n;
-    const float* tw = ctx->twiddles[n];
     if (is_dft(g.transform)) {
         // per set and chunk: half plane -> stage 1 (complex) -> stage 2 (real, scaled)
         const int prec = dft_prec(g.transform);
@@ -1306,6 +1430,24 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
         }
         return BLINK_OK;
     }
+    if (!g.overlap || !inter2) return enqueue_chunks(ctx, sets, bp, t, g, inter, inter, s, s);
+    // overlapped: fork the pass stream from s, join it back at the end (also
+    // after a failure, so that a capture on s never ends with it attached)
+    AuxFork pf{ctx, s, ctx->pass_stream, ctx->pass_fork, ctx->pass_join};
+    int rc = pf.fork();
+    if (!rc) rc = enqueue_chunks(ctx, sets, bp, t, g, inter, inter2, s, ctx->pass_stream);
+    return pf.join_after(rc);
+}
+
+// The FFT group's chunks: row passes on s, second passes and averages on p.
+// With p != s, chunk k uses buffer k % 2 (inter, inter2): its second pass
+// waits for its row pass, and the row pass of chunk k + 2 waits for it.
+static int enqueue_chunks(blink_ctx* ctx, const blink_image_set* sets, const BatchPlan& bp,
+                          const BatchTables& t, const BatchGroup& g, float* inter,
+                          float* inter2, hipStream_t s, hipStream_t p) {
+    const int n = g.n;
+    const float* tw = ctx->twiddles[n];
+    const bool two = p != s;
     const SetDesc* sd = t.group_sets(g);
     const SetDesc& first = bp.host_desc(g);
     const int* is = t.img_set(g);
@@ -1319,22 +1461,35 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
     for (int i : g.sets) any_images |= sets[i].images != nullptr;
     auto second_pass = [&](const SetDesc& f, const SetDesc* tab, const int* img_set, int n_sets,
                            int64_t gi0, int cnt, const float* src) {
-        return g.reg ? launch_cols_reg(f, tab, img_set, n_sets, n, tl.second, gi0, cnt, src, tw, s)
-                     : launch_cols_half(f, tab, img_set, n_sets, plan, gi0, cnt, src, tw, s);
+        return g.reg || g.kb_blocked
+                   ? launch_cols_reg(f, tab, img_set, n_sets, n, tl.second, gi0, cnt, src, tw, p)
+                   : launch_cols_half(f, tab, img_set, n_sets, plan, gi0, cnt, src, tw, p);
     };
-    for (int64_t gi0 = 0; gi0 < g.n_img; gi0 += g.chunk) {
+    int k = 0;
+    for (int64_t gi0 = 0; gi0 < g.n_img; gi0 += g.chunk, k++) {
         const int cnt = (int)std::min<int64_t>(g.chunk, g.n_img - gi0);
+        float* buf = two && (k & 1) ? inter2 : inter;
+        hipEvent_t rows_done = ctx->pass_ev[2 * (k % kPassEvents)];
+        hipEvent_t pass_done = ctx->pass_ev[2 * (k % kPassEvents) + 1];
+        // the buffer's previous chunk (k - 2) must have been read
+        if (two && k >= 2)
+            HIP_TRY(hipStreamWaitEvent(s, ctx->pass_ev[2 * ((k - 2) % kPassEvents) + 1], 0));
         if (g.reg)
             LAUNCH("grid_rows_fft", s,
-                   launch_grid_rows_reg(first, sd, is, ns, n, tl.first, gi0, cnt, inter, tw, s,
+                   launch_grid_rows_reg(first, sd, is, ns, n, tl.first, gi0, cnt, buf, tw, s,
                                         g.uniform));
         else if (g.kb_size)
             LAUNCH("grid_rows_kb", s,
-                   launch_grid_rows_kb(first, sd, is, ns, plan, gi0, cnt, inter, tw, s));
+                   launch_grid_rows_kb(first, sd, is, ns, plan, gi0, cnt, buf, tw, g.kb_blocked,
+                                       s));
         else
             LAUNCH("grid_rows_fft", s,
-                   launch_grid_rows_fft(first, sd, is, ns, plan, gi0, cnt, inter, tw, s));
-        if (any_images) LAUNCH("fft_cols", s, second_pass(first, sd, is, ns, gi0, cnt, inter));
+                   launch_grid_rows_fft(first, sd, is, ns, plan, gi0, cnt, buf, tw, s));
+        if (two) {
+            HIP_TRY(hipEventRecord(rows_done, s));
+            HIP_TRY(hipStreamWaitEvent(p, rows_done, 0));
+        }
+        if (any_images) LAUNCH("fft_cols", p, second_pass(first, sd, is, ns, gi0, cnt, buf));
         // sets with an average: add this chunk's images of the set
         for (size_t j = 0; j < g.sets.size(); j++) {
             const SetScratch& sc = bp.sc[g.sets[j]];
@@ -1343,10 +1498,11 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
             const int64_t a = std::max<int64_t>(gi0, d.img_base);
             const int64_t b = std::min<int64_t>(gi0 + cnt, d.img_base + d.n_img);
             if (a >= b) continue;
-            LAUNCH("average_rows", s,
-                   launch_average_rows(d, a - d.img_base, (int)(b - a), a - gi0, inter,
-                                       sc.avg_acc, a > d.img_base, sc.avg_meta, s));
+            LAUNCH("average_rows", p,
+                   launch_average_rows(d, a - d.img_base, (int)(b - a), a - gi0, buf,
+                                       sc.avg_acc, a > d.img_base, sc.avg_meta, p));
         }
+        if (two) HIP_TRY(hipEventRecord(pass_done, p));
     }
     // the averages: one second pass each over the summed rows, scaled by 1/n_img
     for (size_t j = 0; j < g.sets.size(); j++) {
@@ -1361,45 +1517,10 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
         d.img_base = 0;
         d.n_img = 1;
         d.uniform = 0;
-        LAUNCH("fft_cols", s, second_pass(d, nullptr, nullptr, 1, 0, 1, sc.avg_acc));
+        LAUNCH("fft_cols", p, second_pass(d, nullptr, nullptr, 1, 0, 1, sc.avg_acc));
     }
     return BLINK_OK;
 }
-
-// Work forked from s onto the aux stream: fork() makes the aux stream wait
-// for s; join() (also on every error path after a successful fork) makes s
-// wait for everything enqueued on the aux stream since, so that a capture on
-// s never ends with the aux stream still part of it.
-struct AuxFork {
-    blink_ctx* ctx;
-    hipStream_t s, a;
-    hipEvent_t fork_ev, join_ev;
-    bool forked = false;
-    int fork() {
-        hipError_t e = hipEventRecord(fork_ev, s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(a, fork_ev, 0);
-        if (e != hipSuccess)
-            return ctx->fail(BLINK_ERR_HIP, "fork to the second stream failed: %s",
-                             hipGetErrorString(e));
-        forked = true;
-        return BLINK_OK;
-    }
-    int join() {
-        if (!forked) return BLINK_OK;
-        forked = false;
-        hipError_t e = hipEventRecord(join_ev, a);
-        if (e == hipSuccess) e = hipStreamWaitEvent(s, join_ev, 0);
-        if (e != hipSuccess)
-            return ctx->fail(BLINK_ERR_HIP, "join of the second stream failed: %s",
-                             hipGetErrorString(e));
-        return BLINK_OK;
-    }
-    // keeps the first error: a failure inside the forked work, else the join's
-    int join_after(int rc) {
-        const int rj = join();
-        return rc ? rc : rj;
-    }
-};
 
 static int enqueue_compaction(blink_ctx* ctx, const BatchPlan& bp, const BatchTables& t,
                               hipStream_t a) {
@@ -1431,14 +1552,15 @@ static int enqueue_batch(blink_ctx* ctx, const blink_image_set* sets, const Batc
     if (compact && !(rc = cf.fork())) rc = enqueue_compaction(ctx, bp, t, a);
     if (!rc) rc = enqueue_plans(ctx, bp, t, s);
     if ((rc = cf.join_after(rc))) return rc;
-    if (bp.groups.size() == 1) return enqueue_group(ctx, sets, bp, t, bp.groups[0], bp.inter, s);
+    if (bp.groups.size() == 1)
+        return enqueue_group(ctx, sets, bp, t, bp.groups[0], bp.inter, s, bp.inter2);
     AuxFork gf{ctx, s, a, ctx->fork_event, ctx->join_event};
     if ((rc = gf.fork())) return rc;
     for (size_t k = 0; k < bp.groups.size() && !rc; k++)
         if ((int)k != bp.main) rc = enqueue_group(ctx, sets, bp, t, bp.groups[k], bp.inter_aux, a);
     if (rc) return gf.join_after(rc);
     // the main group runs on s while the aux stream works; join at the end
-    rc = enqueue_group(ctx, sets, bp, t, bp.groups[bp.main], bp.inter, s);
+    rc = enqueue_group(ctx, sets, bp, t, bp.groups[bp.main], bp.inter, s, bp.inter2);
     return gf.join_after(rc);
 }
 
@@ -1476,6 +1598,10 @@ static std::string graph_key(const blink_image_set* sets, int n_sets, const Batc
     put(n_sets);
     put(bp.inter_budget);
     put(bp.compact_on);
+    put(bp.compact_dense);
+    put(bp.cached_mode);
+    put(bp.overlap_chunks);
+    put(bp.overlap_bytes);
     for (int i = 0; i < n_sets; i++) {
         const blink_image_set& st = sets[i];
         put(st.n_ant), put(st.n_ch), put(st.n_t);

@@ -1472,12 +1472,13 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
 // reach back below N/2); each entry adds its own footprint and, for rows 1 ..
 // N/2-1, its partner's, keeping the cells in rows r0 .. r1.  Counts stay at
 // the nearest cells, as the reference keeps them.  Every row of the tile is
-// transformed and written.
+// transformed and written, row-major or, for the register sides, in the
+// column-blocked layout of the register second pass.
 template <int S, bool kBlockTeam, bool kMulti>
 __global__ void __launch_bounds__(kFftThreads)
 k_grid_rows_kb(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
                FftPlan plan, int team, int rb, int64_t gi0, float2* __restrict__ inter,
-               const float2* __restrict__ tw) {
+               const float2* __restrict__ tw, int blocked) {
 #pragma clang fp contract(off)
     extern __shared__ float4 grk_lds4[];
     const int n = plan.n;
@@ -1563,20 +1564,30 @@ k_grid_rows_kb(SetDesc first, const SetDesc* __restrict__ sets, const int* __res
     if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();
     const int rounds = kBlockTeam ? nr : (nr + teams - 1) / teams;
+    const int64_t h1 = h + 1;
     for (int k = 0; k < rounds; k++) {
         const int r = kBlockTeam ? k : k * teams + t_id;
         if (!kBlockTeam && r >= nr) break;
         float2* row = tile + (size_t)r * n;
         const float2* res = fft_row<S, kBlockTeam>(row, tmp, plan, tw, lane, team);
-        float2* dst = inter + ((int64_t)li * (h + 1) + r0 + r) * n;
-        for (int i = lane; i < n; i += team) store_stream(&dst[i], res[i]);
+        const int v = r0 + r;
+        if (blocked) {  // element (v, x) at ((x / 32) (N/2+1) + v) 32 + x % 32
+            float2* dst = inter + (int64_t)li * h1 * n + (int64_t)v * kInterBlockCols;
+            for (int i = lane; i < n; i += team)
+                store_stream(&dst[(int64_t)(i / kInterBlockCols) * h1 * kInterBlockCols +
+                                  i % kInterBlockCols],
+                             res[i]);
+        } else {
+            float2* dst = inter + ((int64_t)li * h1 + v) * n;
+            for (int i = lane; i < n; i += team) store_stream(&dst[i], res[i]);
+        }
         team_sync<kBlockTeam>();
     }
 }
 
 hipError_t launch_grid_rows_kb(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
-                               float* inter, const float* tw, hipStream_t stream) {
+                               float* inter, const float* tw, bool blocked, hipStream_t stream) {
     if (n_img == 0) return hipSuccess;
     if (first.kb_size < 1) return hipErrorInvalidValue;
     const int n = plan.n;
@@ -1588,7 +1599,9 @@ hipError_t launch_grid_rows_kb(const SetDesc& first, const SetDesc* sets, const 
     const int64_t want = std::max<int64_t>(1, (int64_t)(n / 2 + 1) * n_img / 2048);
     int rb = (int)std::max<int64_t>(
         1, std::min<int64_t>({(int64_t)grid_rows_tile_rows(n), want, 8}));
-    const size_t lds = sizeof(float2) * ((size_t)rb * n + (size_t)teams * n);
+    auto lds_of = [&](int r) { return sizeof(float2) * ((size_t)r * n + (size_t)teams * n); };
+    while (rb > 1 && !lds_fits(lds_of(rb))) rb--;
+    const size_t lds = lds_of(rb);
     if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
     const int n_rb = (n / 2 + 1 + rb - 1) / rb;
     const int64_t blocks = (int64_t)n_rb * n_img;
@@ -1598,7 +1611,8 @@ hipError_t launch_grid_rows_kb(const SetDesc& first, const SetDesc* sets, const 
                  ? (multi ? k_grid_rows_kb<1, true, true> : k_grid_rows_kb<1, true, false>)
                  : (multi ? k_grid_rows_kb<1, false, true> : k_grid_rows_kb<1, false, false>);
     hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kFftThreads), lds, stream, first, sets,
-                       img_set, plan, team, rb, gi0, (float2*)inter, (const float2*)tw);
+                       img_set, plan, team, rb, gi0, (float2*)inter, (const float2*)tw,
+                       blocked ? 1 : 0);
     return hipGetLastError();
 }
 

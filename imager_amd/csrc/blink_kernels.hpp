@@ -16,6 +16,8 @@ constexpr int kMaxBaselines = 1 << 18;  // plan packs the baseline in 18 bits
 constexpr int kPlanThreads = 1024;
 constexpr int kTileThreads = 256;
 constexpr int kFftThreads = 256;
+// Columns per block of the register kernels' column-blocked row-pass buffer.
+constexpr int kInterBlockCols = 32;
 constexpr int kMaxTileRows = 16;
 constexpr int kMaxColWidth = 16;
 
@@ -86,6 +88,10 @@ struct SetDesc {
     const double* kb_table;
     int kb_size;
     int all_rows;
+    // the register row pass stores the row-pass buffer with plain (cached)
+    // stores instead of streaming ones, so that a chunk small enough for the
+    // Infinity Cache can be read back from it by the second pass
+    int cached_inter;
 };
 
 // The set of group image gi, by value.  Kernels are built twice: for a
@@ -181,13 +187,15 @@ hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int
                             int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
                             const float* inter, const float* tw, hipStream_t stream);
 // Row pass of convolutional (Kaiser-Bessel) gridding on the generic kernels:
-// every plan row 0 .. N/2 of every image is gridded, transformed and written
-// (row-major, as launch_grid_rows_fft writes); a tile gathers the plan
-// entries of its rows +- K/2 and, near row N/2, of the rows whose Hermitian
-// partners' footprints reach it.  The sets' kb_size must be > 0.
+// every plan row 0 .. N/2 of every image is gridded, transformed and written,
+// row-major as launch_grid_rows_fft writes or, with `blocked`, in the
+// column-blocked layout of the register second pass (N = 128 .. 2048); a
+// tile gathers the plan entries of its rows +- K/2 and, near row N/2, of the
+// rows whose Hermitian partners' footprints reach it.  The sets' kb_size must
+// be > 0.
 hipError_t launch_grid_rows_kb(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                int n_sets, const FftPlan& plan, int64_t gi0, int n_img,
-                               float* inter, const float* tw, hipStream_t stream);
+                               float* inter, const float* tw, bool blocked, hipStream_t stream);
 // Gridding plans of a batched call: one unit per (set, channel); `parts`
 // workgroups per unit split its baselines (at most kPlanPartMax each).  A plan
 // lists the cells of rows 0 .. N/2 only (each baseline's cell and conjugate

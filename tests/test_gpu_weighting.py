@@ -242,3 +242,74 @@ def test_batch_argument_checks(ctx):
         st, keep = _set(o, images, **kw)
         with pytest.raises(capi.BlinkInvalidArgument):
             ctx.image_batch([st])
+
+
+@pytest.mark.parametrize("n,overlap_mib", [(1024, 10), (256, 1)])
+def test_overlapped_chunks(ctx, n, overlap_mib):
+    """Overlapped chunks (each chunk's second pass on the pass stream beside
+    the next chunk's row pass, two row-pass buffers in turn, cached row-pass
+    stores), forced by a small target chunk size on sets with several
+    timesteps per channel: one group of two sets, one of them also averaged
+    (its row sums accumulate across chunks on the pass stream), then a
+    uniformly weighted set; every image, count sum and mean against the
+    oracle, on capture, on replay and eagerly (where the launches are counted:
+    several row / second-pass pairs)."""
+    import torch
+    if n == 1024:
+        a = synth.mwa_array(n_ch=3, n_t=3, n_pixels=1024, seed=81)
+        b = synth.mwa_array(n_ch=2, n_t=2, n_pixels=1024, seed=82)
+    else:
+        a = synth.small(n_ant=48, n_ch=4, n_t=3, n_pixels=n, seed=83, height_sigma_m=1.0)
+        b = synth.small(n_ant=64, n_ch=2, n_t=2, n_pixels=n, seed=84, height_sigma_m=1.0)
+    per_image = 8 * (n // 2 + 1) * n
+    n_img = a.n_images + b.n_images
+    assert n_img * per_image > 3 * (overlap_mib << 20), "3+ chunks wanted"
+    assert n_img >= 2 * (a.n_ch + b.n_ch), "plans shared by 2+ images each"
+    ra, ca = _reference(a, uniform=False)
+    rb, cb = _reference(b, uniform=False)
+    img_a = torch.empty((a.n_t, a.n_ch, n, n), dtype=torch.complex64, device="cuda")
+    img_b = torch.empty((b.n_t, b.n_ch, n, n), dtype=torch.complex64, device="cuda")
+    avg_a = torch.empty((n, n), dtype=torch.complex64, device="cuda")
+    sums_a = torch.empty(a.n_images, dtype=torch.float64, device="cuda")
+    sums_b = torch.empty(b.n_images, dtype=torch.float64, device="cuda")
+    sta, ka = _set(a, img_a, sums_a, average=avg_a)
+    stb, kb = _set(b, img_b, sums_b)
+    u = covering_array(n if n <= 256 else 256, n_ch=3, n_t=2, seed=85)
+    ru, _ = _reference(u, uniform=True)
+    img_u = torch.empty((u.n_t, u.n_ch, u.n_pixels, u.n_pixels), dtype=torch.complex64,
+                        device="cuda")
+    stu, ku = _set(u, img_u, weighting="U")
+    old = os.environ.get("BLINK_OVERLAP_MIB")
+    os.environ["BLINK_OVERLAP_MIB"] = str(overlap_mib)
+    try:
+        for rep in range(3):   # capture, replay, then eagerly with the launches counted
+            eager = rep == 2
+            ctx.profile_enable(eager)
+            ctx.profile_reset()
+            for t in (img_a, img_b, avg_a, img_u):
+                t.fill_(float("nan"))
+            _say(f"overlapped chunks N={n} target {overlap_mib} MiB, rep {rep}")
+            ctx.image_batch([sta, stb])
+            ea = oracle.peak_normalised_error(_host(img_a), ra)[0]
+            eb = oracle.peak_normalised_error(_host(img_b), rb)[0]
+            _say(f"  set a {ea:.2e}, set b {eb:.2e}")
+            assert ea < IMAGE_TOL and eb < IMAGE_TOL
+            assert np.array_equal(_host(sums_a), ca.reshape(a.n_images, -1).sum(1).astype(np.float64))
+            assert np.array_equal(_host(sums_b), cb.reshape(b.n_images, -1).sum(1).astype(np.float64))
+            _check_average(_host(avg_a), ra)
+            if eager:
+                launches = ctx.profile()
+                _say(f"  launches {launches}")
+                assert launches["grid_rows_fft"][0] >= 3
+                # the images' second passes and the average's
+                assert launches["fft_cols"][0] == launches["grid_rows_fft"][0] + 1
+            ctx.image_batch([stu])
+            eu = oracle.peak_normalised_error(_host(img_u), ru)[0]
+            _say(f"  uniform {eu:.2e}")
+            assert eu < IMAGE_TOL
+    finally:
+        ctx.profile_enable(False)
+        if old is None:
+            os.environ.pop("BLINK_OVERLAP_MIB", None)
+        else:
+            os.environ["BLINK_OVERLAP_MIB"] = old
