@@ -34,7 +34,7 @@ FFT_FORWARD, FFT_BACKWARD = -1, 1
 GRID_TILED, GRID_ATOMIC = 0, 1
 KERNEL_KAISER_BESSEL, KERNEL_KAISER_BESSEL_NO_SINC = 1, 2
 TRANSFORM_FFT, TRANSFORM_DFT_BF16X3, TRANSFORM_DFT_FP16X3 = 0, 1, 2
-MAX_PIXELS = 4096
+MAX_PIXELS = 8192
 
 # Every symbol include/blink_mi355x.h declares.
 EXPORTED_SYMBOLS = (

@@ -699,9 +699,14 @@ static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
 
 // Built variants: plan rows per row tile, and output lines per second-pass
 // block (image rows up to N = 2048, image columns at 4096; two per team).
+// Small tiles keep few images' visibility records in flight per XCD: dense
+// grids gather each image's records at random, and an XCD's 4 MiB L2 holds
+// only a few images' records (768 x 256^2, 32896 baselines: rows per tile
+// 32, 16, 8, 4 ran 509, 477, 447, 412 us, and 4 rows fetched 2.4x the
+// record bytes).
 #define BLINK_ROW_VARIANTS(X) \
-    X(128, 8) X(128, 16) X(128, 32) \
-    X(256, 4) X(256, 8) X(256, 16) X(256, 32) \
+    X(128, 4) X(128, 8) X(128, 16) X(128, 32) \
+    X(256, 1) X(256, 2) X(256, 4) X(256, 8) X(256, 16) X(256, 32) \
     X(512, 2) X(512, 4) X(512, 8) X(512, 16) \
     X(1024, 4) X(1024, 8) X(1024, 16) \
     X(2048, 4) X(2048, 8)

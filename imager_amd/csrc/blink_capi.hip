@@ -308,7 +308,8 @@ static int64_t dft_inter_bytes(int64_t n) {
 int check_fft_size(blink_ctx* ctx, int n, FftPlan& plan) {
     if (!make_fft_plan(n, plan))
         return ctx->fail(BLINK_ERR_UNSUPPORTED,
-                         "grid side %d is not supported (need 2 <= N <= %d with prime factors 2, 3, 5)",
+                         "grid side %d is not supported (need 2 <= N <= %d with prime factors "
+                         "2, 3, 5, 7, 11, 13)",
                          n, kMaxPixels);
     return BLINK_OK;
 }
@@ -1692,6 +1693,13 @@ static int check_sets(blink_ctx* ctx, const blink_image_set* sets, int n_sets) {
         }
         FftPlan plan;
         if ((rc = check_fft_size(ctx, st.n_pixels, plan))) return rc;
+        // above 4096 the generic row tile holds one row and its FFT scratch;
+        // the per-cell counts of uniform weighting no longer fit beside them
+        if (st.weighting == BLINK_WEIGHTING_UNIFORM && st.n_pixels > 4096)
+            return ctx->fail(BLINK_ERR_UNSUPPORTED,
+                             "set %d: uniform weighting on the batched path needs N <= 4096, got "
+                             "%d (blink_gridding + blink_image take larger sides)", i,
+                             st.n_pixels);
     }
     return BLINK_OK;
 }

@@ -9,7 +9,8 @@
 // span Ns*R, does a radix-R DFT and writes
 // y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns].  Passes ping-pong between two LDS rows,
 // so no index reversal is needed and every butterfly needs only R registers.
-// Radices 2, 3, 4, 5, 8 and 16 cover N = 2^a 3^b 5^c.
+// Radices 2, 3, 4, 5, 8 and 16, and the primes 7, 11 and 13 as direct
+// sums, cover N = 2^a 3^b 5^c 7^d 11^e 13^f.
 //
 // Twiddles come from a table tw[m] = exp(+2 pi i m / N), m < N, built on the
 // host in double and rounded to FP32 (accurate to 1/2 ulp); the forward
@@ -151,6 +152,28 @@ __device__ __forceinline__ void dft16(float2* x) {
     }
 }
 
+// Radix-R DFT for a prime R (7, 11, 13) as a direct sum,
+// y[k] = sum_r x[r] w^(r k) with w = exp(S 2 pi i / R) = tw[n / R] (the
+// conjugate for S < 0): R^2 complex products, for the rare sides that need it.
+template <int S, int R>
+__device__ __forceinline__ void dft_prime(float2* x, const float2* __restrict__ tw, int n) {
+    const int step = n / R;
+    float2 y[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        float2 acc = x[0];
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            float2 w = tw[((r * k) % R) * step];
+            if (S < 0) w.y = -w.y;
+            acc = cadd(acc, cmul(x[r], w));
+        }
+        y[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) x[k] = y[k];
+}
+
 template <int S>
 __device__ __forceinline__ void dft_any(float2* x, int R) {
     switch (R) {
@@ -188,6 +211,7 @@ __device__ __forceinline__ void stockham_pass(const float2* __restrict__ src, fl
         else if constexpr (R == 4) dft4<S>(v);
         else if constexpr (R == 5) dft5<S>(v);
         else if constexpr (R == 8) dft8<S>(v);
+        else if constexpr (R == 7 || R == 11 || R == 13) dft_prime<S, R>(v, tw, n);
         else dft16<S>(v);
         const int base = (j - k) * R + k;
 #pragma unroll
@@ -211,7 +235,10 @@ __device__ float2* fft_row(float2* buf, float2* tmp, const FftPlan& plan,
             case 3: stockham_pass<S, 3>(src, dst, plan.n, ns, tw, lane, team); break;
             case 4: stockham_pass<S, 4>(src, dst, plan.n, ns, tw, lane, team); break;
             case 5: stockham_pass<S, 5>(src, dst, plan.n, ns, tw, lane, team); break;
+            case 7: stockham_pass<S, 7>(src, dst, plan.n, ns, tw, lane, team); break;
             case 8: stockham_pass<S, 8>(src, dst, plan.n, ns, tw, lane, team); break;
+            case 11: stockham_pass<S, 11>(src, dst, plan.n, ns, tw, lane, team); break;
+            case 13: stockham_pass<S, 13>(src, dst, plan.n, ns, tw, lane, team); break;
             default: stockham_pass<S, 16>(src, dst, plan.n, ns, tw, lane, team); break;
         }
         team_sync<kBlockTeam>();

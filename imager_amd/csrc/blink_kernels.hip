@@ -1324,6 +1324,9 @@ bool make_fft_plan(int n, FftPlan& plan) {
         else if (rem % 2 == 0) r = 2;
         else if (rem % 3 == 0) r = 3;
         else if (rem % 5 == 0) r = 5;
+        else if (rem % 7 == 0) r = 7;
+        else if (rem % 11 == 0) r = 11;
+        else if (rem % 13 == 0) r = 13;
         else return false;
         if (plan.n_passes >= kMaxFftPasses) return false;
         plan.radix[plan.n_passes] = r;

@@ -11,7 +11,7 @@
 
 namespace blink {
 
-constexpr int kMaxPixels = 4096;      // largest grid side (plan packs u in 13 bits)
+constexpr int kMaxPixels = 8192;      // largest grid side (plan entries pack positions in 13 bits)
 constexpr int kMaxBaselines = 1 << 18;  // plan packs the baseline in 18 bits
 constexpr int kPlanThreads = 1024;
 constexpr int kTileThreads = 256;

@@ -93,8 +93,10 @@ enum blink_grid_algo {
     BLINK_GRID_ATOMIC = 1  /* one thread per visibility, global atomics (plain path) */
 };
 
-/* Largest grid side supported by the in-house FFT and the gridding plan. */
-#define BLINK_MAX_PIXELS 4096
+/* Largest grid side supported by the in-house FFT and the gridding plan
+ * (sides with prime factors 2, 3, 5, 7, 11 and 13; above 4096 the batched
+ * path grids with natural weighting only). */
+#define BLINK_MAX_PIXELS 8192
 
 /* ---------------------------------------------------------------- context */
 
@@ -197,7 +199,7 @@ BLINK_API int blink_uv_cells(blink_ctx* ctx, const float* u_m, const float* v_m,
 
 /* Batched, unnormalised 2-D complex FFT of `batch` N x N arrays, in-house
  * LDS-staged Stockham passes (no hipFFT/rocFFT).  in == out is allowed.
- * N = 2^a 3^b 5^c, 2 <= N <= BLINK_MAX_PIXELS.  Replaces gpufftPlanMany +
+ * N = 2^a 3^b 5^c 7^d 11^e 13^f, 2 <= N <= BLINK_MAX_PIXELS.  Replaces gpufftPlanMany +
  * gpufftExecC2C (src/gpu/pacer_imager_hip.cpp:83-99). */
 BLINK_API int blink_fft2d_c2c(blink_ctx* ctx, const float* in, float* out, int n_pixels,
                               int batch, int direction, void* stream);

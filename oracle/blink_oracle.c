@@ -453,8 +453,8 @@ int64_t oracle_gridding_kb_f64(const float* vis, int n_ant, int n_ch, int n_t,
 
 static int smallest_factor(int n) {
     if (n % 4 == 0) return 4;
-    static const int primes[] = {2, 3, 5, 7};
-    for (int i = 0; i < 4; i++)
+    static const int primes[] = {2, 3, 5, 7, 11, 13};
+    for (int i = 0; i < 6; i++)
         if (n % primes[i] == 0) return primes[i];
     return 0;
 }
@@ -492,7 +492,7 @@ static int fft_supported(int n) {
         int m = n / p;                                                                         \
         for (int r = 0; r < p; r++) fft1d_##SUFFIX(in + 2 * r * is, is * p, out + 2 * r * m, m, tw, N); \
         const int step = N / n; /* w_n^j = tw[j*step] */                                       \
-        REAL yr[8], yi[8];                                                                     \
+        REAL yr[16], yi[16];                                                                     \
         for (int k = 0; k < m; k++) {                                                          \
             for (int r = 0; r < p; r++) {                                                      \
                 REAL a = out[2 * (r * m + k)], b = out[2 * (r * m + k) + 1];                   \

@@ -42,9 +42,9 @@ def test_library_targets_gfx950():
 def test_version_and_fft_sizes():
     from imager_amd import capi
     assert "gfx950" in capi.version()
-    for n in (2, 8, 15, 128, 180, 256, 512, 1024, 2048, 4096):
+    for n in (2, 7, 8, 11, 13, 15, 128, 154, 180, 208, 256, 512, 1024, 2048, 4096, 6000, 8192):
         assert capi.fft_supported(n), n
-    for n in (0, 1, 7, 11, 4097, 8192):
+    for n in (0, 1, 17, 34, 4097, 8194, 16384):
         assert not capi.fft_supported(n), n
 
 

@@ -173,9 +173,9 @@ def test_gridding_skips_nan_and_maxvis(ctx):
 
 # ------------------------------------------------------------------ FFT
 
-@pytest.mark.parametrize("n,batch", [(2, 3), (8, 2), (15, 2), (45, 1), (64, 3), (128, 2),
-                                     (180, 3), (256, 2), (512, 1), (1024, 2), (2048, 1),
-                                     (4096, 1)])
+@pytest.mark.parametrize("n,batch", [(2, 3), (7, 2), (8, 2), (13, 2), (15, 2), (45, 1), (64, 3),
+                                     (77, 2), (128, 2), (143, 1), (180, 3), (256, 2), (512, 1),
+                                     (1024, 2), (2002, 1), (2048, 1), (4096, 1), (8192, 1)])
 @pytest.mark.parametrize("direction", [1, -1])
 def test_fft2d_matches_numpy(ctx, n, batch, direction):
     import torch
@@ -679,9 +679,9 @@ def test_image_batch_grouped_sets(ctx):
 def test_error_paths(ctx):
     import torch
     from imager_amd import capi
-    x = torch.zeros((1, 11, 11), dtype=torch.complex64, device="cuda")
+    x = torch.zeros((1, 17, 17), dtype=torch.complex64, device="cuda")
     with pytest.raises(capi.BlinkUnsupported):
-        ctx.fft2d_c2c(x, x, 11, 1)
+        ctx.fft2d_c2c(x, x, 17, 1)
     g = torch.zeros((1, 1, 8, 8), dtype=torch.complex64, device="cuda")
     c = torch.zeros((1, 1, 8, 8), dtype=torch.float32, device="cuda")
     o = synth.small(n_ant=4, n_ch=1, n_t=1, n_pixels=8)

@@ -1,0 +1,116 @@
+"""Grid sides beyond the powers of two and 2^a 3^b 5^c up to 4096: sides with
+the prime factors 7, 11 and 13 (generic FFT passes of those radices), and
+sides above 4096 up to 8192 (the reference's FFTW / hipFFT plans take any N,
+src/pacer_imager.cpp:150-153, src/gpu/pacer_imager_hip.cpp:88; its
+experiments image 8192^2, experiments/compare_blink_images.sh:8).
+
+Each image of the batched call and of the stage entry points (blink_gridding
++ blink_image) against the oracle's reference-precision pipeline (peak-
+normalised error below 1e-5) and every count exactly; the refusals that stay.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from imager_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+IMAGE_TOL = 1e-5
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _obs(n, n_ch=2, n_t=1):
+    if n >= 4096:
+        # the 128-tile array (1.5 km) at the large sides
+        return synth.mwa_array(n_ch=n_ch, n_t=n_t, n_pixels=n, seed=n, height_sigma_m=1.0)
+    return synth.small(n_ant=48, n_ch=n_ch, n_t=n_t, n_pixels=n, seed=n, height_sigma_m=1.0)
+
+
+def _reference(o):
+    ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, o.n_pixels)
+    vc = oracle.cable_corrections(oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq),
+                                  o.n_ant, o.cable, o.freq)
+    _, c, _ = oracle.gridding(vc, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v,
+                              o.min_uv, o.n_pixels, "XX")
+    return ref, c
+
+
+# 154 = 2 7 11, 208 = 2^4 13, 2002 = 2 7 11 13, 6000 = 2^4 3 5^3, 8192 = 2^13
+@pytest.mark.parametrize("n,n_ch", [(154, 2), (208, 2), (2002, 2), (6000, 1), (8192, 1)])
+def test_batch_other_sides(ctx, n, n_ch):
+    import torch
+    from imager_amd import capi
+    o = _obs(n, n_ch=n_ch)
+    ref, c = _reference(o)
+    d = {k: _dev(getattr(o, k)) for k in ("vis", "u", "v", "w", "flags", "freq", "cable")}
+    imgs = torch.full((o.n_t, o.n_ch, n, n), float("nan"), dtype=torch.complex64, device="cuda")
+    sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
+    st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                       d["freq"], d["cable"], capi.CORR_GEOMETRIC | capi.CORR_CABLE, o.delta_u,
+                       o.delta_v, o.min_uv, n, "XX", imgs, sums)
+    for rep in range(2):   # capture, then replay
+        _say(f"batch N={n}, {o.n_images} images, rep {rep}")
+        ctx.image_batch([st])
+        err = oracle.peak_normalised_error(_host(imgs), ref)[0]
+        _say(f"  vs FP32 oracle {err:.2e}")
+        assert err < IMAGE_TOL
+        assert np.array_equal(_host(sums), c.reshape(o.n_images, -1).sum(1).astype(np.float64))
+
+
+@pytest.mark.parametrize("n", [2002, 8192])
+def test_stage_path_other_sides(ctx, n):
+    """blink_apply_*_corrections + blink_gridding (tiled plan) + blink_image at
+    a side with the factors 7, 11, 13 and at 8192."""
+    import torch
+    o = _obs(n, n_ch=1)
+    ref, c = _reference(o)
+    d = {k: _dev(getattr(o, k)) for k in ("vis", "u", "v", "w", "flags", "freq", "cable")}
+    grids = torch.zeros((1, 1, n, n), dtype=torch.complex64, device="cuda")
+    counts = torch.zeros((1, 1, n, n), dtype=torch.float32, device="cuda")
+    images = torch.empty((1, 1, n, n), dtype=torch.complex64, device="cuda")
+    _say(f"stage path N={n}")
+    ctx.apply_geometric_corrections(d["vis"], o.n_ant, 1, 1, d["w"], d["freq"])
+    ctx.apply_cable_corrections(d["vis"], o.n_ant, 1, 1, d["cable"], d["freq"])
+    ctx.gridding(d["vis"], o.n_ant, 1, 1, d["u"], d["v"], d["flags"], d["freq"], o.delta_u,
+                 o.delta_v, o.min_uv, n, "XX", 0, grids, counts)
+    assert np.array_equal(_host(counts), c.reshape(1, 1, n, n))
+    ctx.image(grids, counts, n, 1, images)
+    err = oracle.peak_normalised_error(_host(images), ref)[0]
+    _say(f"  vs FP32 oracle {err:.2e}")
+    assert err < IMAGE_TOL
+
+
+def test_side_refusals(ctx):
+    """Still refused: a prime factor above 13 (17), sides above 8192, and
+    uniform weighting on the batched path above 4096."""
+    import torch
+    from imager_amd import capi
+    x = torch.zeros((1, 34, 34), dtype=torch.complex64, device="cuda")
+    with pytest.raises(capi.BlinkUnsupported):
+        ctx.fft2d_c2c(x, x, 34, 1)
+    assert not capi.fft_supported(8194) and not capi.fft_supported(16384)
+    o = synth.small(n_ant=8, n_ch=1, n_t=1, n_pixels=64)
+    d = {k: _dev(getattr(o, k)) for k in ("vis", "u", "v", "w", "flags", "freq", "cable")}
+    img = torch.empty((1, 1, 8, 8), dtype=torch.complex64, device="cuda")
+    for n, weighting in ((34, "N"), (8192, "U")):
+        st = capi.make_set(d["vis"], 8, 1, 1, d["u"], d["v"], d["w"], d["flags"], d["freq"],
+                           d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, n, "XX", img,
+                           weighting=weighting)
+        with pytest.raises(capi.BlinkUnsupported):
+            ctx.image_batch([st])

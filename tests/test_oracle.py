@@ -63,7 +63,8 @@ def _naive_dft2(x, sign):
     return np.einsum("yv,...vu,xu->...yx", w, xl, w)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 20, 25, 30, 45])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 20, 22, 25, 26,
+                               30, 39, 45, 49, 77])
 @pytest.mark.parametrize("sign", [+1, -1])
 def test_fft_matches_textbook_dft(n, sign):
     rng = np.random.default_rng(100 + n)
@@ -75,7 +76,7 @@ def test_fft_matches_textbook_dft(n, sign):
     assert np.max(np.abs(got32 - ref)) <= 2e-6 * max(1.0, np.max(np.abs(ref)))
 
 
-@pytest.mark.parametrize("n", [64, 128, 180, 256])
+@pytest.mark.parametrize("n", [64, 128, 154, 180, 208, 256])
 def test_fft_matches_numpy(n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
@@ -100,7 +101,7 @@ def test_fft_parseval_and_roundtrip():
 
 def test_fft_unsupported_size_raises():
     with pytest.raises(ValueError):
-        oracle.fft2d(np.zeros((11, 11), np.complex128))
+        oracle.fft2d(np.zeros((17, 17), np.complex128))
 
 
 # ------------------------------------------------------------- cell indices
