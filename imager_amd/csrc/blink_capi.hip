@@ -89,8 +89,12 @@ struct blink_ctx {
     hipEvent_t noalloc_copied = nullptr;
     bool use_graphs = true;
     bool autotune = true;
-    // tile choice of the batched kernels per grid side: {row tile, column block}
-    std::map<int, std::pair<int, int>> tiles;
+    // tile choice of the batched kernels, {row tile, column block}: pinned
+    // per grid side (blink_set_batch_tiles), or tuned per (grid side, batch
+    // class: images per launch, in factors of 4), and the last tuned per side
+    std::map<int, std::pair<int, int>> pinned_tiles;
+    std::map<std::pair<int, int>, std::pair<int, int>> tuned_tiles;
+    std::map<int, std::pair<int, int>> last_tuned;
 
     // measurement hooks: event pairs around every launch while enabled
     struct ProfRec {
@@ -1266,6 +1270,29 @@ static std::pair<int, int> default_tiles(int n) {
     }
 }
 
+// Batch class of a group: its images per launch in factors of 4 (1, 2-3,
+// 4-15, 16-63, ... images), so that a side re-tunes when the launches it
+// makes change size materially.
+static int batch_class(const BatchGroup& g) {
+    const int64_t cnt = std::max<int64_t>(1, std::min<int64_t>(g.chunk, g.n_img));
+    int c = 0;  // floor(log4(cnt))
+    for (int64_t v = cnt; v >= 4; v >>= 2) c++;
+    return cnt == 1 ? 0 : 1 + c;
+}
+
+// Tiles for a register-side group: pinned, tuned for its batch class, or the
+// defaults.
+static bool has_tiles(const blink_ctx* ctx, const BatchGroup& g) {
+    return ctx->pinned_tiles.count(g.n) || ctx->tuned_tiles.count({g.n, batch_class(g)});
+}
+static std::pair<int, int> tiles_for(const blink_ctx* ctx, const BatchGroup& g) {
+    auto p = ctx->pinned_tiles.find(g.n);
+    if (p != ctx->pinned_tiles.end()) return p->second;
+    auto t = ctx->tuned_tiles.find({g.n, batch_class(g)});
+    if (t != ctx->tuned_tiles.end()) return t->second;
+    return default_tiles(g.n);
+}
+
 // Average GPU milliseconds of `reps` launches made by fn() on stream s.
 }  // extern "C"
 template <class F>
@@ -1337,7 +1364,8 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
             if (rows[r] == dflt.first && cols[c] == dflt.second) t_default = tp[r][c];
         }
     if (t_default <= 1.03f * best) best_r = dflt.first, best_c = dflt.second;
-    ctx->tiles[n] = {best_r, best_c};
+    ctx->tuned_tiles[{n, batch_class(g)}] = {best_r, best_c};
+    ctx->last_tuned[n] = {best_r, best_c};
     return BLINK_OK;
 }
 
@@ -1455,8 +1483,7 @@ static int enqueue_chunks(blink_ctx* ctx, const blink_image_set* sets, const Bat
     const int ns = (int)g.sets.size();
     FftPlan plan;
     make_fft_plan(n, plan);
-    auto it = ctx->tiles.find(n);
-    const std::pair<int, int> tl = it != ctx->tiles.end() ? it->second : default_tiles(n);
+    const std::pair<int, int> tl = tiles_for(ctx, g);
     // the second pass is skipped when no set of the group forms its images
     bool any_images = false;
     for (int i : g.sets) any_images |= sets[i].images != nullptr;
@@ -1570,12 +1597,12 @@ static int autotune_batch(blink_ctx* ctx, const BatchPlan& bp, const BatchTables
                           hipStream_t s) {
     if (!ctx->autotune) return BLINK_OK;
     bool need = false;
-    for (const BatchGroup& g : bp.groups) need |= g.reg && !ctx->tiles.count(g.n);
+    for (const BatchGroup& g : bp.groups) need |= g.reg && !has_tiles(ctx, g);
     if (!need) return BLINK_OK;
     int rc = enqueue_plans(ctx, bp, t, s);
     if (rc) return rc;
     for (const BatchGroup& g : bp.groups) {
-        if (!g.reg || ctx->tiles.count(g.n)) continue;
+        if (!g.reg || has_tiles(ctx, g)) continue;
         // the timed row passes gather what the call will gather
         if (g.compact_max_bl)
             HIP_TRY(g.sets.size() == 1
@@ -1892,7 +1919,7 @@ BLINK_API int blink_set_batch_tiles(blink_ctx* ctx, int n_pixels, int row_tile, 
     CHECK_ARG(batch_reg_supported(n_pixels), "side %d uses the generic kernels (no tiles)",
               n_pixels);
     if (row_tile == 0 && col_tile == 0) {
-        ctx->tiles.erase(n_pixels);
+        ctx->pinned_tiles.erase(n_pixels);
     } else {
         int t[8];
         const int nr = batch_row_variants(n_pixels, t, 8);
@@ -1901,7 +1928,7 @@ BLINK_API int blink_set_batch_tiles(blink_ctx* ctx, int n_pixels, int row_tile, 
         const int nc = batch_col_variants(n_pixels, t, 8);
         CHECK_ARG(std::find(t, t + nc, col_tile) != t + nc, "no column block %d built for side %d",
                   col_tile, n_pixels);
-        ctx->tiles[n_pixels] = {row_tile, col_tile};
+        ctx->pinned_tiles[n_pixels] = {row_tile, col_tile};
     }
     drop_graphs(ctx);  // captured graphs hold the previous tiles
     return BLINK_OK;
@@ -1910,10 +1937,12 @@ BLINK_API int blink_set_batch_tiles(blink_ctx* ctx, int n_pixels, int row_tile, 
 BLINK_API int blink_batch_tiles(blink_ctx* ctx, int n_pixels, int* row_tile, int* col_tile) {
     if (!ctx) return BLINK_ERR_INVALID_ARGUMENT;
     CHECK_ARG(row_tile && col_tile, "null pointer argument");
-    auto it = ctx->tiles.find(n_pixels);
-    if (it != ctx->tiles.end()) {
-        *row_tile = it->second.first;
-        *col_tile = it->second.second;
+    auto pin = ctx->pinned_tiles.find(n_pixels);
+    auto tun = ctx->last_tuned.find(n_pixels);
+    if (pin != ctx->pinned_tiles.end() || tun != ctx->last_tuned.end()) {
+        const std::pair<int, int> t = pin != ctx->pinned_tiles.end() ? pin->second : tun->second;
+        *row_tile = t.first;
+        *col_tile = t.second;
     } else if (batch_reg_supported(n_pixels)) {
         const std::pair<int, int> t = default_tiles(n_pixels);
         *row_tile = -t.first;

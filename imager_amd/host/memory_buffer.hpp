@@ -5,12 +5,22 @@
 // tests/gridding_test.cpp:22-30): allocate / data / size / operator[] /
 // operator bool / to_gpu / to_cpu / on_gpu / from_dump / dump.  A build uses
 // either these headers or astroio's, never both.
+//
+// Device memory is stream-ordered on the null stream (hipMallocAsync /
+// hipFreeAsync from the device's default pool, which keeps freed memory for
+// reuse): freeing a buffer that queued work still uses does not wait for the
+// device, and a new buffer of the same size comes from the pool.  Work that
+// uses a buffer on another stream must be synchronised with the null stream
+// before the buffer is freed.  Devices without memory pools use hipMalloc /
+// hipFree.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 
 #include <cstddef>
+#include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -21,6 +31,30 @@ enum class MemoryType { HOST, DEVICE };
 inline void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess)
         throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Whether the current device allocates from its default memory pool; the
+// first call per device makes the pool keep freed memory (release threshold
+// at the maximum) instead of returning it at every synchronisation.
+inline bool device_pool_ready() {
+    static std::mutex m;
+    static int state[64] = {};  // 0 unknown, 1 pool, 2 no pool
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
+    std::lock_guard<std::mutex> lock(m);
+    if (!state[dev]) {
+        int supported = 0;
+        hipMemPool_t pool = nullptr;
+        uint64_t keep = UINT64_MAX;
+        const bool ok =
+            hipDeviceGetAttribute(&supported, hipDeviceAttributeMemoryPoolsSupported, dev) ==
+                hipSuccess &&
+            supported && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess &&
+            hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess;
+        if (!ok) (void)hipGetLastError();
+        state[dev] = ok ? 1 : 2;
+    }
+    return state[dev] == 1;
 }
 
 template <typename T>
@@ -37,6 +71,7 @@ class MemoryBuffer {
             ptr_ = o.ptr_;
             n_ = o.n_;
             on_gpu_ = o.on_gpu_;
+            pooled_ = o.pooled_;
             o.ptr_ = nullptr;
             o.n_ = 0;
             o.on_gpu_ = false;
@@ -51,7 +86,7 @@ class MemoryBuffer {
         on_gpu_ = type == MemoryType::DEVICE;
         if (n == 0) return;
         if (on_gpu_) {
-            hip_check(hipMalloc(reinterpret_cast<void**>(&ptr_), sizeof(T) * n), "hipMalloc");
+            ptr_ = device_alloc(n);
         } else {
             ptr_ = new T[n]();
         }
@@ -68,8 +103,7 @@ class MemoryBuffer {
     // Moves the contents to device memory (no-op when already there).
     void to_gpu() {
         if (on_gpu_ || !ptr_) return;
-        T* d = nullptr;
-        hip_check(hipMalloc(reinterpret_cast<void**>(&d), sizeof(T) * n_), "hipMalloc");
+        T* d = device_alloc(n_);
         hip_check(hipMemcpy(d, ptr_, sizeof(T) * n_, hipMemcpyHostToDevice), "hipMemcpy H2D");
         delete[] ptr_;
         ptr_ = d;
@@ -82,7 +116,7 @@ class MemoryBuffer {
         T* h = new T[n_];
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         hip_check(hipMemcpy(h, ptr_, sizeof(T) * n_, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-        hip_check(hipFree(ptr_), "hipFree");
+        device_free(ptr_);
         ptr_ = h;
         on_gpu_ = false;
     }
@@ -105,9 +139,23 @@ class MemoryBuffer {
     }
 
    private:
+    T* device_alloc(size_t n) {
+        T* d = nullptr;
+        pooled_ = device_pool_ready();
+        if (pooled_)
+            hip_check(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(T) * n, nullptr),
+                      "hipMallocAsync");
+        else
+            hip_check(hipMalloc(reinterpret_cast<void**>(&d), sizeof(T) * n), "hipMalloc");
+        return d;
+    }
+    void device_free(T* d) {
+        if (pooled_) (void)hipFreeAsync(d, nullptr);
+        else (void)hipFree(d);
+    }
     void release() {
         if (!ptr_) return;
-        if (on_gpu_) (void)hipFree(ptr_);
+        if (on_gpu_) device_free(ptr_);
         else delete[] ptr_;
         ptr_ = nullptr;
         n_ = 0;
@@ -115,4 +163,5 @@ class MemoryBuffer {
     T* ptr_ = nullptr;
     size_t n_ = 0;
     bool on_gpu_ = false;
+    bool pooled_ = false;  // device memory from the stream-ordered pool
 };

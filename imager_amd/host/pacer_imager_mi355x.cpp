@@ -65,15 +65,27 @@ void CPacerImagerMI355X::check(int rc) const {
     throw std::runtime_error(msg);
 }
 
+// Copies n host values to the device buffer unless they equal the values the
+// last copy uploaded (kept in `shadow`): an unchanged input is not copied
+// again, so a call does not wait (hipMemcpy) for the work still reading it.
+template <typename T>
+static void upload_changed(MemoryBuffer<T>& dev, std::vector<T>& shadow, const T* host, size_t n,
+                           const char* what) {
+    if (dev.size() == n && shadow.size() == n &&
+        (n == 0 || std::memcmp(shadow.data(), host, sizeof(T) * n) == 0))
+        return;
+    if (dev.size() != n) dev.allocate(n, MemoryType::DEVICE);
+    if (n) hip_check(hipMemcpy(dev.data(), host, sizeof(T) * n, hipMemcpyHostToDevice), what);
+    shadow.assign(host, host + n);
+}
+
 void CPacerImagerMI355X::UpdateAntennaFlags(int n_ant) {
     const size_t n_bl = (size_t)n_ant * (n_ant + 1) / 2;
     if (m_FlaggedBaselines.size() != n_bl) UpdateFlags();
     if (m_FlaggedBaselines.size() != n_bl)
         throw std::invalid_argument("the array has " + std::to_string(m_FlaggedBaselines.size()) +
                                     " baselines but the visibilities have " + std::to_string(n_bl));
-    if (baseline_flags_gpu.size() != n_bl) baseline_flags_gpu.allocate(n_bl, MemoryType::DEVICE);
-    hip_check(hipMemcpy(baseline_flags_gpu.data(), m_FlaggedBaselines.data(), sizeof(int) * n_bl,
-                        hipMemcpyHostToDevice), "flags H2D");
+    upload_changed(baseline_flags_gpu, flags_up_, m_FlaggedBaselines.data(), n_bl, "flags H2D");
 }
 
 void CPacerImagerMI355X::upload_inputs(const Visibilities& xcorr) {
@@ -83,17 +95,11 @@ void CPacerImagerMI355X::upload_inputs(const Visibilities& xcorr) {
         throw std::invalid_argument("UVW are given for " + std::to_string(u_cpu.size()) +
                                     " baselines but the visibilities have " + std::to_string(n_bl));
     UpdateAntennaFlags(n_ant);
-    auto up = [&](MemoryBuffer<float>& d, MemoryBuffer<float>& h) {
-        if (d.size() != h.size()) d.allocate(h.size(), MemoryType::DEVICE);
-        hip_check(hipMemcpy(d.data(), h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice), "uvw H2D");
-    };
-    up(u_gpu, u_cpu);
-    up(v_gpu, v_cpu);
-    up(w_gpu, w_cpu);
-    if (frequencies_gpu.size() != frequencies.size())
-        frequencies_gpu.allocate(frequencies.size(), MemoryType::DEVICE);
-    hip_check(hipMemcpy(frequencies_gpu.data(), frequencies.data(), sizeof(double) * frequencies.size(),
-                        hipMemcpyHostToDevice), "frequencies H2D");
+    upload_changed(u_gpu, u_up_, u_cpu.data(), u_cpu.size(), "u H2D");
+    upload_changed(v_gpu, v_up_, v_cpu.data(), v_cpu.size(), "v H2D");
+    upload_changed(w_gpu, w_up_, w_cpu.data(), w_cpu.size(), "w H2D");
+    upload_changed(frequencies_gpu, freq_up_, frequencies.data(), frequencies.size(),
+                   "frequencies H2D");
 }
 
 // blink_grid_kernel of m_nConvolvingKernelSize / _gridMode, 0 for nearest-cell
@@ -166,11 +172,8 @@ void CPacerImagerMI355X::ApplyGeometricCorrections(Visibilities& xcorr, MemoryBu
     xcorr.to_gpu();
     const size_t n_bl = xcorr.n_baselines();
     if (w.size() != n_bl) throw std::invalid_argument("w has the wrong number of baselines");
-    if (w_gpu.size() != n_bl) w_gpu.allocate(n_bl, MemoryType::DEVICE);
-    hip_check(hipMemcpy(w_gpu.data(), w.data(), sizeof(float) * n_bl, hipMemcpyHostToDevice), "w H2D");
-    if (frequencies_gpu.size() != freqs.size()) frequencies_gpu.allocate(freqs.size(), MemoryType::DEVICE);
-    hip_check(hipMemcpy(frequencies_gpu.data(), freqs.data(), sizeof(double) * freqs.size(),
-                        hipMemcpyHostToDevice), "frequencies H2D");
+    upload_changed(w_gpu, w_up_, w.data(), n_bl, "w H2D");
+    upload_changed(frequencies_gpu, freq_up_, freqs.data(), freqs.size(), "frequencies H2D");
     check(blink_apply_geometric_corrections(ctx_, (float*)xcorr.data(), (int)xcorr.obsInfo.nAntennas,
                                             (int)xcorr.nFrequencies, (int)xcorr.integration_intervals(),
                                             w_gpu.data(), frequencies_gpu.data(), nullptr));
@@ -181,13 +184,8 @@ void CPacerImagerMI355X::ApplyCableCorrections(Visibilities& xcorr, MemoryBuffer
     xcorr.to_gpu();
     if (cables.size() != xcorr.obsInfo.nAntennas)
         throw std::invalid_argument("cable lengths have the wrong number of antennas");
-    if (cable_lengths_gpu.size() != cables.size())
-        cable_lengths_gpu.allocate(cables.size(), MemoryType::DEVICE);
-    hip_check(hipMemcpy(cable_lengths_gpu.data(), cables.data(), sizeof(double) * cables.size(),
-                        hipMemcpyHostToDevice), "cables H2D");
-    if (frequencies_gpu.size() != freqs.size()) frequencies_gpu.allocate(freqs.size(), MemoryType::DEVICE);
-    hip_check(hipMemcpy(frequencies_gpu.data(), freqs.data(), sizeof(double) * freqs.size(),
-                        hipMemcpyHostToDevice), "frequencies H2D");
+    upload_changed(cable_lengths_gpu, cable_up_, cables.data(), cables.size(), "cables H2D");
+    upload_changed(frequencies_gpu, freq_up_, freqs.data(), freqs.size(), "frequencies H2D");
     check(blink_apply_cable_corrections(ctx_, (float*)xcorr.data(), (int)xcorr.obsInfo.nAntennas,
                                         (int)xcorr.nFrequencies, (int)xcorr.integration_intervals(),
                                         cable_lengths_gpu.data(), frequencies_gpu.data(), nullptr));
@@ -202,9 +200,7 @@ Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
     fill_cable_lengths(n_ant);
     set_grid_scalars(xcorr);
     upload_inputs(xcorr);
-    if (cable_lengths_gpu.size() != n_ant) cable_lengths_gpu.allocate(n_ant, MemoryType::DEVICE);
-    hip_check(hipMemcpy(cable_lengths_gpu.data(), cable_lengths.data(), sizeof(double) * n_ant,
-                        hipMemcpyHostToDevice), "cables H2D");
+    upload_changed(cable_lengths_gpu, cable_up_, cable_lengths.data(), n_ant, "cables H2D");
     // the visibilities are read in place on the device; host visibilities are
     // copied (run() moves them to the device, as the reference's GPU path does)
     const float* vis = (const float*)xcorr.data();

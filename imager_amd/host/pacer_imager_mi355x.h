@@ -67,6 +67,10 @@ class CPacerImagerMI355X : public CPacerImager {
     MemoryBuffer<float> u_gpu, v_gpu, w_gpu;
     MemoryBuffer<int> baseline_flags_gpu;
     MemoryBuffer<double> frequencies_gpu, cable_lengths_gpu;
+    // the host values of the last upload of each (upload_changed)
+    std::vector<float> u_up_, v_up_, w_up_;
+    std::vector<int> flags_up_;
+    std::vector<double> freq_up_, cable_up_;
     blink_ctx* ctx_{nullptr};
     // run_batched(): a device copy of the visibilities when the caller's are
     // on the host (reused while the shape is unchanged)

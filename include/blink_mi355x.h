@@ -341,11 +341,12 @@ BLINK_API int blink_profile_reset(blink_ctx* ctx);
 /* Tile sizes of the batched kernels for grid side n_pixels: grid rows per
  * first-pass tile (grid columns u up to 2048^2, where the first pass runs
  * along v) and output lines per second-pass block (image rows up to 2048^2,
- * image columns at 4096^2).  Positive when chosen by the built-in
- * autotuner (the first blink_image_batch call on a new side times every
- * built variant unless BLINK_AUTOTUNE=0 is set in the environment),
- * negative for the defaults not yet tuned, 0 when the side uses the generic
- * kernels. */
+ * image columns at 4096^2).  Positive when pinned or chosen by the
+ * built-in autotuner (the first blink_image_batch call that launches a new
+ * side with a new batch class -- images per launch of 1, 2-3, 4-15, 16-63,
+ * ... -- times every built variant unless BLINK_AUTOTUNE=0 is set in the
+ * environment; the last tuned pair is reported), negative for the defaults
+ * not yet tuned, 0 when the side uses the generic kernels. */
 BLINK_API int blink_batch_tiles(blink_ctx* ctx, int n_pixels, int* row_tile, int* col_tile);
 /* Pins the tile sizes of the batched kernels for grid side n_pixels (row_tile
  * lines per first-pass tile, col_tile output lines per second-pass block, as
