@@ -26,10 +26,11 @@
  *     graphs.  Contexts are not thread-safe: use one per host thread.  Calls
  *     on one context share its scratch, so they must not run concurrently on
  *     different streams (order them, or use one context per stream).
- *     blink_image_batch also uses the context's own second stream, forked
- *     from and joined back to `stream` within the call: for the visibility
- *     compaction of sparse sets on sides up to 2048 (beside the gridding
- *     plan), and, in a call with several grid sides, for the smaller sides.
+ *     blink_image_batch also uses two streams of the context, forked from
+ *     and joined back to `stream` within the call: the second stream for the
+ *     visibility compaction of sparse sets on sides up to 2048 (beside the
+ *     gridding plan) and, in a call with several grid sides, for the smaller
+ *     sides; the pass stream for the second passes of overlapped chunks.
  */
 #ifndef BLINK_MI355X_H
 #define BLINK_MI355X_H
