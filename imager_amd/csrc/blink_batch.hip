@@ -547,7 +547,10 @@ k_grid_rows_wide(SetDesc first, const SetDesc* __restrict__ sets, const int* __r
     F::template run<1>(v, row, lane, TwSplit64{twl, twl + 64});
     if (team < nr && rs[team + 1] != rs[team]) {
         float2* dst = inter + ((int64_t)li * (H + 1) + r0 + team) * N;
-        F::store(v, lane, [&](int i, float2 x) { store_stream(&dst[i], x); });
+        if (sd.cached_inter)
+            F::store(v, lane, [&](int i, float2 x) { dst[i] = x; });
+        else
+            F::store(v, lane, [&](int i, float2 x) { store_stream(&dst[i], x); });
     }
 }
 
