@@ -892,13 +892,15 @@ static int batch_buffer(blink_ctx* ctx, Arena* arena, const char* name, size_t b
 // stores them with plain stores, the second pass reads them back from the
 // cache, and each chunk's second pass runs beside the next chunk's row pass.
 // kOverlapChunkBytes is the target chunk size (40 images at 1024^2, 3 at
-// 4096^2).  Up to 2048^2 only groups whose images share their plans (two or
+// 4096^2).  Below 2048^2 only groups whose images share their plans (two or
 // more timesteps per channel on average) are overlapped: where every image
 // has a plan of its own, the row pass gathers from cold plan entries and the
 // overlapped passes measured slower than one pass after the other (768
 // channels x 1 timestep at 1024^2: 336k against 356k images/s; 64 x 10 at
-// 1024^2: 416k against 376k).  At 4096^2 the rows are long enough to hide
-// that (768 channels x 1 timestep: 137.6 against 159.0 ms per 3072 images).
+// 1024^2: 416k against 376k).  From 2048^2 the rows are long enough to hide
+// that (768 channels x 1 timestep at 4096^2: 137.6 against 159.0 ms per 3072
+// images; single-image sets of six sides, main group 2048^2: 262k against
+// 250k images/s).
 static constexpr int64_t kOverlapChunkBytes = 168ll << 20;
 static constexpr int64_t kCachedChunkBytes = 256ll << 20;
 
@@ -1092,7 +1094,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             if (bp.overlap_chunks >= 0)
                 chunks = bp.overlap_chunks;
             else if (g.reg && bp.overlap_bytes > 0 && per * g.n_img > bp.overlap_bytes &&
-                     (g.n > 2048 || g.n_img >= 2 * group_channels(g)))
+                     (g.n >= 2048 || g.n_img >= 2 * group_channels(g)))
                 chunks = (int)((per * g.n_img + bp.overlap_bytes - 1) / bp.overlap_bytes);
         }
         const bool overlap = chunks >= 2;

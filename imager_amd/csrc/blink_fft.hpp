@@ -174,18 +174,6 @@ __device__ __forceinline__ void dft_prime(float2* x, const float2* __restrict__ 
     for (int k = 0; k < R; k++) x[k] = y[k];
 }
 
-template <int S>
-__device__ __forceinline__ void dft_any(float2* x, int R) {
-    switch (R) {
-        case 2: dft2<S>(x); break;
-        case 3: dft3<S>(x); break;
-        case 4: dft4<S>(x); break;
-        case 5: dft5<S>(x); break;
-        case 8: dft8<S>(x); break;
-        default: dft16<S>(x); break;
-    }
-}
-
 // One Stockham pass of compile-time radix R over a row: src -> dst.
 template <int S, int R>
 __device__ __forceinline__ void stockham_pass(const float2* __restrict__ src, float2* __restrict__ dst,
