@@ -1033,7 +1033,8 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     std::vector<size_t> av_off(n_sets, SIZE_MAX);
     size_t av_bytes = 0;
     for (int i = 0; i < n_sets; i++) {
-        if (!sets[i].average || is_dft(sets[i].transform)) continue;
+        // odd sides average their formed images
+        if (!sets[i].average || is_dft(sets[i].transform) || sets[i].n_pixels % 2) continue;
         const size_t n = (size_t)sets[i].n_pixels;
         av_off[i] = av_bytes;
         av_bytes += align256(sizeof(float2) * (n / 2 + 1) * n) + align256(4 * (n + 2));
@@ -1551,6 +1552,15 @@ static int enqueue_chunks(blink_ctx* ctx, const blink_image_set* sets, const Bat
         d.uniform = 0;
         LAUNCH("fft_cols", p, second_pass(d, nullptr, nullptr, 1, 0, 1, sc.avg_acc));
     }
+    // odd sides: the mean of the formed images (image_averaging_gpu's arithmetic)
+    if (n % 2)
+        for (int i : g.sets) {
+            const blink_image_set& st = sets[i];
+            if (st.average)
+                LAUNCH("average", p,
+                       launch_average(st.images, (int64_t)n * n, st.n_ch * st.n_t, 0, st.average,
+                                      p));
+        }
     return BLINK_OK;
 }
 
@@ -1712,7 +1722,16 @@ static int check_sets(blink_ctx* ctx, const blink_image_set* sets, int n_sets) {
                   "set %d: vis, images and average must be 16-byte aligned", i);
         CHECK_ARG(st.transform == BLINK_TRANSFORM_FFT || is_dft(st.transform),
                   "set %d: unknown transform %d", i, st.transform);
-        CHECK_ARG(st.n_pixels % 2 == 0, "set %d: the batched path needs an even grid side", i);
+        if (st.n_pixels % 2) {
+            // odd sides: the FFT transform with nearest-cell gridding; a mean
+            // is taken from the formed images
+            if (st.transform != BLINK_TRANSFORM_FFT || st.kernel_size > 0)
+                return ctx->fail(BLINK_ERR_UNSUPPORTED,
+                                 "set %d: an odd grid side (%d) takes the FFT transform with "
+                                 "nearest-cell gridding on the batched path", i, st.n_pixels);
+            CHECK_ARG(st.images || !st.average,
+                      "set %d: an odd grid side needs `images` (also with an average)", i);
+        }
         if (is_dft(st.transform)) {
             if (st.n_pixels < 2 || st.n_pixels > BLINK_MAX_DFT_PIXELS)
                 return ctx->fail(BLINK_ERR_UNSUPPORTED,

@@ -1184,8 +1184,10 @@ k_grid_rows_fft(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         atomicAdd(&tile[r * n + u_idx].x, v.x);
         atomicAdd(&tile[r * n + u_idx].y, v.y);
         if (uniform) atomicAdd(&cnt[r * n + u_idx], 1u);
-        const int row = r0 + r;  // an entry of rows 1 .. N/2-1 stands for its mirror too
-        return (row == 0 || row == h) ? 1 : 2;
+        // an entry stands for its mirror too, except in the self-conjugate rows
+        // (0, and N/2 for even N), which list both
+        const int row = r0 + r;
+        return (row == 0 || 2 * row == n) ? 1 : 2;
     });
     const int tot = block_sum_int(n_ok, red);
     if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
@@ -1619,6 +1621,91 @@ hipError_t launch_grid_rows_kb(const SetDesc& first, const SetDesc* sets, const 
     return hipGetLastError();
 }
 
+// Second pass of the generic batched path for odd N = 2m + 1, from the
+// row-major row-pass buffer inter[img][p][y], plan rows p = 0 .. m.  As for
+// even N the images are real, so one complex FFT over p of z = c_y + i
+// c_{y+1} gives image rows y and y + 1, with rows N - p (p = 1 .. m: an odd
+// side has no self-conjugate middle row) the conjugates of rows p; column
+// pairs are slots s = 0 .. m (y = 2s), and the last column N - 1 pairs with
+// zero columns.  A workgroup owns `pairs` slots; 8-byte accesses (odd rows
+// are not 16-byte aligned); the store applies the odd-side fftshift
+// (shift_index: p <-> p + m + 1 for p < m, the middle element fixed,
+// src/pacer_imager.cpp:30-35,110-128) element by element.
+template <int S, bool kBlockTeam, bool kMulti>
+__global__ void __launch_bounds__(kFftThreads)
+k_cols_half_odd(SetDesc first, const SetDesc* __restrict__ sets, const int* __restrict__ img_set,
+                FftPlan plan, int team, int pairs, int64_t gi0, const float2* __restrict__ inter,
+                const float2* __restrict__ tw) {
+    extern __shared__ float4 cho_lds4[];
+    const int n = plan.n, h = n / 2;
+    const int ld = n + 1;
+    float2* rows = (float2*)cho_lds4;  // [pairs][ld]
+    const int teams = kBlockTeam ? 1 : (int)blockDim.x / team;
+    const int t_id = kBlockTeam ? 0 : (int)threadIdx.x / team;
+    const int lane = kBlockTeam ? (int)threadIdx.x : (int)threadIdx.x % team;
+    float2* tmp = rows + (size_t)pairs * ld + (size_t)t_id * n;
+    unsigned char* nonempty = (unsigned char*)(rows + (size_t)pairs * ld + (size_t)teams * n);
+
+    const int n_slots = h + 1;
+    const int n_cb = (n_slots + pairs - 1) / pairs;
+    const int cbk = (int)(blockIdx.x % n_cb);
+    const int li = (int)(blockIdx.x / n_cb);
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    if (!sd.images) return;  // uniform per workgroup
+    const int img = (int)(gi - sd.img_base);
+    const uint32_t count = sd.isums[img];
+    if (cbk == 0 && threadIdx.x == 0 && sd.count_sums) sd.count_sums[img] = (double)count;
+    const int* rsg = sd.row_start + (int64_t)(img % sd.g.n_ch) * (n + 1);
+    const float empty = sd.uniform ? __builtin_nanf("") : 0.f;  // rows without entries
+    for (int i = threadIdx.x; i <= h; i += blockDim.x)
+        nonempty[i] = sd.all_rows || rsg[i + 1] != rsg[i];
+    __syncthreads();
+
+    const int s0 = cbk * pairs;
+    const int np = min(pairs, n_slots - s0);  // slots of this workgroup (uniform)
+    const float2* src = inter + (int64_t)li * (h + 1) * n;
+    const int stage = (h + 1) * np;
+    for (int i = threadIdx.x; i < stage; i += (int)blockDim.x) {
+        const int v = i / np, t = i % np;
+        const int y = 2 * (s0 + t);
+        const bool has_b = y + 1 < n;  // the last column pairs with zeros
+        float2 a = make_float2(empty, empty);
+        float2 b = has_b ? make_float2(empty, empty) : make_float2(0.f, 0.f);
+        if (nonempty[v]) {
+            a = src[(int64_t)v * n + y];
+            if (has_b) b = src[(int64_t)v * n + y + 1];
+        }
+        float2* zr = rows + (size_t)t * ld;
+        zr[v] = make_float2(a.x - b.y, a.y + b.x);                   // c_y[v] + i c_{y+1}[v]
+        if (v > 0) zr[n - v] = make_float2(a.x + b.y, b.x - a.y);    // row N - v
+    }
+    __syncthreads();
+    const int rounds = kBlockTeam ? np : (np + teams - 1) / teams;
+    for (int k = 0; k < rounds; k++) {
+        const int t = kBlockTeam ? k : k * teams + t_id;
+        if (!kBlockTeam && t >= np) break;
+        float2* zr = rows + (size_t)t * ld;
+        const float2* res = fft_row<S, kBlockTeam>(zr, tmp, plan, tw, lane, team);
+        if (res != zr) {
+            for (int i = lane; i < n; i += team) zr[i] = res[i];
+            team_sync<kBlockTeam>();
+        }
+    }
+    __syncthreads();
+    const float s = (float)(1.0 / (double)count);
+    float2* out = sd.images + (int64_t)img * n * n;
+    for (int i = threadIdx.x; i < np * n; i += (int)blockDim.x) {
+        const int t = i / n, x = i % n;
+        const int y = 2 * (s0 + t);
+        const float2 z = rows[(size_t)t * ld + x];
+        const int ox = shift_index(x, n);
+        store_stream(&out[(int64_t)shift_index(y, n) * n + ox], make_float2(z.x * s, 0.f));
+        if (y + 1 < n)
+            store_stream(&out[(int64_t)shift_index(y + 1, n) * n + ox], make_float2(z.y * s, 0.f));
+    }
+}
+
 // Buffer columns (image rows) per workgroup of k_cols_half: the largest even divisor of
 // n, at most 64, whose LDS rows fit next to the team scratch and that still
 // gives at least 256 workgroups over the n_img images (one per CU), else the
@@ -1641,10 +1728,32 @@ hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int
                             const float* inter, const float* tw, hipStream_t stream) {
     if (n_img == 0) return hipSuccess;
     const int n = plan.n;
-    if (n % 2) return hipErrorInvalidValue;
     const int team = fft_team_for(n);
     const bool block_team = team == kFftThreads;
     const int teams = block_team ? 1 : kFftThreads / team;
+    if (n % 2) {
+        // odd N: slots of column pairs per workgroup, as many as fit up to 32,
+        // and at least ~256 workgroups over the images
+        const int n_slots = n / 2 + 1;
+        auto lds_of = [&](int p) {
+            return sizeof(float2) * ((size_t)p * (n + 1) + (size_t)teams * n) + (size_t)(n / 2 + 1);
+        };
+        int pairs = std::min(32, n_slots);
+        while (pairs > 1 && (!lds_fits(lds_of(pairs)) ||
+                             (int64_t)((n_slots + pairs - 1) / pairs) * n_img < 256))
+            pairs--;
+        if (!lds_fits(lds_of(pairs))) return hipErrorInvalidConfiguration;
+        const int64_t blocks = (int64_t)((n_slots + pairs - 1) / pairs) * n_img;
+        if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+        const bool multi = n_sets > 1;
+        auto k = block_team
+                     ? (multi ? k_cols_half_odd<1, true, true> : k_cols_half_odd<1, true, false>)
+                     : (multi ? k_cols_half_odd<1, false, true> : k_cols_half_odd<1, false, false>);
+        hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(kFftThreads), lds_of(pairs), stream,
+                           first, sets, img_set, plan, team, pairs, gi0, (const float2*)inter,
+                           (const float2*)tw);
+        return hipGetLastError();
+    }
     const int cb = half_cols_block(n, teams, n_img);
     if (!cb) return hipErrorInvalidConfiguration;
     const size_t lds = sizeof(float2) * ((size_t)(cb / 2) * (n + 1) + (size_t)teams * n) +
@@ -1701,6 +1810,10 @@ hipError_t set_kernel_attributes() {
     set((const void*)k_cols_half<1, false, false>);
     set((const void*)k_cols_half<1, true, true>);
     set((const void*)k_cols_half<1, false, true>);
+    set((const void*)k_cols_half_odd<1, true, false>);
+    set((const void*)k_cols_half_odd<1, false, false>);
+    set((const void*)k_cols_half_odd<1, true, true>);
+    set((const void*)k_cols_half_odd<1, false, true>);
     set((const void*)k_grid_tiles);
     set((const void*)k_grid_half_rows);
     set((const void*)k_plan_rows);

@@ -212,9 +212,10 @@ Images CPacerImagerMI355X::run_batched(Visibilities& xcorr) {
     }
     const size_t n_img = n_gridded_channels * n_gridded_intervals;
     const size_t image_size = (size_t)n_pixels * n_pixels;
-    // with average_images only the mean is formed, unless a DFT transform
-    // needs the images
-    const bool keep_images = !average_images || m_nBatchTransform != BLINK_TRANSFORM_FFT;
+    // with average_images only the mean is formed, unless a DFT transform or
+    // an odd grid side needs the images
+    const bool keep_images =
+        !average_images || m_nBatchTransform != BLINK_TRANSFORM_FFT || n_pixels % 2 != 0;
     MemoryBuffer<std::complex<float>> images_buffer(keep_images ? image_size * n_img : 0,
                                                     MemoryType::DEVICE);
     MemoryBuffer<std::complex<float>> avg(average_images ? image_size : 0, MemoryType::DEVICE);

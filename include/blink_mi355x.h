@@ -250,7 +250,8 @@ typedef struct blink_image_set {
     const double* cable_m;   /* [n_ant], read when BLINK_CORR_CABLE */
     unsigned corrections;    /* BLINK_CORR_* bits, applied on the fly (vis is not modified) */
     double delta_u, delta_v, min_uv;
-    int n_pixels;
+    int n_pixels;            /* grid side N; odd N with the FFT transform and nearest-cell
+                                gridding only (`images` then required with an average) */
     int pol;                 /* blink_pol */
     float* images;           /* [n_t][n_ch][N][N] complex64 output, 16-byte aligned.  The
                                 gridded plane is Hermitian, so the images are real and every

@@ -692,9 +692,13 @@ def test_error_paths(ctx):
     with pytest.raises(capi.BlinkInvalidArgument):
         ctx.image(g, c, 8, 1, g)   # images aliasing grids
     o2 = synth.small(n_ant=8, n_ch=1, n_t=1, n_pixels=45)
-    _, _, _, st = _batch(ctx, o2)
+    d2 = _obs_dev(o2)
+    avg = torch.zeros((45, 45), dtype=torch.complex64, device="cuda")
+    st = capi.make_set(d2["vis"], 8, 1, 1, d2["u"], d2["v"], d2["w"], d2["flags"], d2["freq"],
+                       d2["cable"], 3, o2.delta_u, o2.delta_v, o2.min_uv, 45, "XX", None,
+                       average=avg)
     with pytest.raises(capi.BlinkInvalidArgument):
-        ctx.image_batch([st])      # odd grid side in the batched path
+        ctx.image_batch([st])      # an odd side's mean needs the images
 
 
 # ------------------------------------------ split-precision DFT on MFMA

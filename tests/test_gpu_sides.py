@@ -114,3 +114,68 @@ def test_side_refusals(ctx):
                            weighting=weighting)
         with pytest.raises(capi.BlinkUnsupported):
             ctx.image_batch([st])
+
+
+# odd sides on the batched path: 45 = 3^2 5, 77 = 7 11, 225 = 3^2 5^2, 1001 = 7 11 13
+@pytest.mark.parametrize("n,pol", [(45, "XX"), (77, "I"), (225, "XX"), (1001, "YY")])
+def test_batch_odd_sides(ctx, n, pol):
+    """Odd N (the reference's CPU path images them, with its swap fftshift for
+    odd sizes, src/pacer_imager.cpp:30-35,110-128): images, count sums and the
+    mean (taken from the formed images) against the oracle, on capture and
+    replay."""
+    import torch
+    from imager_amd import capi
+    o = synth.small(n_ant=40, n_ch=2, n_t=2, n_pixels=n, seed=7 * n, height_sigma_m=1.0)
+    ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, n, pol)
+    vc = oracle.cable_corrections(oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq),
+                                  o.n_ant, o.cable, o.freq)
+    _, c, _ = oracle.gridding(vc, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v,
+                              o.min_uv, n, pol)
+    d = {k: _dev(getattr(o, k)) for k in ("vis", "u", "v", "w", "flags", "freq", "cable")}
+    imgs = torch.full((o.n_t, o.n_ch, n, n), float("nan"), dtype=torch.complex64, device="cuda")
+    sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
+    avg = torch.full((n, n), float("nan"), dtype=torch.complex64, device="cuda")
+    st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                       d["freq"], d["cable"], capi.CORR_GEOMETRIC | capi.CORR_CABLE, o.delta_u,
+                       o.delta_v, o.min_uv, n, pol, imgs, sums, average=avg)
+    flat = np.ascontiguousarray(ref.reshape(-1, n, n))
+    for rep in range(2):
+        _say(f"odd side N={n} pol={pol}, rep {rep}")
+        ctx.image_batch([st])
+        err = oracle.peak_normalised_error(_host(imgs), ref)[0]
+        _say(f"  vs FP32 oracle {err:.2e}")
+        assert err < IMAGE_TOL
+        assert np.array_equal(_host(sums), c.reshape(o.n_images, -1).sum(1).astype(np.float64))
+        assert oracle.peak_normalised_error(_host(avg), oracle.image_averaging_complex(flat))[0] < IMAGE_TOL
+
+
+def test_batch_odd_side_uniform_and_refusals(ctx):
+    """Uniform weighting at an odd side (a grid whose every cell is covered, so
+    the image is finite), and the odd-side refusals: the DFT transforms,
+    convolutional gridding, an average without images."""
+    import torch
+    from imager_amd import capi
+    from test_gpu_weighting import covering_array, _reference
+    o = covering_array(45, n_ch=2, n_t=1, seed=9)
+    ref, _ = _reference(o, uniform=True)
+    assert np.isfinite(ref).all()
+    d = {k: _dev(getattr(o, k)) for k in ("vis", "u", "v", "w", "flags", "freq", "cable")}
+    imgs = torch.empty((o.n_t, o.n_ch, 45, 45), dtype=torch.complex64, device="cuda")
+    st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                       d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, 45, "XX", imgs,
+                       weighting="U")
+    _say("odd side 45, uniform weighting")
+    ctx.image_batch([st])
+    err = oracle.peak_normalised_error(_host(imgs), ref)[0]
+    _say(f"  vs FP32 oracle {err:.2e}")
+    assert err < IMAGE_TOL
+    avg = torch.empty((45, 45), dtype=torch.complex64, device="cuda")
+    for kw, exc in ((dict(transform=capi.TRANSFORM_DFT_BF16X3), capi.BlinkUnsupported),
+                    (dict(kernel_size=7), capi.BlinkUnsupported),
+                    (dict(average=avg), capi.BlinkInvalidArgument)):
+        bad = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                            d["freq"], d["cable"], 3, o.delta_u, o.delta_v, o.min_uv, 45, "XX",
+                            None if "average" in kw else imgs, **kw)
+        with pytest.raises(exc):
+            ctx.image_batch([bad])
