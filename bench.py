@@ -11,7 +11,8 @@ Default workload: 128-tile MWA-like array (8256 baselines), 64 channels
 170-200 MHz x 10 timesteps = 640 images of 1024^2.  Other workloads:
   mwa128_768ch_1024    the same array, 768 fine channels x 1 timestep, 1024^2
   eda2_1ch_180         256-antenna station, 1 channel, one 180^2 image
-  eda2_768ch_180/_256  the station over 768 fine channels (also --transform dft)
+  eda2_768ch_180/_256  the station over 768 fine channels (also --transform dft);
+                       _128 and _512 the same at those sides
   mixed512             512 single-image sets: 128- or 256-antenna array, grid side
                        drawn from {128, 180, 256, 512, 1024, 2048}, own random flags
                        (0-30 %); one call per step, plus per-size throughput
@@ -52,8 +53,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-WORKLOADS = ("mwa128_64ch10t_1024", "mwa128_768ch_1024", "eda2_1ch_180", "eda2_768ch_180",
-             "eda2_768ch_256", "mixed512", "mwa128_768ch4t_4096")
+WORKLOADS = ("mwa128_64ch10t_1024", "mwa128_768ch_1024", "eda2_1ch_180", "eda2_768ch_128",
+             "eda2_768ch_180", "eda2_768ch_256", "eda2_768ch_512", "mixed512",
+             "mwa128_768ch4t_4096")
 
 
 def make_workload(name: str):
@@ -71,7 +73,7 @@ def make_workload(name: str):
                                freq=synth.mwa_fine_channels(768), height_sigma_m=0.5)
     if name == "eda2_1ch_180":
         return synth.eda2_station(n_ch=1, n_t=1, n_pixels=180, seed=1001)
-    if name in ("eda2_768ch_180", "eda2_768ch_256"):
+    if name in ("eda2_768ch_128", "eda2_768ch_180", "eda2_768ch_256", "eda2_768ch_512"):
         n = int(name.rsplit("_", 1)[1])
         return synth.eda2_station(n_ch=768, n_t=1, n_pixels=n, seed=1004,
                                   freq=synth.mwa_fine_channels(768))
@@ -121,14 +123,17 @@ def compacted(n_bl: int, n_pix: int) -> bool:
 
 
 def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
-                   kernel_size: int = 0) -> float | None:
+                   kernel_size: int = 0, direct: bool = False) -> float | None:
     """HBM bytes a fused kernel must move per launch over n_img images of
     observation o (images cycle over its channels), given the plan: what it
     has to read and write, with every byte counted once.
 
       grid_rows_fft  plan entries (4 B each) + the visibilities once (8 B per
                      baseline compacted, else the 32 B records) + the
-                     transformed non-empty plan rows (8 N B each)
+                     transformed non-empty plan rows (8 N B each); on the
+                     direct path (no plan): u, v, flags once (12 B per
+                     baseline) + the records once + every plan row 0 .. N/2
+      direct_prep    w and the phase coefficient per baseline, the count sums
       fft_cols       the non-empty plan rows (8 N B each) + the image (8 N^2 B)
       vis_compact    the records (32 B) + the compacted value (8 B) per baseline
       plan           per channel: u, v, flags (12 B) read, the packed cell (4 B)
@@ -138,14 +143,19 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
       dft_cols       the stage-1 product + the image"""
     n, n_bl = o.n_pixels, o.n_bl
     entries, nonempty = stats
-    if kernel_size:
-        # convolutional gridding writes and reads every plan row 0 .. N/2
+    if kernel_size or direct:
+        # convolutional gridding and the direct row pass write and read every
+        # plan row 0 .. N/2
         nonempty = np.full_like(nonempty, n // 2 + 1)
     ch = np.arange(n_img) % o.n_ch
     e_sum = float(entries[ch].sum())
     r_sum = float(nonempty[ch].sum())
     h1 = n // 2 + 1
     pad = -(-h1 // 16) * 16
+    if kernel == "grid_rows_fft" and direct:
+        return 12.0 * n_bl + 32.0 * n_bl * n_img + 8.0 * n * r_sum
+    if kernel == "direct_prep":
+        return 12.0 * n_bl + 4.0 * n_img
     if kernel == "grid_rows_fft":
         vis_b = 8.0 if compacted(n_bl, n) else 32.0
         return 4.0 * e_sum + vis_b * n_bl * n_img + 8.0 * n * r_sum
@@ -533,6 +543,7 @@ def main():
                                f"{v['launches_per_step']:.0f})" for k, v in kernels.items()))
     roof = None
     stats = None
+    direct = "direct_prep" in prof   # the direct row pass ran (no plan)
     if N is not None:
         stats = plan_stats(obs, transposed=args.transform == "fft")
         n_launch = max(prof.get("grid_rows_fft", (0, 0))[0], prof.get("grid_tiles", (0, 0))[0],
@@ -557,7 +568,7 @@ def main():
                             "or 2 (dft_cols) real products over padded tiles"}
         else:
             phys = physical_bytes(dom, obs, stats, imgs_per_launch, args.transform,
-                                  args.kernel_size)
+                                  args.kernel_size, direct)
             if phys is not None:
                 ach = phys / avg_s / 1e9
                 per_image = read_traffic(dom, args.workload)
@@ -598,11 +609,12 @@ def main():
     if stats is not None:
         step_bytes = 0.0
         for k, v in kernels.items():
-            if k == "plan":
-                b = v["launches_per_step"] * physical_bytes(k, obs, stats, obs.n_ch)
+            if k == "plan":   # no plan units on the direct path: an empty launch
+                b = 0.0 if direct else \
+                    v["launches_per_step"] * physical_bytes(k, obs, stats, obs.n_ch)
             else:
                 b = physical_bytes(k, obs, stats, job["images_per_step"], args.transform,
-                                   args.kernel_size)
+                                   args.kernel_size, direct)
             step_bytes += b or 0.0
         pipeline.update({"bytes_per_step": step_bytes,
                          "achieved_gbs": step_bytes / (ms_per_step * 1e-3) / 1e9,

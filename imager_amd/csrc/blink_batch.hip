@@ -27,6 +27,8 @@
 #include "blink_gather.hpp"
 #include "blink_kernels.hpp"
 
+#include <algorithm>
+
 namespace blink {
 
 // Row stride (float2) of an LDS tile of `rows` rows: at least the padded row,
@@ -219,6 +221,74 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
             F::store(v, lane, [&](int i, float2 x) { inter[inter_index(N, li, vrow, i)] = x; });
         else
             F::store(v, lane, [&](int i, float2 x) { store_stream(&inter[inter_index(N, li, vrow, i)], x); });
+    }
+}
+
+// Direct row pass for dense sets whose images each have a plan of their own
+// (one timestep per channel): no plan is built.  One workgroup per image
+// holds its whole half plane (plan rows 0 .. N/2) in LDS, walks every
+// baseline of the image in order (direct_walk: flags, u, v and the records
+// coalesced, each record line read once), computes the cells as the plan's
+// count pass does and accumulates each visibility and its conjugate; then it
+// transforms and stores every row (rows without visibilities as zeros: the
+// second pass reads every row).  Built where the half plane fits in LDS with
+// room for a second workgroup per CU (N = 128); tiles of part of the rows
+// would each walk all baselines, and measured slower than the plan path
+// (768 channels x 256^2, 32896 baselines: two tiles per image 0.70 against
+// 0.66 ms per step).  The count sums start at zero (k_direct_prep).
+template <int N, int NT, bool kMulti>
+__global__ void __launch_bounds__(NT)
+k_grid_rows_direct(SetDesc first, const SetDesc* __restrict__ sets,
+                   const int* __restrict__ img_set, int64_t gi0, float2* __restrict__ inter,
+                   const float2* __restrict__ tw) {
+    constexpr int L = BatchLanes<N>::L, TS = padded_len(N), H = N / 2, TEAMS = NT / L;
+    using F = RegFft<N, L>;
+    extern __shared__ float4 rows_direct_lds4[];
+    float2* tile = (float2*)rows_direct_lds4;  // [H + 1][TS]
+    float2* twl = tile + (H + 1) * TS;         // twiddle table copy [N]
+    __shared__ int red[NT / 64 + 1];
+
+    const int li = (int)xcd_order(blockIdx.x, gridDim.x);  // XCD-contiguous images
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const int img = (int)(gi - sd.img_base);  // image within its set
+    const int ch = img % sd.g.n_ch;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < N; i += NT) twl[i] = tw[i];
+    for (int i = tid; i < (H + 1) * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const int n_ok = direct_walk<4>(sd, img, ch, N, 0, H + 1, tid, NT,
+                                    [&](int r, int c) { return &tile[r * TS + pad_index(c)]; });
+    const int tot = block_sum_int(n_ok, red);
+    if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
+    __syncthreads();
+
+    // every row, one team of L lanes per row
+    const int team = tid / L, lane = tid % L;
+    for (int r = team; r <= H; r += TEAMS) {
+        float2* row = tile + r * TS;
+        float2 v[F::V];
+        F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
+        team_sync<false>();
+        F::template run<1>(v, row, lane, TwTable{twl});
+        if (sd.cached_inter)
+            F::store(v, lane, [&](int i, float2 x) { inter[inter_index(N, li, r, i)] = x; });
+        else
+            F::store(v, lane, [&](int i, float2 x) { store_stream(&inter[inter_index(N, li, r, i)], x); });
+    }
+}
+
+// Set-up of the direct row pass, per set of the group: the count sums start
+// at zero, and the phase coefficients of the baselines are written once (the
+// plan's count pass does both for the planned sets).
+__global__ void __launch_bounds__(256)
+k_direct_prep(const SetDesc* __restrict__ sets) {
+    const SetDesc& sd = sets[blockIdx.y];
+    const int n = max(sd.n_img, sd.g.n_bl);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (i < sd.n_img) sd.isums[i] = 0u;
+        if (i < sd.g.n_bl && sd.g.corrections)
+            sd.coef_out[i] = phase_coefficient(sd.g.corrections, sd.w_m, sd.cable_m, (unsigned)i);
     }
 }
 
@@ -680,6 +750,23 @@ static hipError_t launch_rows_reg_t(const SetDesc& first, const SetDesc* sets,
     return hipGetLastError();
 }
 
+template <int N>
+constexpr size_t direct_lds_bytes() {
+    return sizeof(float2) * ((size_t)(N / 2 + 1) * padded_len(N) + N);
+}
+
+template <int N, int NT>
+static hipError_t launch_rows_direct_t(const SetDesc& first, const SetDesc* sets,
+                                       const int* img_set, int n_sets, int64_t gi0, int n_img,
+                                       float* inter, const float* tw, hipStream_t stream) {
+    constexpr size_t lds = direct_lds_bytes<N>();
+    if ((int64_t)lds > g_lds_limit) return hipErrorInvalidConfiguration;
+    auto k = n_sets == 1 ? k_grid_rows_direct<N, NT, false> : k_grid_rows_direct<N, NT, true>;
+    hipLaunchKernelGGL(k, dim3((unsigned)n_img), dim3(NT), lds, stream, first, sets, img_set,
+                       gi0, (float2*)inter, (const float2*)tw);
+    return hipGetLastError();
+}
+
 template <int N, int CB>
 static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
                                     const int* img_set, int n_sets, int64_t gi0, int n_img,
@@ -725,6 +812,9 @@ static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
     X(512, 16) X(512, 32) X(512, 64) \
     X(1024, 8) X(1024, 16) X(1024, 32) \
     X(2048, 4) X(2048, 8) X(2048, 16)
+
+// Register sides of the direct row pass: (side, threads).
+#define BLINK_DIRECT_VARIANTS(X) X(128, 512)
 
 bool batch_reg_supported(int n) {
     switch (n) {
@@ -821,6 +911,35 @@ hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int*
     return hipErrorNotSupported;
 }
 
+bool batch_direct_supported(int n) {
+#define X(NN, T) \
+    if (n == NN) return true;
+    BLINK_DIRECT_VARIANTS(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_grid_rows_direct(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                                   int n_sets, int n, int64_t gi0, int n_img, float* inter,
+                                   const float* tw, hipStream_t stream) {
+    if (n_img == 0) return hipSuccess;
+#define X(NN, T) \
+    if (n == NN) \
+        return launch_rows_direct_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, tw, \
+                                           stream);
+    BLINK_DIRECT_VARIANTS(X)
+#undef X
+    return hipErrorNotSupported;
+}
+
+hipError_t launch_direct_prep(const SetDesc* sets, int n_sets, int max_work,
+                              hipStream_t stream) {
+    if (n_sets == 0 || max_work <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min(64, (max_work + 255) / 256);
+    hipLaunchKernelGGL(k_direct_prep, dim3(blocks, (unsigned)n_sets), dim3(256), 0, stream, sets);
+    return hipGetLastError();
+}
+
 hipError_t set_batch_kernel_attributes() {
     const int max_lds = (int)g_lds_limit;
     hipError_t e = hipSuccess;
@@ -837,6 +956,11 @@ hipError_t set_batch_kernel_attributes() {
     set((const void*)k_grid_rows_reg<NN, T, false, true>);   \
     set((const void*)k_grid_rows_reg<NN, T, true, true>);
     BLINK_ROW_UNIFORM_VARIANTS(X)
+#undef X
+#define X(NN, T)                                             \
+    set((const void*)k_grid_rows_direct<NN, T, false>);   \
+    set((const void*)k_grid_rows_direct<NN, T, true>);
+    BLINK_DIRECT_VARIANTS(X)
 #undef X
 #define X(NN, T)                                  \
     set((const void*)k_cols_reg<NN, T, false>);   \

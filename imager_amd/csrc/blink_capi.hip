@@ -824,6 +824,9 @@ struct BatchGroup {
     bool overlap = false;
     // the register row pass stores its chunks with plain (cached) stores
     bool cached_inter = false;
+    // direct row pass (no plan): dense sets with a plan per image
+    bool direct = false;
+    int direct_work = 0;  // the largest n_img or n_bl of its sets (launch_direct_prep)
 };
 
 struct BatchPlan {
@@ -846,6 +849,7 @@ struct BatchPlan {
     int cached_mode = -1;       // BLINK_INTER_CACHED: 0 never, 1 always, -1 by chunk size
     int overlap_chunks = -1;    // BLINK_OVERLAP_CHUNKS: chunks of the main group (-1: by size)
     int64_t overlap_bytes = 0;  // BLINK_OVERLAP_MIB: target chunk bytes of an overlapped group
+    int direct_mode = -1;       // BLINK_DIRECT: 0 never, 1 every eligible group, -1 dense ones
     // device copy of the blob, when build_batch placed it (arena)
     void* blob_dev = nullptr;
     // host copy of a group's first set descriptor (a kernel argument)
@@ -902,6 +906,14 @@ static int batch_buffer(blink_ctx* ctx, Arena* arena, const char* name, size_t b
 // images; single-image sets of six sides, main group 2048^2: 262k against
 // 250k images/s).
 static constexpr int64_t kOverlapChunkBytes = 168ll << 20;
+// Direct row pass: a group whose half plane fits one workgroup's LDS (sides
+// 128 and generic sides up to 186), whose sets have one timestep per channel
+// and more than kDirectMinPerRow baselines per plan row, grids without a plan
+// (k_grid_rows_direct, k_grid_rows_direct_fft): one workgroup per image walks
+// all its baselines in order, computing their cells, instead of gathering
+// plan entries (768 channels x 128^2, 32896 baselines: 0.32 against 0.60 ms
+// per step).
+static constexpr int64_t kDirectMinPerRow = 64;
 static constexpr int64_t kCachedChunkBytes = 256ll << 20;
 
 static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, BatchPlan& bp,
@@ -925,6 +937,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     if (const char* e = getenv("BLINK_OVERLAP_CHUNKS")) bp.overlap_chunks = std::max(0, atoi(e));
     if (const char* e = getenv("BLINK_INTER_CACHED")) bp.cached_mode = e[0] == '1' ? 1 : 0;
     if (arena) bp.overlap_chunks = 0, bp.overlap_bytes = 0;  // one row-pass buffer
+    // Direct row pass (see kDirectMinPerRow); BLINK_DIRECT=0 / 1 turns it off /
+    // on for every group of a side it supports, dense or not.
+    if (const char* e = getenv("BLINK_DIRECT")) bp.direct_mode = e[0] == '1' ? 1 : 0;
     // groups (grid side, transform, weighting, gridding kernel) in order of
     // first appearance
     std::map<std::tuple<int, int, int, int, int>, int> gidx;
@@ -954,6 +969,30 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         g.sets.push_back(i);
         g.n_img += (int64_t)sets[i].n_ch * sets[i].n_t;
     }
+    // Groups on the direct row pass (no plan): sides whose half plane fits a
+    // workgroup (direct_row_pass_fits), the FFT transform, natural weighting
+    // and nearest-cell gridding, sets with one timestep per channel (every
+    // image has a plan of its own) and dense grids (more than
+    // kDirectMinPerRow baselines per plan row).
+    std::vector<char> set_direct(n_sets, 0);
+    for (BatchGroup& g : bp.groups) {
+        if (g.transform != BLINK_TRANSFORM_FFT || g.uniform || g.kb_size ||
+            bp.direct_mode == 0 || !direct_row_pass_fits(g.n))
+            continue;
+        bool ok = true;
+        int work = 0;
+        for (int i : g.sets) {
+            const int64_t n_bl = (int64_t)sets[i].n_ant * (sets[i].n_ant + 1) / 2;
+            if (bp.direct_mode < 0 &&
+                (sets[i].n_t != 1 || n_bl <= kDirectMinPerRow * (int64_t)(g.n / 2 + 1)))
+                ok = false;
+            work = std::max<int>({work, (int)n_bl, sets[i].n_ch * sets[i].n_t});
+        }
+        if (!ok) continue;
+        g.direct = true;
+        g.direct_work = work;
+        for (int i : g.sets) set_direct[i] = 1;
+    }
     // scratch sizes of the plans
     size_t rs_bytes = 0, en_bytes = 0;
     std::vector<size_t> rs_off(n_sets), en_off(n_sets), is_off(n_sets), co_off(n_sets);
@@ -966,14 +1005,15 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         rs_off[i] = rs_bytes;
         rs_bytes += align16(sizeof(int) * (size_t)st.n_ch * (n + 1)) +
                     align16(sizeof(int) * (size_t)st.n_ch * row_list_len((int)n));
-        en_off[i] = en_bytes;
-        en_bytes += align16(sizeof(uint32_t) * 2 * (size_t)st.n_ch * std::max(n_bl, 1));
         is_off[i] = (size_t)n_img_all;
         n_img_all += (int64_t)st.n_ch * st.n_t;
-        n_units += st.n_ch;
-        n_cells += (int64_t)st.n_ch * n_bl;
         co_off[i] = (size_t)n_bl_all;
         n_bl_all += n_bl;
+        en_off[i] = en_bytes;
+        if (set_direct[i]) continue;  // no plan
+        en_bytes += align16(sizeof(uint32_t) * 2 * (size_t)st.n_ch * std::max(n_bl, 1));
+        n_units += st.n_ch;
+        n_cells += (int64_t)st.n_ch * n_bl;
         min_bl = std::min<int64_t>(min_bl, n_bl);
         max_bl = std::max<int64_t>(max_bl, n_bl);
         n_max = std::max(n_max, st.n_pixels);
@@ -983,6 +1023,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     // (fewer parts shorten the scan, which walks every part of every row:
     // 64 channels x 8256 baselines plan in 30 us with 16 parts, 41 us with 32)
     int64_t parts = std::max<int64_t>(1, (1024 + n_units - 1) / std::max<int64_t>(n_units, 1));
+    if (n_units == 0) min_bl = max_bl = 0, n_max = 1;
     parts = std::min<int64_t>({parts, 64, std::max<int64_t>(1, min_bl / 256)});
     parts = std::max<int64_t>(parts, (max_bl + kPlanPartMax - 1) / kPlanPartMax);
     bp.parts = (int)parts;
@@ -1007,7 +1048,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         bp.compact_on = on;
         bp.compact_dense = dense;
         for (BatchGroup& g : bp.groups) {
-            if (!on || !g.reg || g.n > 2048) continue;
+            if (!on || !g.reg || g.n > 2048 || g.direct) continue;
             for (int i : g.sets) {
                 const int64_t n_bl = (int64_t)sets[i].n_ant * (sets[i].n_ant + 1) / 2;
                 if (!dense && n_bl > 64 * (int64_t)(g.n / 2 + 1)) continue;
@@ -1211,7 +1252,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.uniform = st.weighting == BLINK_WEIGHTING_UNIFORM ? 1 : 0;
             d.kb_size = st.kernel_size;
             d.kb_table = bp.sc[i].kb_table;
-            d.all_rows = st.kernel_size > 0 ? 1 : 0;
+            d.all_rows = st.kernel_size > 0 || g.direct ? 1 : 0;
             d.cached_inter = g.cached_inter ? 1 : 0;
             base += d.n_img;
             desc_of[i] = (int)desc.size();
@@ -1223,6 +1264,7 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     PlanUnit* units = (PlanUnit*)(bp.blob.data() + bp.units_off);
     int64_t u = 0, cell_off = 0;
     for (int i = 0; i < n_sets; i++) {
+        if (set_direct[i]) continue;
         const int n_bl = sets[i].n_ant * (sets[i].n_ant + 1) / 2;
         for (int ch = 0; ch < sets[i].n_ch; ch++) {
             units[u++] = PlanUnit{desc_of[i], ch, cell_off};
@@ -1282,7 +1324,8 @@ static int batch_class(const BatchGroup& g) {
     const int64_t cnt = std::max<int64_t>(1, std::min<int64_t>(g.chunk, g.n_img));
     int c = 0;  // floor(log4(cnt))
     for (int64_t v = cnt; v >= 4; v >>= 2) c++;
-    return cnt == 1 ? 0 : 1 + c;
+    // direct groups tune their column block apart (row tile 0: none)
+    return (cnt == 1 ? 0 : 1 + c) + (g.direct ? 32 : 0);
 }
 
 // Tiles for a register-side group: pinned, tuned for its batch class, or the
@@ -1292,8 +1335,15 @@ static bool has_tiles(const blink_ctx* ctx, const BatchGroup& g) {
 }
 static std::pair<int, int> tiles_for(const blink_ctx* ctx, const BatchGroup& g) {
     auto p = ctx->pinned_tiles.find(g.n);
-    if (p != ctx->pinned_tiles.end()) return p->second;
     auto t = ctx->tuned_tiles.find({g.n, batch_class(g)});
+    if (g.direct) {
+        // a pinned pair pins the column block; the direct row pass has no tile
+        std::pair<int, int> r{0, default_tiles(g.n).second};
+        if (t != ctx->tuned_tiles.end()) r = t->second;
+        if (p != ctx->pinned_tiles.end()) r.second = p->second.second;
+        return r;
+    }
+    if (p != ctx->pinned_tiles.end()) return p->second;
     if (t != ctx->tuned_tiles.end()) return t->second;
     return default_tiles(g.n);
 }
@@ -1327,7 +1377,8 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
                           const BatchTables& t, hipStream_t s) {
     const int n = g.n;
     int rows[8], cols[8];
-    const int nr = batch_row_variants(n, rows, 8);
+    rows[0] = 0;  // direct groups: the one row kernel
+    const int nr = g.direct ? 1 : batch_row_variants(n, rows, 8);
     const int nc = batch_col_variants(n, cols, 8);
     // timed on up to 2 GiB of row-pass buffer (all of a 640-image 1024^2
     // chunk, 30 images at 4096^2), which keeps the tuning under a second
@@ -1338,6 +1389,7 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
     const SetDesc& first = bp.host_desc(g);
     const int* is = t.img_set(g);
     const int ns = (int)g.sets.size();
+    if (g.direct) HIP_TRY(launch_direct_prep(sd, ns, g.direct_work, s));
     // Every (row tile, column block) pair is timed as the pipeline runs it
     // (row pass then column pass), in interleaved rounds; each pair keeps its
     // fastest round, so that clock ramp-up and neighbour noise do not favour
@@ -1351,7 +1403,10 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
             for (int c = 0; c < nc; c++) {
                 float ms;
                 int rc = time_launches(ctx, s, 2, [&] {
-                    hipError_t e = launch_grid_rows_reg(first, sd, is, ns, n, rows[r], 0, cnt,
+                    hipError_t e =
+                        g.direct ? launch_grid_rows_direct(first, sd, is, ns, n, 0, cnt, bp.inter,
+                                                           tw, s)
+                                 : launch_grid_rows_reg(first, sd, is, ns, n, rows[r], 0, cnt,
                                                         bp.inter, tw, s);
                     if (e != hipSuccess) return e;
                     return launch_cols_reg(first, sd, is, ns, n, cols[c], 0, cnt, bp.inter, tw,
@@ -1362,7 +1417,8 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
             }
     int best_r = rows[0], best_c = cols[0];
     float best = 1e30f, t_default = 1e30f;
-    const std::pair<int, int> dflt = default_tiles(n);
+    const std::pair<int, int> dflt =
+        g.direct ? std::pair<int, int>{0, default_tiles(n).second} : default_tiles(n);
     for (int r = 0; r < nr; r++)
         for (int c = 0; c < nc; c++) {
             if (tp[r][c] < best) best = tp[r][c], best_r = rows[r], best_c = cols[c];
@@ -1370,13 +1426,15 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
         }
     if (t_default <= 1.03f * best) best_r = dflt.first, best_c = dflt.second;
     ctx->tuned_tiles[{n, batch_class(g)}] = {best_r, best_c};
-    ctx->last_tuned[n] = {best_r, best_c};
+    if (!g.direct) ctx->last_tuned[n] = {best_r, best_c};  // reported: plan-path tiles
     return BLINK_OK;
 }
 
-// Zeroes the count sums and builds every plan of the call on s.
+// Zeroes the count sums and builds every plan of the call on s (none when
+// every group takes the direct row pass).
 static int enqueue_plans(blink_ctx* ctx, const BatchPlan& bp, const BatchTables& t,
                          hipStream_t s) {
+    if (bp.n_units == 0) return BLINK_OK;
     LAUNCH("plan", s,
            launch_plan_units(t.sets, t.units, bp.n_units, bp.parts, bp.n_max, bp.partial,
                              bp.cells, s));
@@ -1489,6 +1547,8 @@ static int enqueue_chunks(blink_ctx* ctx, const blink_image_set* sets, const Bat
     FftPlan plan;
     make_fft_plan(n, plan);
     const std::pair<int, int> tl = tiles_for(ctx, g);
+    // the direct row pass's count sums and phase coefficients
+    if (g.direct) LAUNCH("direct_prep", s, launch_direct_prep(sd, ns, g.direct_work, s));
     // the second pass is skipped when no set of the group forms its images
     bool any_images = false;
     for (int i : g.sets) any_images |= sets[i].images != nullptr;
@@ -1507,7 +1567,13 @@ static int enqueue_chunks(blink_ctx* ctx, const blink_image_set* sets, const Bat
         // the buffer's previous chunk (k - 2) must have been read
         if (two && k >= 2)
             HIP_TRY(hipStreamWaitEvent(s, ctx->pass_ev[2 * ((k - 2) % kPassEvents) + 1], 0));
-        if (g.reg)
+        if (g.direct && g.reg)
+            LAUNCH("grid_rows_fft", s,
+                   launch_grid_rows_direct(first, sd, is, ns, n, gi0, cnt, buf, tw, s));
+        else if (g.direct)
+            LAUNCH("grid_rows_fft", s,
+                   launch_grid_rows_direct_fft(first, sd, is, ns, plan, gi0, cnt, buf, tw, s));
+        else if (g.reg)
             LAUNCH("grid_rows_fft", s,
                    launch_grid_rows_reg(first, sd, is, ns, n, tl.first, gi0, cnt, buf, tw, s,
                                         g.uniform));
@@ -1644,6 +1710,7 @@ static std::string graph_key(const blink_image_set* sets, int n_sets, const Batc
     put(bp.cached_mode);
     put(bp.overlap_chunks);
     put(bp.overlap_bytes);
+    put(bp.direct_mode);
     for (int i = 0; i < n_sets; i++) {
         const blink_image_set& st = sets[i];
         put(st.n_ant), put(st.n_ch), put(st.n_t);

@@ -86,6 +86,136 @@ struct GatherBatch {
     }
 };
 
+// Per-channel constants of the plan's cell computation: the wavelength, the
+// reciprocals of the wavelength and of the cell sizes (div_rn), and the
+// products k = 1 / (wavelength * cell size) of the one-product rounding
+// (round_quotient_fast); `fast` when all of them are in div_rn_range.
+struct CellUnit {
+    double wl, r_wl, r_du, r_dv, ku, kv;
+    bool fast;
+    __device__ __forceinline__ CellUnit(double freq_hz, const SetDesc& sd) {
+#pragma clang fp contract(off)
+        wl = wavelength_of(freq_hz);
+        r_wl = 1.0 / wl;
+        r_du = 1.0 / sd.du;
+        r_dv = 1.0 / sd.dv;
+        ku = 1.0 / (wl * sd.du);
+        kv = 1.0 / (wl * sd.dv);
+        fast = div_rn_range(wl) && div_rn_range(sd.du) && div_rn_range(sd.dv);
+    }
+};
+
+// cell_of with the min_uv test reduced to a NaN test when min_uv is -inf
+// (sqrt(u^2 + v^2) > -inf holds for every non-NaN u, v) and the divisions by
+// the wavelength and the cell sizes done with div_rn and per-unit reciprocals:
+// the same quotients, so the same cells.  Operands outside div_rn_range (a
+// zero, huge, infinite or NaN coordinate or cell size) take the IEEE division.
+// Almost every cell is decided first by one product per coordinate
+// (round_quotient_fast), which gives the same integers.
+__device__ __forceinline__ bool plan_cell(float u_m, float v_m, const CellUnit& cu,
+                                          const SetDesc& sd, int& up, int& vp) {
+#pragma clang fp contract(off)
+    if (sd.min_uv == -INFINITY) {
+        const double um = (double)u_m, vm = (double)v_m;
+        if (cu.fast && round_quotient_fast(um, cu.ku, up) && round_quotient_fast(vm, cu.kv, vp))
+            return true;
+        const bool fast = cu.fast && div_rn_range(um) && div_rn_range(vm);
+        double u, v, qu, qv;
+        if (fast) {
+            u = div_rn(um, cu.wl, cu.r_wl);
+            v = div_rn(vm, cu.wl, cu.r_wl);
+            qu = div_rn(u, sd.du, cu.r_du);
+            qv = div_rn(v, sd.dv, cu.r_dv);
+        } else {
+            u = um / cu.wl;
+            v = vm / cu.wl;
+            qu = u / sd.du;
+            qv = v / sd.dv;
+        }
+        if (isnan(u) || isnan(v)) return false;
+        const double ru = round(qu);
+        const double rv = round(qv);
+        if (!(fabs(ru) < 1073741824.0) || !(fabs(rv) < 1073741824.0)) return false;
+        up = (int)ru;
+        vp = (int)rv;
+        return true;
+    }
+    return cell_of(u_m, v_m, cu.wl, sd.du, sd.dv, sd.min_uv, sd.n_pixels, up, vp);
+}
+
+// The direct row pass's walk over all baselines of one image (no plan): U
+// baselines per thread and round, `nt` threads; every load of a round (flag,
+// u, v, the record or two, the phase coefficient) is issued at once, so that
+// a round costs one memory round trip.  Each baseline's cell is the plan's
+// (plan_cell, transposed when uv_swap); the corrected visibility is added at
+// its cell and its conjugate at the mirrored cell (N - r, N - c) when those
+// fall in the plan rows r0 .. r0 + nr - 1, through at(row - r0, position),
+// an LDS float2.  Returns the count the plan path gives: entries in the
+// self-conjugate rows (0, and N/2 for even N) count once, others twice.
+template <int U, class At>
+__device__ __forceinline__ int direct_walk(const SetDesc& sd, int64_t img, int ch, int n, int r0,
+                                           int nr, int tid, int nt, At at) {
+    const GatherArgs& g = sd.g;
+    const double nu = g.freq_hz[ch];
+    const CellUnit cu(nu, sd);
+    const float2* rec = (const float2*)g.vis + 4 * (img * g.n_bl);
+    // XX: XX; YY: YY; I: XX and YY; V: XY and YX
+    const int p0 = g.pol == 1 ? 3 : (g.pol == 3 ? 1 : 0);
+    const int p1 = g.pol == 3 ? 2 : 3;
+    const bool two = g.pol >= 2;
+    const int n_bl = g.n_bl;
+    int n_ok = 0;
+    for (int b0 = 0; b0 < n_bl; b0 += nt * U) {
+        int fl[U];
+        float uu[U], vv[U];
+        float2 x0[U], x1[U];
+        double cf[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int bl = b0 + tid + k * nt;
+            fl[k] = 1;
+            uu[k] = vv[k] = 0.f;
+            x0[k] = x1[k] = make_float2(0.f, 0.f);
+            cf[k] = 0.0;
+            if (bl < n_bl) {
+                fl[k] = sd.flags[bl];
+                uu[k] = sd.u_m[bl];
+                vv[k] = sd.v_m[bl];
+                x0[k] = rec[4 * (size_t)bl + p0];
+                if (two) x1[k] = rec[4 * (size_t)bl + p1];
+                if (g.corrections) cf[k] = g.phase_coef[bl];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            int up, vp;
+            if (fl[k] || !plan_cell(uu[k], vv[k], cu, sd, up, vp)) continue;
+            const int r = wrap_index(sd.uv_swap ? up : vp, n);
+            const int c = wrap_index(sd.uv_swap ? vp : up, n);
+            const int r2 = r == 0 ? 0 : n - r;
+            const bool in1 = (unsigned)(r - r0) < (unsigned)nr;
+            const bool in2 = (unsigned)(r2 - r0) < (unsigned)nr;
+            if (!in1 && !in2) continue;
+            float2 v = select_pol2(x0[k], x1[k], g.pol);
+            if (g.corrections) v = cmul(v, unit_phasor(((cf[k] * nu) / kVelLight)));
+            if (!vis_ok(v)) continue;
+            if (in1) {
+                float2* d = at(r - r0, c);
+                atomicAdd(&d->x, v.x);
+                atomicAdd(&d->y, v.y);
+                n_ok += (r == 0 || 2 * r == n) ? 1 : 2;
+            }
+            if (in2) {
+                float2* d = at(r2 - r0, c == 0 ? 0 : n - c);
+                atomicAdd(&d->x, v.x);
+                atomicAdd(&d->y, -v.y);
+                n_ok += (r2 == 0 || 2 * r2 == n) ? 1 : 2;
+            }
+        }
+    }
+    return n_ok;
+}
+
 // Phase coefficient of baseline bl (metres): w - (L[a2] - L[a1]), each term
 // when its correction is on; exp(+2 pi i coef nu / c) is the product of the
 // geometric (src/gpu/corrections_gpu.cpp:61-63) and cable (:27-28) rotations.

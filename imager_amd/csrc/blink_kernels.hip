@@ -444,7 +444,6 @@ hipError_t launch_plan_rows(const float* u_m, const float* v_m, const int* flags
 // any order either way.
 
 constexpr int kUnitThreads = 512;
-constexpr uint32_t kNoCell = 0xffffffffu;
 // baselines per thread in a plan part (parts hold at most kPlanPartMax)
 constexpr int kPlanPerThread = kPlanPartMax / kUnitThreads;
 static_assert(kPlanPartMax % kUnitThreads == 0, "plan parts split evenly over the threads");
@@ -453,41 +452,6 @@ __device__ __forceinline__ void part_range(int n_bl, int parts, int part, int& b
     const int per = (n_bl + parts - 1) / parts;
     b0 = min(n_bl, part * per);
     b1 = min(n_bl, b0 + per);
-}
-
-// cell_of with the min_uv test reduced to a NaN test when min_uv is -inf
-// (sqrt(u^2 + v^2) > -inf holds for every non-NaN u, v) and the divisions by
-// the wavelength and the cell sizes done with div_rn and per-unit reciprocals:
-// the same quotients, so the same cells.  Operands outside div_rn_range (a
-// zero, huge, infinite or NaN coordinate or cell size) take the IEEE division.
-__device__ __forceinline__ bool plan_cell(float u_m, float v_m, double wl, double r_wl,
-                                          double r_du, double r_dv, bool unit_fast,
-                                          const SetDesc& sd, int& up, int& vp) {
-#pragma clang fp contract(off)
-    if (sd.min_uv == -INFINITY) {
-        const double um = (double)u_m, vm = (double)v_m;
-        const bool fast = unit_fast && div_rn_range(um) && div_rn_range(vm);
-        double u, v, qu, qv;
-        if (fast) {
-            u = div_rn(um, wl, r_wl);
-            v = div_rn(vm, wl, r_wl);
-            qu = div_rn(u, sd.du, r_du);
-            qv = div_rn(v, sd.dv, r_dv);
-        } else {
-            u = um / wl;
-            v = vm / wl;
-            qu = u / sd.du;
-            qv = v / sd.dv;
-        }
-        if (isnan(u) || isnan(v)) return false;
-        const double ru = round(qu);
-        const double rv = round(qv);
-        if (!(fabs(ru) < 1073741824.0) || !(fabs(rv) < 1073741824.0)) return false;
-        up = (int)ru;
-        vp = (int)rv;
-        return true;
-    }
-    return cell_of(u_m, v_m, wl, sd.du, sd.dv, sd.min_uv, sd.n_pixels, up, vp);
 }
 
 // Exclusive scan of x over the workgroup (kUnitThreads threads); returns the
@@ -519,9 +483,7 @@ k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ unit
     const PlanUnit pu = units[unit];
     const SetDesc sd = sets[pu.set];
     const int n = sd.n_pixels, n_bl = sd.g.n_bl;
-    const double wl = wavelength_of(sd.g.freq_hz[pu.ch]);
-    const double r_wl = 1.0 / wl, r_du = 1.0 / sd.du, r_dv = 1.0 / sd.dv;
-    const bool unit_fast = div_rn_range(wl) && div_rn_range(sd.du) && div_rn_range(sd.dv);
+    const CellUnit cu(sd.g.freq_hz[pu.ch], sd);
     for (int i = threadIdx.x; i < n; i += kUnitThreads) hist[i] = 0;
     __syncthreads();
     int b0, b1;
@@ -552,7 +514,7 @@ k_plan_count(const SetDesc* __restrict__ sets, const PlanUnit* __restrict__ unit
         if (bl >= b1) break;
         int up, vp;
         uint32_t packed = kNoCell;
-        if (!fl[k] && plan_cell(uu[k], vv[k], wl, r_wl, r_du, r_dv, unit_fast, sd, up, vp)) {
+        if (!fl[k] && plan_cell(uu[k], vv[k], cu, sd, up, vp)) {
             const int r = wrap_index(sd.uv_swap ? up : vp, n);
             packed = ((uint32_t)r << 16) | (uint32_t)wrap_index(sd.uv_swap ? vp : up, n);
             // only rows 0 .. N/2 are planned (the consumers rebuild the rest
@@ -1463,6 +1425,71 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
     return hipGetLastError();
 }
 
+// Direct row pass of the generic kernels (no plan), for dense sets with one
+// timestep per channel, as k_grid_rows_direct does it for the register
+// sides: one workgroup per image with the whole half plane (plan rows
+// 0 .. N/2 of an even or odd side) in LDS, every baseline walked in order
+// (direct_walk), then the rows transformed by teams of one wave and stored
+// row-major as the generic second pass reads them.  Sides whose half plane
+// and team scratch fit in LDS (up to 186).  The count sums start at zero
+// (the direct set-up launch).
+constexpr int kDirectThreads = 1024, kDirectTeam = 64;
+static size_t direct_fft_lds_bytes(int n) {
+    return sizeof(float2) * ((size_t)(n / 2 + 1) * n + (size_t)(kDirectThreads / kDirectTeam) * n);
+}
+
+template <bool kMulti>
+__global__ void __launch_bounds__(kDirectThreads)
+k_grid_rows_direct_fft(SetDesc first, const SetDesc* __restrict__ sets,
+                       const int* __restrict__ img_set, FftPlan plan, int64_t gi0,
+                       float2* __restrict__ inter, const float2* __restrict__ tw) {
+    extern __shared__ float4 grd_lds4[];
+    constexpr int teams = kDirectThreads / kDirectTeam;
+    const int n = plan.n, h = n / 2;
+    float2* tile = (float2*)grd_lds4;  // [h + 1][n]
+    const int t_id = (int)threadIdx.x / kDirectTeam, lane = (int)threadIdx.x % kDirectTeam;
+    float2* tmp = tile + (size_t)(h + 1) * n + (size_t)t_id * n;
+    __shared__ int red[kDirectThreads / 64];
+
+    const int li = (int)xcd_order(blockIdx.x, gridDim.x);
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const int img = (int)(gi - sd.img_base);
+    const int ch = img % sd.g.n_ch;
+    for (int i = threadIdx.x; i < (h + 1) * n; i += kDirectThreads) tile[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const int n_ok = direct_walk<4>(sd, img, ch, n, 0, h + 1, (int)threadIdx.x, kDirectThreads,
+                                    [&](int r, int c) { return &tile[r * n + c]; });
+    const int tot = block_sum_int(n_ok, red);
+    if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
+    __syncthreads();
+    for (int r = t_id; r <= h; r += teams) {
+        float2* row = tile + (size_t)r * n;
+        const float2* res = fft_row<1, false>(row, tmp, plan, tw, lane, kDirectTeam);
+        float2* dst = inter + ((int64_t)li * (h + 1) + r) * n;
+        for (int i = lane; i < n; i += kDirectTeam) store_stream(&dst[i], res[i]);
+        team_sync<false>();
+    }
+}
+
+bool direct_row_pass_fits(int n) {
+    if (batch_reg_supported(n)) return batch_direct_supported(n);
+    return n >= 2 && n <= kMaxPixels && lds_fits(direct_fft_lds_bytes(n));
+}
+
+hipError_t launch_grid_rows_direct_fft(const SetDesc& first, const SetDesc* sets,
+                                       const int* img_set, int n_sets, const FftPlan& plan,
+                                       int64_t gi0, int n_img, float* inter, const float* tw,
+                                       hipStream_t stream) {
+    if (n_img == 0) return hipSuccess;
+    const size_t lds = direct_fft_lds_bytes(plan.n);
+    if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
+    auto k = n_sets > 1 ? k_grid_rows_direct_fft<true> : k_grid_rows_direct_fft<false>;
+    hipLaunchKernelGGL(k, dim3((unsigned)n_img), dim3(kDirectThreads), lds, stream, first, sets,
+                       img_set, plan, gi0, (float2*)inter, (const float2*)tw);
+    return hipGetLastError();
+}
+
 // Convolutional gridding fused with the row FFT (generic kernels, transposed
 // plan: plan rows are grid columns x, positions rows y).  Workgroup = (tile of
 // rb consecutive plan rows r0 .. r1 of 0 .. N/2, image).  Every visibility
@@ -1802,6 +1829,8 @@ hipError_t set_kernel_attributes() {
     set((const void*)k_grid_rows_kb<1, false, false>);
     set((const void*)k_grid_rows_kb<1, true, true>);
     set((const void*)k_grid_rows_kb<1, false, true>);
+    set((const void*)k_grid_rows_direct_fft<false>);
+    set((const void*)k_grid_rows_direct_fft<true>);
     set((const void*)k_grid_rows_fft<1, true, false>);
     set((const void*)k_grid_rows_fft<1, false, false>);
     set((const void*)k_grid_rows_fft<1, true, true>);

@@ -94,6 +94,9 @@ struct SetDesc {
     int cached_inter;
 };
 
+// A packed plan cell that is not gridded (flagged, NaN or out of range).
+constexpr uint32_t kNoCell = 0xffffffffu;
+
 // The set of group image gi, by value.  Kernels are built twice: for a
 // group of one set (kMulti false) the descriptor is `first`, a kernel
 // argument, so its fields arrive with the other arguments and add no
@@ -248,6 +251,23 @@ hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const
                                 int n_sets, int n, int tile, int64_t gi0, int n_img, float* inter,
                                 const float* tw, hipStream_t stream, bool uniform = false);
 int batch_uniform_row_tile(int n);
+// Direct row pass of dense single-timestep sets (no plan; every row of the
+// half plane written, so the sets' descriptors carry all_rows): one
+// workgroup per image with its whole half plane in LDS, for the register
+// sides batch_direct_supported (128) and the generic sides that fit (up to
+// 186); direct_row_pass_fits tells which.  The count sums and phase
+// coefficients come from launch_direct_prep over the group's sets (max_work:
+// the largest n_img or n_bl among them), launched before.
+bool batch_direct_supported(int n);
+bool direct_row_pass_fits(int n);
+hipError_t launch_grid_rows_direct(const SetDesc& first, const SetDesc* sets, const int* img_set,
+                                   int n_sets, int n, int64_t gi0, int n_img, float* inter,
+                                   const float* tw, hipStream_t stream);
+hipError_t launch_grid_rows_direct_fft(const SetDesc& first, const SetDesc* sets,
+                                       const int* img_set, int n_sets, const FftPlan& plan,
+                                       int64_t gi0, int n_img, float* inter, const float* tw,
+                                       hipStream_t stream);
+hipError_t launch_direct_prep(const SetDesc* sets, int n_sets, int max_work, hipStream_t stream);
 hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
                            int n_sets, int n, int tile, int64_t gi0, int n_img,
                            const float* inter, const float* tw, hipStream_t stream);

@@ -1,0 +1,195 @@
+"""The direct row pass: dense sets with one timestep per channel (every image
+has a plan of its own) at sides whose half plane fits one workgroup's LDS
+(128 on the register kernels, generic sides up to 186, odd ones included)
+are gridded without a plan, one workgroup per image walking all baselines of
+its image and computing their cells itself (k_grid_rows_direct,
+k_grid_rows_direct_fft).
+
+Against the oracle's reference-precision pipeline (peak-normalised error below
+1e-5) and the oracle's counts (exact): the 256-antenna station at 128^2,
+180^2, 175^2 and 90^2, the four polarisations, flagged baselines, NaN and
+|V| >= 1e12 visibilities, a uv-distance cut, a group of two sets (one
+averaged), on capture, replay and eagerly (where the launches show the direct
+path); the same calls on the plan path (BLINK_DIRECT=0) for identical count
+sums; and a sparse set with two timesteps forced onto the direct path.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from imager_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+IMAGE_TOL = 1e-5
+FREQ = np.array([151.25e6, 159.375e6, 171.0e6])
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+class _env:
+    """Sets environment variables for the calls inside the block."""
+
+    def __init__(self, **kv):
+        self.kv = kv
+        self.old = {}
+
+    def __enter__(self):
+        for k, v in self.kv.items():
+            self.old[k] = os.environ.get(k)
+            os.environ[k] = str(v)
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def _station(n, seed, **kw):
+    o = synth.eda2_station(n_ch=3, n_t=1, n_pixels=n, seed=seed, freq=FREQ,
+                           height_sigma_m=1.0, bl_flag_fraction=0.1, **kw)
+    good = np.flatnonzero(o.flags == 0)
+    o.vis[0, 1, good[5], 0] = np.nan
+    o.vis[0, 2, good[9], 0] = np.complex64(3e12)
+    o.vis[0, 0, good[11], 3] = np.complex64(complex(0.5, -2e12))
+    return o
+
+
+def _reference(o, pol):
+    ref, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, o.n_pixels, pol)
+    vc = oracle.cable_corrections(oracle.geometric_corrections(o.vis, o.n_ant, o.w, o.freq),
+                                  o.n_ant, o.cable, o.freq)
+    _, c, _ = oracle.gridding(vc, o.n_ant, o.u, o.v, o.flags, o.freq, o.delta_u, o.delta_v,
+                              o.min_uv, o.n_pixels, pol)
+    return ref, c.reshape(o.n_images, -1).sum(1).astype(np.float64)
+
+
+def _set(o, pol, images, sums, average=None):
+    from imager_amd import capi
+    d = {k: _dev(getattr(o, k)) for k in ("vis", "u", "v", "w", "flags", "freq", "cable")}
+    st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                       d["freq"], d["cable"], capi.CORR_GEOMETRIC | capi.CORR_CABLE, o.delta_u,
+                       o.delta_v, o.min_uv, o.n_pixels, pol, images, sums, average=average)
+    return st, d
+
+
+def _buffers(o):
+    import torch
+    n = o.n_pixels
+    imgs = torch.empty((o.n_t, o.n_ch, n, n), dtype=torch.complex64, device="cuda")
+    sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
+    return imgs, sums
+
+
+@pytest.mark.parametrize("n,pol", [(128, "I"), (128, "YY"), (180, "XX"), (175, "V"),
+                                   (90, "XX")])
+def test_direct_sides(ctx, n, pol):
+    o = _station(n, seed=500 + n)
+    assert o.n_bl > 64 * (n // 2 + 1), "a dense grid"
+    ref, c = _reference(o, pol)
+    imgs, sums = _buffers(o)
+    st, keep = _set(o, pol, imgs, sums)
+    for rep in range(3):   # capture, replay, then eagerly with the launches counted
+        eager = rep == 2
+        ctx.profile_enable(eager)
+        ctx.profile_reset()
+        imgs.fill_(float("nan"))
+        _say(f"direct N={n} pol={pol} rep {rep}")
+        ctx.image_batch([st])
+        got = _host(imgs)
+        err = oracle.peak_normalised_error(got, ref)[0]
+        _say(f"  vs FP32 oracle {err:.2e}")
+        assert err < IMAGE_TOL
+        assert not np.any(got.imag)
+        assert np.array_equal(_host(sums), c)
+        if eager:
+            launches = ctx.profile()
+            assert launches["direct_prep"][0] == 1, launches
+            assert launches["grid_rows_fft"][0] == 1, launches
+            assert "plan" not in launches, launches
+    ctx.profile_enable(False)
+
+
+def test_direct_forced_sparse_two_timesteps(ctx):
+    """BLINK_DIRECT=1 puts a sparse 128^2 set with two timesteps per channel on
+    the direct path (each image walks the baselines itself)."""
+    o = synth.small(n_ant=24, n_ch=3, n_t=2, n_pixels=128, seed=31, height_sigma_m=1.0)
+    ref, c = _reference(o, "XX")
+    imgs, sums = _buffers(o)
+    st, keep = _set(o, "XX", imgs, sums)
+    with _env(BLINK_DIRECT=1):
+        for rep in range(3):
+            eager = rep == 2
+            ctx.profile_enable(eager)
+            ctx.profile_reset()
+            imgs.fill_(float("nan"))
+            _say(f"forced direct, rep {rep}")
+            ctx.image_batch([st])
+            assert oracle.peak_normalised_error(_host(imgs), ref)[0] < IMAGE_TOL
+            assert np.array_equal(_host(sums), c)
+            if eager:
+                assert "direct_prep" in ctx.profile()
+    ctx.profile_enable(False)
+
+
+@pytest.mark.parametrize("n", [128, 180])
+def test_direct_group_average_cut_and_plan_path(ctx, n):
+    """Two sets in one group (the image -> set table) at 128^2 and at 180^2,
+    the second with a uv-distance cut (the cell test's min_uv branch) and an
+    average; then the same call on the plan path: identical count sums,
+    images within 1e-6."""
+    import torch
+    a = _station(n, seed=601)
+    b = _station(n, seed=602)
+    b.min_uv = 3.0
+    ra, ca = _reference(a, "XX")
+    rb, cb = _reference(b, "XX")
+    ia, sa = _buffers(a)
+    ib, sb = _buffers(b)
+    avg = torch.empty((n, n), dtype=torch.complex64, device="cuda")
+    sta, ka = _set(a, "XX", ia, sa)
+    stb, kb = _set(b, "XX", ib, sb, average=avg)
+    out = {}
+    for mode in ("1", "0"):
+        with _env(BLINK_DIRECT=mode):
+            for rep in range(3):
+                eager = rep == 2
+                ctx.profile_enable(eager)
+                ctx.profile_reset()
+                for t in (ia, ib, avg):
+                    t.fill_(float("nan"))
+                _say(f"group of two sets N={n}, BLINK_DIRECT={mode}, rep {rep}")
+                ctx.image_batch([sta, stb])
+                ea = oracle.peak_normalised_error(_host(ia), ra)[0]
+                eb = oracle.peak_normalised_error(_host(ib), rb)[0]
+                _say(f"  set a {ea:.2e}, set b {eb:.2e}")
+                assert ea < IMAGE_TOL and eb < IMAGE_TOL
+                assert np.array_equal(_host(sa), ca) and np.array_equal(_host(sb), cb)
+                flat = np.ascontiguousarray(rb.reshape(-1, n, n))
+                ev = oracle.peak_normalised_error(_host(avg), oracle.image_averaging_complex(flat))[0]
+                assert ev < IMAGE_TOL
+                if eager:
+                    launches = ctx.profile()
+                    assert ("direct_prep" in launches) == (mode == "1"), launches
+        out[mode] = (_host(ia).copy(), _host(ib).copy(), _host(avg).copy())
+    ctx.profile_enable(False)
+    for x, y in zip(out["1"], out["0"]):
+        assert oracle.peak_normalised_error(x, y)[0] < 1e-6
