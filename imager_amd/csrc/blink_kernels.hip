@@ -27,6 +27,17 @@ int device_cu_count() {
 // not launched.
 static bool lds_fits(size_t bytes) { return (int64_t)bytes <= g_lds_limit; }
 
+// Sides up to kTwLdsMax copy the twiddle table into LDS in the generic
+// second-pass kernels, so that the Stockham passes do not wait on global
+// loads between team syncs (768 x 180^2: 121 -> 115 us per launch; the
+// plan-path row kernel, whose workgroups are short, measured 444 -> 469 us
+// with it and keeps the global table).
+constexpr int kTwLdsMax = 1024;
+__host__ __device__ constexpr size_t tw_lds_bytes(int n) {
+    return n <= kTwLdsMax ? sizeof(float2) * (size_t)n : 0;
+}
+
+
 // ------------------------------------------------------------------ helpers
 
 __global__ void k_zero(float4* __restrict__ p, int64_t n_vec) {
@@ -1195,7 +1206,11 @@ k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
     const int t_id = kBlockTeam ? 0 : (int)threadIdx.x / team;
     const int lane = kBlockTeam ? (int)threadIdx.x : (int)threadIdx.x % team;
     float2* tmp = rows + (size_t)pairs * ld + (size_t)t_id * n;
-    unsigned char* nonempty = (unsigned char*)(rows + (size_t)pairs * ld + (size_t)teams * n);
+    float2* twl = rows + (size_t)pairs * ld + (size_t)teams * n;  // [n] when n <= kTwLdsMax
+    unsigned char* nonempty = (unsigned char*)twl + tw_lds_bytes(n);
+    const float2* twp = n <= kTwLdsMax ? twl : tw;
+    if (n <= kTwLdsMax)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) twl[i] = tw[i];
 
     const int n_cb = n / cb;
     const int cbk = (int)(blockIdx.x % n_cb);
@@ -1241,7 +1256,7 @@ k_cols_half(SetDesc first, const SetDesc* __restrict__ sets, const int* __restri
         const int t = kBlockTeam ? k : k * teams + t_id;
         if (!kBlockTeam && t >= pairs) break;
         float2* zr = rows + (size_t)t * ld;
-        const float2* res = fft_row<S, kBlockTeam>(zr, tmp, plan, tw, lane, team);
+        const float2* res = fft_row<S, kBlockTeam>(zr, tmp, plan, twp, lane, team);
         if (res != zr) {
             for (int i = lane; i < n; i += team) zr[i] = res[i];
             team_sync<kBlockTeam>();
@@ -1435,7 +1450,8 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
 // (the direct set-up launch).
 constexpr int kDirectThreads = 1024, kDirectTeam = 64;
 static size_t direct_fft_lds_bytes(int n) {
-    return sizeof(float2) * ((size_t)(n / 2 + 1) * n + (size_t)(kDirectThreads / kDirectTeam) * n);
+    return sizeof(float2) *
+           ((size_t)(n / 2 + 1) * n + (size_t)(kDirectThreads / kDirectTeam) * n + n);
 }
 
 template <bool kMulti>
@@ -1449,6 +1465,9 @@ k_grid_rows_direct_fft(SetDesc first, const SetDesc* __restrict__ sets,
     float2* tile = (float2*)grd_lds4;  // [h + 1][n]
     const int t_id = (int)threadIdx.x / kDirectTeam, lane = (int)threadIdx.x % kDirectTeam;
     float2* tmp = tile + (size_t)(h + 1) * n + (size_t)t_id * n;
+    // twiddle table copy [n] after the teams' scratch: the passes read it
+    // from LDS instead of waiting on global loads between team syncs
+    float2* twl = tile + (size_t)(h + 1) * n + (size_t)teams * n;
     __shared__ int red[kDirectThreads / 64];
 
     const int li = (int)xcd_order(blockIdx.x, gridDim.x);
@@ -1457,6 +1476,7 @@ k_grid_rows_direct_fft(SetDesc first, const SetDesc* __restrict__ sets,
     const int img = (int)(gi - sd.img_base);
     const int ch = img % sd.g.n_ch;
     for (int i = threadIdx.x; i < (h + 1) * n; i += kDirectThreads) tile[i] = make_float2(0.f, 0.f);
+    for (int i = threadIdx.x; i < n; i += kDirectThreads) twl[i] = tw[i];
     __syncthreads();
     const int n_ok = direct_walk<4>(sd, img, ch, n, 0, h + 1, (int)threadIdx.x, kDirectThreads,
                                     [&](int r, int c) { return &tile[r * n + c]; });
@@ -1465,7 +1485,7 @@ k_grid_rows_direct_fft(SetDesc first, const SetDesc* __restrict__ sets,
     __syncthreads();
     for (int r = t_id; r <= h; r += teams) {
         float2* row = tile + (size_t)r * n;
-        const float2* res = fft_row<1, false>(row, tmp, plan, tw, lane, kDirectTeam);
+        const float2* res = fft_row<1, false>(row, tmp, plan, twl, lane, kDirectTeam);
         float2* dst = inter + ((int64_t)li * (h + 1) + r) * n;
         for (int i = lane; i < n; i += kDirectTeam) store_stream(&dst[i], res[i]);
         team_sync<false>();
@@ -1671,7 +1691,11 @@ k_cols_half_odd(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
     const int t_id = kBlockTeam ? 0 : (int)threadIdx.x / team;
     const int lane = kBlockTeam ? (int)threadIdx.x : (int)threadIdx.x % team;
     float2* tmp = rows + (size_t)pairs * ld + (size_t)t_id * n;
-    unsigned char* nonempty = (unsigned char*)(rows + (size_t)pairs * ld + (size_t)teams * n);
+    float2* twl = rows + (size_t)pairs * ld + (size_t)teams * n;  // [n] when n <= kTwLdsMax
+    unsigned char* nonempty = (unsigned char*)twl + tw_lds_bytes(n);
+    const float2* twp = n <= kTwLdsMax ? twl : tw;
+    if (n <= kTwLdsMax)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) twl[i] = tw[i];
 
     const int n_slots = h + 1;
     const int n_cb = (n_slots + pairs - 1) / pairs;
@@ -1713,7 +1737,7 @@ k_cols_half_odd(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
         const int t = kBlockTeam ? k : k * teams + t_id;
         if (!kBlockTeam && t >= np) break;
         float2* zr = rows + (size_t)t * ld;
-        const float2* res = fft_row<S, kBlockTeam>(zr, tmp, plan, tw, lane, team);
+        const float2* res = fft_row<S, kBlockTeam>(zr, tmp, plan, twp, lane, team);
         if (res != zr) {
             for (int i = lane; i < n; i += team) zr[i] = res[i];
             team_sync<kBlockTeam>();
@@ -1742,7 +1766,7 @@ static int half_cols_block(int n, int teams, int64_t n_img) {
     for (int cb = std::min(64, n); cb >= 2; cb--) {
         if (cb % 2 || n % cb) continue;
         const int64_t lds = sizeof(float2) * ((int64_t)(cb / 2) * (n + 1) + (int64_t)teams * n) +
-                            (n / 2 + 1);
+                            (int64_t)tw_lds_bytes(n) + (n / 2 + 1);
         if (lds > g_lds_budget) continue;
         if ((int64_t)(n / cb) * n_img >= 256) return cb;
         fallback = cb;
@@ -1763,7 +1787,8 @@ hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int
         // and at least ~256 workgroups over the images
         const int n_slots = n / 2 + 1;
         auto lds_of = [&](int p) {
-            return sizeof(float2) * ((size_t)p * (n + 1) + (size_t)teams * n) + (size_t)(n / 2 + 1);
+            return sizeof(float2) * ((size_t)p * (n + 1) + (size_t)teams * n) + tw_lds_bytes(n) +
+                   (size_t)(n / 2 + 1);
         };
         int pairs = std::min(32, n_slots);
         while (pairs > 1 && (!lds_fits(lds_of(pairs)) ||
@@ -1784,7 +1809,7 @@ hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int
     const int cb = half_cols_block(n, teams, n_img);
     if (!cb) return hipErrorInvalidConfiguration;
     const size_t lds = sizeof(float2) * ((size_t)(cb / 2) * (n + 1) + (size_t)teams * n) +
-                       (size_t)(n / 2 + 1);
+                       tw_lds_bytes(n) + (size_t)(n / 2 + 1);
     if (!lds_fits(lds)) return hipErrorInvalidConfiguration;
     const int64_t blocks = (int64_t)(n / cb) * n_img;
     if (blocks > 0x7fffffff) return hipErrorInvalidConfiguration;
