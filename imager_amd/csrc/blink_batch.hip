@@ -257,7 +257,7 @@ k_grid_rows_direct(SetDesc first, const SetDesc* __restrict__ sets,
     for (int i = tid; i < N; i += NT) twl[i] = tw[i];
     for (int i = tid; i < (H + 1) * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    const int n_ok = direct_walk<4>(sd, img, ch, N, 0, H + 1, tid, NT,
+    const int n_ok = direct_walk<2>(sd, img, ch, N, 0, H + 1, tid, NT,
                                     [&](int r, int c) { return &tile[r * TS + pad_index(c)]; });
     const int tot = block_sum_int(n_ok, red);
     if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);

@@ -976,9 +976,8 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     // kDirectMinPerRow baselines per plan row).
     std::vector<char> set_direct(n_sets, 0);
     for (BatchGroup& g : bp.groups) {
-        if (g.transform != BLINK_TRANSFORM_FFT || g.uniform || g.kb_size ||
-            bp.direct_mode == 0 || !direct_row_pass_fits(g.n))
-            continue;
+        // (the DFT transforms' half plane too: the same walk, no row FFT)
+        if (g.uniform || g.kb_size || bp.direct_mode == 0 || !direct_row_pass_fits(g.n)) continue;
         bool ok = true;
         int work = 0;
         for (int i : g.sets) {
@@ -1489,9 +1488,13 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
     if (is_dft(g.transform)) {
         // per set and chunk: half plane -> stage 1 (complex) -> stage 2 (real, scaled)
         const int prec = dft_prec(g.transform);
+        if (g.direct)
+            LAUNCH("direct_prep", s,
+                   launch_direct_prep(t.group_sets(g), (int)g.sets.size(), g.direct_work, s));
         const uint16_t* pl1 = ctx->dft_planes[n * 8 + 2 + prec];
         const uint16_t* pl2 = ctx->dft_planes[n * 8 + 4 + prec];
-        for (int i : g.sets) {
+        for (size_t j = 0; j < g.sets.size(); j++) {
+            const int i = g.sets[j];
             const blink_image_set& st = sets[i];
             const SetScratch& sc = bp.sc[i];
             const int n_img = st.n_ch * st.n_t;
@@ -1505,11 +1508,15 @@ static int enqueue_group(blink_ctx* ctx, const blink_image_set* sets, const Batc
             ga.n_ch = st.n_ch;
             float* half = inter;
             float* tbuf = inter + 2 * (int64_t)(n / 2 + 1) * n * sc.chunk;
+            const SetDesc& d = ((const SetDesc*)bp.blob.data())[g.first_desc + j];
             for (int img0 = 0; img0 < n_img; img0 += sc.chunk) {
                 const int cnt = std::min(sc.chunk, n_img - img0);
-                LAUNCH("grid_half_rows", s,
-                       launch_grid_half_rows(ga, sc.row_start, sc.entries, n, cnt, img0, half,
-                                             sc.isums, s, g.uniform));
+                if (g.direct)
+                    LAUNCH("grid_half_rows", s, launch_grid_half_direct(d, cnt, img0, half, s));
+                else
+                    LAUNCH("grid_half_rows", s,
+                           launch_grid_half_rows(ga, sc.row_start, sc.entries, n, cnt, img0,
+                                                 half, sc.isums, s, g.uniform));
                 LAUNCH("dft_rows", s, launch_dft_rows(half, pl1, tbuf, n, cnt, prec, s));
                 LAUNCH("dft_cols", s,
                        launch_dft_cols(tbuf, pl2, st.images, n, cnt, img0, sc.isums,

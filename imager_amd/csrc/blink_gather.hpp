@@ -143,15 +143,47 @@ __device__ __forceinline__ bool plan_cell(float u_m, float v_m, const CellUnit& 
     return cell_of(u_m, v_m, cu.wl, sd.du, sd.dv, sd.min_uv, sd.n_pixels, up, vp);
 }
 
-// The direct row pass's walk over all baselines of one image (no plan): U
-// baselines per thread and round, `nt` threads; every load of a round (flag,
-// u, v, the record or two, the phase coefficient) is issued at once, so that
-// a round costs one memory round trip.  Each baseline's cell is the plan's
+// The direct row pass's walk over all baselines of one image (no plan), `nt`
+// threads, U baselines per thread and round.  Every load of a round (flag,
+// u, v, the record or two, the phase coefficient) is issued together, and
+// the next round's loads are issued before this round's arithmetic, so that
+// the memory traffic of one round overlaps the work of the previous one (all
+// workgroups walk in step, so without it the whole chip alternates between
+// load bursts and arithmetic).  Each baseline's cell is the plan's
 // (plan_cell, transposed when uv_swap); the corrected visibility is added at
 // its cell and its conjugate at the mirrored cell (N - r, N - c) when those
 // fall in the plan rows r0 .. r0 + nr - 1, through at(row - r0, position),
 // an LDS float2.  Returns the count the plan path gives: entries in the
 // self-conjugate rows (0, and N/2 for even N) count once, others twice.
+template <int U>
+struct WalkBatch {
+    int fl[U];
+    float uu[U], vv[U];
+    float2 x0[U], x1[U];
+    double cf[U];
+    // Branch-free: indices past the end read the last baseline and are
+    // flagged, and the second pol product and the phase coefficient are read
+    // whether or not they are used (the same 32-byte record; the coefficient
+    // table always exists), so that the compiler can keep the loads of the
+    // next round in flight while this round is consumed.
+    __device__ __forceinline__ void load(const SetDesc& sd, const float2* rec, int p0, int p1,
+                                         int b0, int tid, int nt) {
+        const int n_bl = sd.g.n_bl;
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int b = b0 + tid + k * nt;
+            const int bl = min(b, n_bl - 1);
+            const int f = sd.flags[bl];
+            fl[k] = b < n_bl ? f : 1;
+            uu[k] = sd.u_m[bl];
+            vv[k] = sd.v_m[bl];
+            x0[k] = rec[4 * (size_t)bl + p0];
+            x1[k] = rec[4 * (size_t)bl + p1];
+            cf[k] = sd.g.phase_coef[bl];
+        }
+    }
+};
+
 template <int U, class At>
 __device__ __forceinline__ int direct_walk(const SetDesc& sd, int64_t img, int ch, int n, int r0,
                                            int nr, int tid, int nt, At at) {
@@ -162,42 +194,26 @@ __device__ __forceinline__ int direct_walk(const SetDesc& sd, int64_t img, int c
     // XX: XX; YY: YY; I: XX and YY; V: XY and YX
     const int p0 = g.pol == 1 ? 3 : (g.pol == 3 ? 1 : 0);
     const int p1 = g.pol == 3 ? 2 : 3;
-    const bool two = g.pol >= 2;
     const int n_bl = g.n_bl;
     int n_ok = 0;
+    if (n_bl <= 0) return 0;
+    WalkBatch<U> cur;
+    cur.load(sd, rec, p0, p1, 0, tid, nt);
     for (int b0 = 0; b0 < n_bl; b0 += nt * U) {
-        int fl[U];
-        float uu[U], vv[U];
-        float2 x0[U], x1[U];
-        double cf[U];
-#pragma unroll
-        for (int k = 0; k < U; k++) {
-            const int bl = b0 + tid + k * nt;
-            fl[k] = 1;
-            uu[k] = vv[k] = 0.f;
-            x0[k] = x1[k] = make_float2(0.f, 0.f);
-            cf[k] = 0.0;
-            if (bl < n_bl) {
-                fl[k] = sd.flags[bl];
-                uu[k] = sd.u_m[bl];
-                vv[k] = sd.v_m[bl];
-                x0[k] = rec[4 * (size_t)bl + p0];
-                if (two) x1[k] = rec[4 * (size_t)bl + p1];
-                if (g.corrections) cf[k] = g.phase_coef[bl];
-            }
-        }
+        WalkBatch<U> nxt;
+        nxt.load(sd, rec, p0, p1, b0 + nt * U, tid, nt);  // past the end: flagged
 #pragma unroll
         for (int k = 0; k < U; k++) {
             int up, vp;
-            if (fl[k] || !plan_cell(uu[k], vv[k], cu, sd, up, vp)) continue;
+            if (cur.fl[k] || !plan_cell(cur.uu[k], cur.vv[k], cu, sd, up, vp)) continue;
             const int r = wrap_index(sd.uv_swap ? up : vp, n);
             const int c = wrap_index(sd.uv_swap ? vp : up, n);
             const int r2 = r == 0 ? 0 : n - r;
             const bool in1 = (unsigned)(r - r0) < (unsigned)nr;
             const bool in2 = (unsigned)(r2 - r0) < (unsigned)nr;
             if (!in1 && !in2) continue;
-            float2 v = select_pol2(x0[k], x1[k], g.pol);
-            if (g.corrections) v = cmul(v, unit_phasor(((cf[k] * nu) / kVelLight)));
+            float2 v = select_pol2(cur.x0[k], cur.x1[k], g.pol);
+            if (g.corrections) v = cmul(v, unit_phasor(((cur.cf[k] * nu) / kVelLight)));
             if (!vis_ok(v)) continue;
             if (in1) {
                 float2* d = at(r - r0, c);
@@ -212,6 +228,7 @@ __device__ __forceinline__ int direct_walk(const SetDesc& sd, int64_t img, int c
                 n_ok += (r2 == 0 || 2 * r2 == n) ? 1 : 2;
             }
         }
+        cur = nxt;
     }
     return n_ok;
 }

@@ -27,6 +27,9 @@ int device_cu_count() {
 // not launched.
 static bool lds_fits(size_t bytes) { return (int64_t)bytes <= g_lds_limit; }
 
+// Workgroup of the generic direct row passes (one per image) and its teams.
+constexpr int kDirectThreads = 1024, kDirectTeam = 64;
+
 // Sides up to kTwLdsMax copy the twiddle table into LDS in the generic
 // second-pass kernels, so that the Stockham passes do not wait on global
 // loads between team syncs (768 x 180^2: 121 -> 115 us per launch; the
@@ -883,6 +886,44 @@ hipError_t launch_grid_half_rows(const GatherArgs& g, const int* row_start, cons
     return hipGetLastError();
 }
 
+// Direct variant of k_grid_half_rows for dense sets with one timestep per
+// channel (no plan; see k_grid_rows_direct_fft): one workgroup per image of
+// the chunk holds rows v = 0 .. N/2 in LDS, walks every baseline in order and
+// writes the half plane row-major to half[li][v][u], rows without entries as
+// zeros.  The count sums start at zero (the direct set-up launch).
+__global__ void __launch_bounds__(kDirectThreads)
+k_grid_half_direct(SetDesc sd, int64_t img0, float2* __restrict__ half) {
+    extern __shared__ float4 ghd_lds4[];
+    const int n = sd.n_pixels, h = n / 2;
+    float2* tile = (float2*)ghd_lds4;  // [h + 1][n]
+    __shared__ int red[kDirectThreads / 64];
+    const int64_t li = xcd_order(blockIdx.x, gridDim.x);
+    const int64_t img = img0 + li;  // image within the set
+    const int ch = (int)(img % sd.g.n_ch);
+    const int q = (h + 1) * n / 2;  // float4 per image (N even)
+    for (int i = threadIdx.x; i < q; i += kDirectThreads) ghd_lds4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    const int n_ok = direct_walk<2>(sd, img, ch, n, 0, h + 1, (int)threadIdx.x, kDirectThreads,
+                                    [&](int r, int c) { return &tile[r * n + c]; });
+    const int tot = block_sum_int(n_ok, red);
+    if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
+    __syncthreads();
+    float4* dst = (float4*)(half + li * (h + 1) * (int64_t)n);
+    for (int i = threadIdx.x; i < q; i += kDirectThreads) dst[i] = ghd_lds4[i];
+}
+
+hipError_t launch_grid_half_direct(const SetDesc& sd, int64_t n_img, int64_t img0, float* half,
+                                   hipStream_t stream) {
+    if (n_img == 0) return hipSuccess;
+    const int n = sd.n_pixels;
+    if (n % 2 || sd.uv_swap) return hipErrorInvalidValue;
+    const size_t lds = sizeof(float2) * (size_t)(n / 2 + 1) * n;
+    if (!lds_fits(lds) || n_img > 0x7fffffff) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k_grid_half_direct, dim3((unsigned)n_img), dim3(kDirectThreads), lds,
+                       stream, sd, img0, (float2*)half);
+    return hipGetLastError();
+}
+
 // grid /= count per cell, the "U" weighting of src/pacer_imager.cpp:421-428.
 __global__ void k_uniform_weight(float2* __restrict__ grids, const float* __restrict__ counts,
                                  int64_t n) {
@@ -1448,7 +1489,6 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
 // row-major as the generic second pass reads them.  Sides whose half plane
 // and team scratch fit in LDS (up to 186).  The count sums start at zero
 // (the direct set-up launch).
-constexpr int kDirectThreads = 1024, kDirectTeam = 64;
 static size_t direct_fft_lds_bytes(int n) {
     return sizeof(float2) *
            ((size_t)(n / 2 + 1) * n + (size_t)(kDirectThreads / kDirectTeam) * n + n);
@@ -1478,7 +1518,7 @@ k_grid_rows_direct_fft(SetDesc first, const SetDesc* __restrict__ sets,
     for (int i = threadIdx.x; i < (h + 1) * n; i += kDirectThreads) tile[i] = make_float2(0.f, 0.f);
     for (int i = threadIdx.x; i < n; i += kDirectThreads) twl[i] = tw[i];
     __syncthreads();
-    const int n_ok = direct_walk<4>(sd, img, ch, n, 0, h + 1, (int)threadIdx.x, kDirectThreads,
+    const int n_ok = direct_walk<2>(sd, img, ch, n, 0, h + 1, (int)threadIdx.x, kDirectThreads,
                                     [&](int r, int c) { return &tile[r * n + c]; });
     const int tot = block_sum_int(n_ok, red);
     if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
@@ -1854,6 +1894,7 @@ hipError_t set_kernel_attributes() {
     set((const void*)k_grid_rows_kb<1, false, false>);
     set((const void*)k_grid_rows_kb<1, true, true>);
     set((const void*)k_grid_rows_kb<1, false, true>);
+    set((const void*)k_grid_half_direct);
     set((const void*)k_grid_rows_direct_fft<false>);
     set((const void*)k_grid_rows_direct_fft<true>);
     set((const void*)k_grid_rows_fft<1, true, false>);

@@ -268,6 +268,10 @@ hipError_t launch_grid_rows_direct_fft(const SetDesc& first, const SetDesc* sets
                                        int64_t gi0, int n_img, float* inter, const float* tw,
                                        hipStream_t stream);
 hipError_t launch_direct_prep(const SetDesc* sets, int n_sets, int max_work, hipStream_t stream);
+// The DFT transform's half plane by the direct row pass (rows v, sd.uv_swap
+// 0), for the images img0 .. img0 + n_img - 1 of one set.
+hipError_t launch_grid_half_direct(const SetDesc& sd, int64_t n_img, int64_t img0, float* half,
+                                   hipStream_t stream);
 hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,
                            int n_sets, int n, int tile, int64_t gi0, int n_img,
                            const float* inter, const float* tw, hipStream_t stream);

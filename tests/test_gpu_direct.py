@@ -128,6 +128,53 @@ def test_direct_sides(ctx, n, pol):
     ctx.profile_enable(False)
 
 
+@pytest.mark.parametrize("n,transform", [(180, "TRANSFORM_DFT_BF16X3"),
+                                         (128, "TRANSFORM_DFT_FP16X3")])
+def test_direct_dft_half_plane(ctx, n, transform):
+    """The split-precision DFT transforms take their half plane from the
+    direct walk too (k_grid_half_direct): images within 1e-5 of the FP32 and
+    FP64 oracles, counts exact, and the plan path's images within the same
+    tolerance."""
+    import torch
+    from imager_amd import capi
+    o = _station(n, seed=900 + n)
+    ref, c = _reference(o, "XX")
+    ref64 = oracle.run_f64(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                           o.delta_u, o.delta_v, o.min_uv, n, "XX")
+    d = {k: _dev(getattr(o, k)) for k in ("vis", "u", "v", "w", "flags", "freq", "cable")}
+    imgs, sums = _buffers(o)
+    st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                       d["freq"], d["cable"], capi.CORR_GEOMETRIC | capi.CORR_CABLE, o.delta_u,
+                       o.delta_v, o.min_uv, n, "XX", imgs, sums,
+                       transform=getattr(capi, transform))
+    out = {}
+    for mode in ("1", "0"):
+        with _env(BLINK_DIRECT=mode):
+            for rep in range(3):
+                eager = rep == 2
+                ctx.profile_enable(eager)
+                ctx.profile_reset()
+                imgs.fill_(float("nan"))
+                _say(f"DFT {transform} N={n}, BLINK_DIRECT={mode}, rep {rep}")
+                ctx.image_batch([st])
+                got = _host(imgs)
+                e32 = oracle.peak_normalised_error(got, ref)[0]
+                e64 = oracle.peak_normalised_error(got, ref64)[0]
+                _say(f"  vs FP32 oracle {e32:.2e}, vs FP64 {e64:.2e}")
+                assert e32 < IMAGE_TOL and e64 < IMAGE_TOL
+                assert not np.any(got.imag)
+                assert np.array_equal(_host(sums), c)
+                if eager:
+                    launches = ctx.profile()
+                    assert ("direct_prep" in launches) == (mode == "1"), launches
+                    assert ("plan" in launches) == (mode == "0"), launches
+        out[mode] = _host(imgs).copy()
+    ctx.profile_enable(False)
+    # the gridded planes differ only in FP32 summation order, which the split
+    # into 16-bit hi + lo operands can carry up to the transform's own error
+    assert oracle.peak_normalised_error(out["1"], out["0"])[0] < IMAGE_TOL
+
+
 def test_direct_forced_sparse_two_timesteps(ctx):
     """BLINK_DIRECT=1 puts a sparse 128^2 set with two timesteps per channel on
     the direct path (each image walks the baselines itself)."""
