@@ -175,6 +175,26 @@ def test_direct_dft_half_plane(ctx, n, transform):
     assert oracle.peak_normalised_error(out["1"], out["0"])[0] < IMAGE_TOL
 
 
+@pytest.mark.parametrize("n", [128, 180])
+def test_direct_average_without_images(ctx, n):
+    """A direct set imaged for its mean only (no image buffer): the mean from
+    the row-pass buffers against the oracle's, and the count sums."""
+    import torch
+    o = _station(n, seed=950 + n)
+    ref, c = _reference(o, "XX")
+    sums = torch.empty(o.n_images, dtype=torch.float64, device="cuda")
+    avg = torch.full((n, n), float("nan"), dtype=torch.complex64, device="cuda")
+    st, keep = _set(o, "XX", None, sums, average=avg)
+    flat = np.ascontiguousarray(ref.reshape(-1, n, n))
+    for rep in range(2):
+        _say(f"direct average only N={n}, rep {rep}")
+        ctx.image_batch([st])
+        err = oracle.peak_normalised_error(_host(avg), oracle.image_averaging_complex(flat))[0]
+        _say(f"  mean vs oracle {err:.2e}")
+        assert err < IMAGE_TOL
+        assert np.array_equal(_host(sums), c)
+
+
 def test_direct_forced_sparse_two_timesteps(ctx):
     """BLINK_DIRECT=1 puts a sparse 128^2 set with two timesteps per channel on
     the direct path (each image walks the baselines itself)."""
