@@ -969,13 +969,28 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         g.sets.push_back(i);
         g.n_img += (int64_t)sets[i].n_ch * sets[i].n_t;
     }
+    // the main group (most image bytes) keeps its own row-pass buffer (two
+    // with overlap); the others share a second one on the aux stream
+    {
+        int64_t best = -1;
+        for (size_t k = 0; k < bp.groups.size(); k++) {
+            const int64_t bytes = bp.groups[k].n_img * (int64_t)bp.groups[k].n * bp.groups[k].n;
+            if (bytes > best) best = bytes, bp.main = (int)k;
+        }
+    }
     // Groups on the direct row pass (no plan): sides whose half plane fits a
     // workgroup (direct_row_pass_fits), the FFT transform, natural weighting
     // and nearest-cell gridding, sets with one timestep per channel (every
     // image has a plan of its own) and dense grids (more than
-    // kDirectMinPerRow baselines per plan row).
+    // kDirectMinPerRow baselines per plan row); with one workgroup per image,
+    // a group needs images for at least half the CUs, unless it runs beside
+    // the call's main group (one 180^2 image: 0.129 against 0.047 ms on the
+    // plan path).
     std::vector<char> set_direct(n_sets, 0);
+    const int64_t cus = device_cu_count();
     for (BatchGroup& g : bp.groups) {
+        const bool beside_main = bp.groups.size() > 1 && &g - bp.groups.data() != bp.main;
+        if (bp.direct_mode < 0 && 2 * g.n_img < cus && !beside_main) continue;
         // (the DFT transforms' half plane too: the same walk, no row FFT)
         if (g.uniform || g.kb_size || bp.direct_mode == 0 || !direct_row_pass_fits(g.n)) continue;
         bool ok = true;
@@ -1098,15 +1113,6 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
     }
     bp.inter_budget = inter_budget;
 
-    // the main group (most image bytes) keeps its own row-pass buffer (two
-    // with overlap); the others share a second one on the aux stream
-    {
-        int64_t best = -1;
-        for (size_t k = 0; k < bp.groups.size(); k++) {
-            const int64_t bytes = bp.groups[k].n_img * (int64_t)bp.groups[k].n * bp.groups[k].n;
-            if (bytes > best) best = bytes, bp.main = (int)k;
-        }
-    }
     // plans of a group: one per (set, channel)
     auto group_channels = [&](const BatchGroup& g) {
         int64_t c = 0;

@@ -113,7 +113,8 @@ def test_direct_sides(ctx, n, pol):
         ctx.profile_reset()
         imgs.fill_(float("nan"))
         _say(f"direct N={n} pol={pol} rep {rep}")
-        ctx.image_batch([st])
+        with _env(BLINK_DIRECT=1):   # 3 images: forced
+            ctx.image_batch([st])
         got = _host(imgs)
         err = oracle.peak_normalised_error(got, ref)[0]
         _say(f"  vs FP32 oracle {err:.2e}")
@@ -176,6 +177,35 @@ def test_direct_dft_half_plane(ctx, n, transform):
 
 
 @pytest.mark.parametrize("n", [128, 180])
+def test_direct_chosen_for_many_images(ctx, n):
+    """Without BLINK_DIRECT a dense single-timestep set with images for at
+    least half the CUs takes the direct path by itself, and a call of three
+    such images does not (one workgroup per image would leave the GPU idle)."""
+    import torch
+    o = synth.eda2_station(n_ch=160, n_t=1, n_pixels=n, seed=970 + n,
+                           freq=synth.mwa_fine_channels(160), height_sigma_m=1.0)
+    ref, c = _reference(o, "XX")
+    imgs, sums = _buffers(o)
+    st, keep = _set(o, "XX", imgs, sums)
+    small = _station(n, seed=980 + n)
+    imgs2, sums2 = _buffers(small)
+    st2, keep2 = _set(small, "XX", imgs2, sums2)
+    ctx.profile_enable(True)
+    try:
+        for call, many in (([st], True), ([st2], False)):
+            ctx.profile_reset()
+            _say(f"auto choice N={n}, {'160' if many else '3'} images")
+            ctx.image_batch(call)
+            assert ("direct_prep" in ctx.profile()) == many
+    finally:
+        ctx.profile_enable(False)
+    err = oracle.peak_normalised_error(_host(imgs), ref)[0]
+    _say(f"  160 images vs FP32 oracle {err:.2e}")
+    assert err < IMAGE_TOL
+    assert np.array_equal(_host(sums), c)
+
+
+@pytest.mark.parametrize("n", [128, 180])
 def test_direct_average_without_images(ctx, n):
     """A direct set imaged for its mean only (no image buffer): the mean from
     the row-pass buffers against the oracle's, and the count sums."""
@@ -188,7 +218,8 @@ def test_direct_average_without_images(ctx, n):
     flat = np.ascontiguousarray(ref.reshape(-1, n, n))
     for rep in range(2):
         _say(f"direct average only N={n}, rep {rep}")
-        ctx.image_batch([st])
+        with _env(BLINK_DIRECT=1):
+            ctx.image_batch([st])
         err = oracle.peak_normalised_error(_host(avg), oracle.image_averaging_complex(flat))[0]
         _say(f"  mean vs oracle {err:.2e}")
         assert err < IMAGE_TOL
