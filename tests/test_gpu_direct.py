@@ -226,6 +226,34 @@ def test_direct_average_without_images(ctx, n):
         assert np.array_equal(_host(sums), c)
 
 
+@pytest.mark.parametrize("n", [128, 180])
+def test_direct_several_chunks(ctx, n):
+    """A direct group cut into row-pass chunks (BLINK_INTER_MIB=1: 15 images
+    per chunk at 128^2, 8 at 180^2, of 20), so that the launches past the
+    first chunk index their images from the chunk's offset."""
+    o = synth.eda2_station(n_ch=20, n_t=1, n_pixels=n, seed=990 + n, height_sigma_m=1.0,
+                           freq=synth.mwa_fine_channels(20))
+    ref, c = _reference(o, "XX")
+    imgs, sums = _buffers(o)
+    st, keep = _set(o, "XX", imgs, sums)
+    ctx.profile_enable(True)
+    ctx.profile_reset()
+    try:
+        with _env(BLINK_DIRECT=1, BLINK_INTER_MIB=1):
+            imgs.fill_(float("nan"))
+            _say(f"direct in chunks N={n}")
+            ctx.image_batch([st])
+            launches = ctx.profile()
+    finally:
+        ctx.profile_enable(False)
+    assert launches["grid_rows_fft"][0] == (2 if n == 128 else 3), launches
+    assert launches["direct_prep"][0] == 1, launches
+    err = oracle.peak_normalised_error(_host(imgs), ref)[0]
+    _say(f"  vs FP32 oracle {err:.2e}")
+    assert err < IMAGE_TOL
+    assert np.array_equal(_host(sums), c)
+
+
 def test_direct_forced_sparse_two_timesteps(ctx):
     """BLINK_DIRECT=1 puts a sparse 128^2 set with two timesteps per channel on
     the direct path (each image walks the baselines itself)."""
