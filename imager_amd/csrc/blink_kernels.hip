@@ -1801,9 +1801,9 @@ k_cols_half_odd(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
 // n, at most 64, whose LDS rows fit next to the team scratch and that still
 // gives at least 256 workgroups over the n_img images (one per CU), else the
 // smallest, 2 (a single 180^2 image then runs 90 workgroups, not 3).
-static int half_cols_block(int n, int teams, int64_t n_img) {
+static int half_cols_block(int n, int teams, int64_t n_img, int max_cb) {
     int fallback = 0;
-    for (int cb = std::min(64, n); cb >= 2; cb--) {
+    for (int cb = std::min(max_cb, n); cb >= 2; cb--) {
         if (cb % 2 || n % cb) continue;
         const int64_t lds = sizeof(float2) * ((int64_t)(cb / 2) * (n + 1) + (int64_t)teams * n) +
                             (int64_t)tw_lds_bytes(n) + (n / 2 + 1);
@@ -1819,18 +1819,24 @@ hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int
                             const float* inter, const float* tw, hipStream_t stream) {
     if (n_img == 0) return hipSuccess;
     const int n = plan.n;
-    const int team = fft_team_for(n);
+    // Sides up to 256: teams of 16 lanes and at most one column pair per team,
+    // so that every pair of a workgroup is transformed in one round (768 x
+    // 180^2: 30-column blocks with 16-lane teams 84 us per launch, against
+    // 115 us with 60-column blocks and 32-lane teams in two rounds).
+    const bool one_round = n <= 256;
+    const int team = one_round ? 16 : fft_team_for(n);
     const bool block_team = team == kFftThreads;
     const int teams = block_team ? 1 : kFftThreads / team;
     if (n % 2) {
-        // odd N: slots of column pairs per workgroup, as many as fit up to 32,
-        // and at least ~256 workgroups over the images
+        // odd N: slots of column pairs per workgroup, as many as fit up to 32
+        // (one per team up to 256), and at least ~256 workgroups over the
+        // images
         const int n_slots = n / 2 + 1;
         auto lds_of = [&](int p) {
             return sizeof(float2) * ((size_t)p * (n + 1) + (size_t)teams * n) + tw_lds_bytes(n) +
                    (size_t)(n / 2 + 1);
         };
-        int pairs = std::min(32, n_slots);
+        int pairs = std::min(one_round ? teams : 32, n_slots);
         while (pairs > 1 && (!lds_fits(lds_of(pairs)) ||
                              (int64_t)((n_slots + pairs - 1) / pairs) * n_img < 256))
             pairs--;
@@ -1846,7 +1852,7 @@ hipError_t launch_cols_half(const SetDesc& first, const SetDesc* sets, const int
                            (const float2*)tw);
         return hipGetLastError();
     }
-    const int cb = half_cols_block(n, teams, n_img);
+    const int cb = half_cols_block(n, teams, n_img, one_round ? 2 * teams : 64);
     if (!cb) return hipErrorInvalidConfiguration;
     const size_t lds = sizeof(float2) * ((size_t)(cb / 2) * (n + 1) + (size_t)teams * n) +
                        tw_lds_bytes(n) + (size_t)(n / 2 + 1);
