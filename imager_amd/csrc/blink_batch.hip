@@ -278,6 +278,175 @@ k_grid_rows_direct(SetDesc first, const SetDesc* __restrict__ sets,
     }
 }
 
+// Largest of a and b over the workgroup (every thread gets both); NaN
+// operands are ignored (fmaxf).
+__device__ __forceinline__ void block_max2(float& a, float& b, float* scratch) {
+    for (int o = 32; o > 0; o >>= 1) {
+        a = fmaxf(a, __shfl_xor(a, o, 64));
+        b = fmaxf(b, __shfl_xor(b, o, 64));
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    if (lane == 0) scratch[wid] = a, scratch[nw + wid] = b;
+    __syncthreads();
+    for (int w = 0; w < nw; w++) a = fmaxf(a, scratch[w]), b = fmaxf(b, scratch[nw + w]);
+}
+
+// Lanes per row of the windowed direct row pass's transforms: 4 values per
+// lane, so that a team keeps two rows in registers.
+constexpr int kWinLanes = 64;
+
+// Transforms local rows 0 .. nr - 1 of an LDS tile of stride s and stores
+// them as plan rows r0 .. of image li of the row-pass buffer.  kFull: whole
+// rows; else each row holds positions 0 .. e_c and N - e_c .. N - 1 (at
+// e_c + 1 .. 2 e_c), the others zero.  Each team of kWinLanes lanes takes two
+// rows at a time into registers and uses their LDS (2 s >= padded_len(N)
+// elements) as the transform's scratch.
+template <int N, int NT, bool kFull>
+__device__ __forceinline__ void win_transform_rows(float2* tile, int r0, int nr, int s, int e_c,
+                                                   int li, float2* __restrict__ inter,
+                                                   const float2* twl, bool cached) {
+    constexpr int L = kWinLanes, TEAMS = NT / L;
+    using F = RegFft<N, L>;
+    const int team = (int)threadIdx.x / L, lane = (int)threadIdx.x % L;
+    const int c_shift = N - 2 * e_c - 1;
+    auto at = [&](const float2* row, int i) {
+        if constexpr (kFull) return row[i];
+        const int k = i <= e_c ? i : (i >= N - e_c ? i - c_shift : -1);
+        return k >= 0 ? row[k] : make_float2(0.f, 0.f);
+    };
+    auto put = [&](int r, const float2(&v)[F::V]) {
+        if (cached)
+            F::store(v, lane, [&](int i, float2 x) { inter[inter_index(N, li, r, i)] = x; });
+        else
+            F::store(v, lane,
+                     [&](int i, float2 x) { store_stream(&inter[inter_index(N, li, r, i)], x); });
+    };
+    for (int p = team; 2 * p < nr; p += TEAMS) {
+        float2* base = tile + 2 * p * s;
+        const bool two = 2 * p + 1 < nr;
+        float2 v0[F::V], v1[F::V];
+        F::load(v0, lane, [&](int i) { return at(base, i); });
+        if (two) F::load(v1, lane, [&](int i) { return at(base + s, i); });
+        team_sync<false>();
+        F::template run<1>(v0, base, lane, TwTable{twl});
+        put(r0 + 2 * p, v0);
+        if (two) {
+            team_sync<false>();
+            F::template run<1>(v1, base, lane, TwTable{twl});
+            put(r0 + 2 * p + 1, v1);
+        }
+    }
+}
+
+// The windowed direct row pass's fallback: blocks of rb whole rows, each
+// walking every baseline; returns the thread's count.  Kept out of line, so
+// that the window's path keeps its registers.
+template <int N, int NT>
+__device__ __attribute__((noinline)) int win_blocks(const SetDesc& sd, int img, int ch, int li,
+                                                    float2* tile, int cap,
+                                                    float2* __restrict__ inter,
+                                                    const float2* twl) {
+    constexpr int H = N / 2;
+    const int tid = (int)threadIdx.x;
+    const int rb = (cap / N) & ~1;  // rows per block (the launch checks rb >= 2)
+    int n_ok = 0;
+    for (int r0 = 0; r0 <= H; r0 += rb) {
+        const int nr = min(rb, H + 1 - r0);
+        __syncthreads();  // the previous block's transforms are done with the tile
+        for (int i = tid; i < ((nr + 1) & ~1) * N; i += NT) tile[i] = make_float2(0.f, 0.f);
+        __syncthreads();
+        n_ok += direct_walk<2>(sd, img, ch, N, r0, nr, tid, NT,
+                               [&](int r, int c) { return &tile[r * N + c]; });
+        __syncthreads();
+        win_transform_rows<N, NT, true>(tile, r0, nr, N, 0, li, inter, twl, sd.cached_inter != 0);
+    }
+    return n_ok;
+}
+
+// Direct row pass at sides whose half plane does not fit in LDS (N = 256).
+// A baseline's cell is round((x / wavelength) / cell size) per coordinate,
+// and |cell| grows with |x|, so the cells of an image lie within E_u, E_v of
+// the origin, the cells of the set's largest unflagged |u| and |v| at the
+// image's wavelength.  When plan rows 0 .. E_r and positions -E_c .. E_c (a
+// window, E < N/2) fit in LDS, the workgroup holds only those, walks every
+// baseline once, transforms the window's rows and stores them; the image's
+// row starts (row p holds entries when start[p + 1] != start[p]) tell the
+// second pass that rows above E_r are empty.  Otherwise (a window too large,
+// or several timesteps per channel, whose images share their row starts) it
+// takes blocks of whole rows, each walking every baseline, and stores every
+// row.  A cell outside the window (not expected: the bound is exact) sends
+// the image to the blocks.  `cap`: tile elements in LDS.
+template <int N, int NT, bool kMulti>
+__global__ void __launch_bounds__(NT)
+k_grid_rows_direct_win(SetDesc first, const SetDesc* __restrict__ sets,
+                       const int* __restrict__ img_set, int64_t gi0, float2* __restrict__ inter,
+                       const float2* __restrict__ tw, int cap) {
+    constexpr int H = N / 2;
+    constexpr int kMinStride = (padded_len(N) + 1) / 2;  // two rows hold the scratch
+    extern __shared__ float4 rows_win_lds4[];
+    float2* tile = (float2*)rows_win_lds4;  // [cap]
+    float2* twl = tile + cap;               // twiddle table copy [N]
+    __shared__ int red[NT / 64];
+    __shared__ float redm[2 * (NT / 64)];
+
+    const int li = (int)xcd_order(blockIdx.x, gridDim.x);
+    const int64_t gi = gi0 + li;
+    const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+    const int img = (int)(gi - sd.img_base);
+    const int ch = img % sd.g.n_ch;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < N; i += NT) twl[i] = tw[i];
+
+    // the window: largest |u|, |v| of the unflagged baselines, then their cells
+    float mu = 0.f, mv = 0.f;
+#pragma unroll 4
+    for (int b = tid; b < sd.g.n_bl; b += NT) {
+        const bool f = sd.flags[b] != 0;
+        const float au = fabsf(sd.u_m[b]), av = fabsf(sd.v_m[b]);
+        mu = f ? mu : fmaxf(mu, au);
+        mv = f ? mv : fmaxf(mv, av);
+    }
+    block_max2(mu, mv, redm);
+    double er, ec;
+    {
+#pragma clang fp contract(off)
+        const double wl = wavelength_of(sd.g.freq_hz[ch]);
+        const double eu = fabs(round(((double)mu / wl) / sd.du));
+        const double ev = fabs(round(((double)mv / wl) / sd.dv));
+        er = sd.uv_swap ? eu : ev;
+        ec = sd.uv_swap ? ev : eu;
+    }
+    bool win = sd.n_img == sd.g.n_ch && er < H && ec < H &&
+               (((int)er + 2) & ~1) * (int64_t)max(2 * (int)ec + 1, kMinStride) <= cap;
+    int n_ok = 0, n_rows = H + 1;
+    if (win) {
+        const int e_c = (int)ec, n_win = (int)er + 1, s = max(2 * e_c + 1, kMinStride);
+        const int c_shift = N - 2 * e_c - 1;  // positions N - e_c .. N - 1 -> e_c + 1 ..
+        for (int i = tid; i < ((n_win + 1) & ~1) * s; i += NT) tile[i] = make_float2(0.f, 0.f);
+        __syncthreads();
+        bool outside = false;
+        n_ok = direct_walk<2, true>(
+            sd, img, ch, N, 0, n_win, tid, NT,
+            [&](int r, int c) -> float2* {
+                const int j = c <= e_c ? c : (c >= N - e_c ? c - c_shift : -1);
+                return j >= 0 ? &tile[r * s + j] : nullptr;
+            },
+            &outside);
+        win = !__syncthreads_or(outside ? 1 : 0);
+        if (win) {
+            win_transform_rows<N, NT, false>(tile, 0, n_win, s, e_c, li, inter, twl,
+                                             sd.cached_inter != 0);
+            n_rows = n_win;
+        }
+    }
+    if (!win) n_ok = win_blocks<N, NT>(sd, img, ch, li, tile, cap, inter, twl);
+    const int tot = block_sum_int(n_ok, red);
+    if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
+    // row starts of the image's channel: rows 0 .. n_rows - 1 hold entries
+    int* rsg = sd.row_start + (int64_t)ch * (N + 1);
+    for (int i = tid; i <= N; i += NT) rsg[i] = min(i, n_rows);
+}
+
 // Set-up of the direct row pass, per set of the group: the count sums start
 // at zero, and the phase coefficients of the baselines are written once (the
 // plan's count pass does both for the planned sets).
@@ -813,8 +982,10 @@ static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
     X(1024, 8) X(1024, 16) X(1024, 32) \
     X(2048, 4) X(2048, 8) X(2048, 16)
 
-// Register sides of the direct row pass: (side, threads).
+// Register sides of the direct row pass: (side, threads), and the side of
+// the windowed direct row pass (k_grid_rows_direct_win) with its threads.
 #define BLINK_DIRECT_VARIANTS(X) X(128, 512)
+constexpr int kDirectWinSide = 256, kDirectWinThreads = 1024;
 
 bool batch_reg_supported(int n) {
     switch (n) {
@@ -916,13 +1087,35 @@ bool batch_direct_supported(int n) {
     if (n == NN) return true;
     BLINK_DIRECT_VARIANTS(X)
 #undef X
-    return false;
+    return n == kDirectWinSide;
+}
+
+bool direct_writes_row_starts(int n) { return n == kDirectWinSide; }
+
+template <int N, int NT>
+static hipError_t launch_rows_direct_win_t(const SetDesc& first, const SetDesc* sets,
+                                           const int* img_set, int n_sets, int64_t gi0,
+                                           int n_img, float* inter, const float* tw,
+                                           hipStream_t stream) {
+    // the tile takes the LDS the twiddle copy leaves; blocks of whole rows
+    // need at least two rows
+    const int64_t cap = g_lds_limit / (int64_t)sizeof(float2) - N;
+    if (((cap / N) & ~1) < 2) return hipErrorInvalidConfiguration;
+    const size_t lds = sizeof(float2) * (size_t)(cap + N);
+    auto k = n_sets == 1 ? k_grid_rows_direct_win<N, NT, false>
+                         : k_grid_rows_direct_win<N, NT, true>;
+    hipLaunchKernelGGL(k, dim3((unsigned)n_img), dim3(NT), lds, stream, first, sets, img_set, gi0,
+                       (float2*)inter, (const float2*)tw, (int)cap);
+    return hipGetLastError();
 }
 
 hipError_t launch_grid_rows_direct(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                    int n_sets, int n, int64_t gi0, int n_img, float* inter,
                                    const float* tw, hipStream_t stream) {
     if (n_img == 0) return hipSuccess;
+    if (n == kDirectWinSide)
+        return launch_rows_direct_win_t<kDirectWinSide, kDirectWinThreads>(
+            first, sets, img_set, n_sets, gi0, n_img, inter, tw, stream);
 #define X(NN, T) \
     if (n == NN) \
         return launch_rows_direct_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, tw, \
@@ -962,6 +1155,8 @@ hipError_t set_batch_kernel_attributes() {
     set((const void*)k_grid_rows_direct<NN, T, true>);
     BLINK_DIRECT_VARIANTS(X)
 #undef X
+    set((const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, false>);
+    set((const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, true>);
 #define X(NN, T)                                  \
     set((const void*)k_cols_reg<NN, T, false>);   \
     set((const void*)k_cols_reg<NN, T, true>);

@@ -992,7 +992,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         const bool beside_main = bp.groups.size() > 1 && &g - bp.groups.data() != bp.main;
         if (bp.direct_mode < 0 && 2 * g.n_img < cus && !beside_main) continue;
         // (the DFT transforms' half plane too: the same walk, no row FFT)
-        if (g.uniform || g.kb_size || bp.direct_mode == 0 || !direct_row_pass_fits(g.n)) continue;
+        if (g.uniform || g.kb_size || bp.direct_mode == 0) continue;
+        if (is_dft(g.transform) ? !direct_half_plane_fits(g.n) : !direct_row_pass_fits(g.n))
+            continue;
         bool ok = true;
         int work = 0;
         for (int i : g.sets) {
@@ -1257,7 +1259,11 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.uniform = st.weighting == BLINK_WEIGHTING_UNIFORM ? 1 : 0;
             d.kb_size = st.kernel_size;
             d.kb_table = bp.sc[i].kb_table;
-            d.all_rows = st.kernel_size > 0 || g.direct ? 1 : 0;
+            // (the windowed direct row pass writes the row starts instead)
+            d.all_rows = st.kernel_size > 0 ||
+                                 (g.direct && (is_dft(g.transform) || !direct_writes_row_starts(g.n)))
+                             ? 1
+                             : 0;
             d.cached_inter = g.cached_inter ? 1 : 0;
             base += d.n_img;
             desc_of[i] = (int)desc.size();

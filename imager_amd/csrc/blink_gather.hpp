@@ -155,6 +155,10 @@ __device__ __forceinline__ bool plan_cell(float u_m, float v_m, const CellUnit& 
 // fall in the plan rows r0 .. r0 + nr - 1, through at(row - r0, position),
 // an LDS float2.  Returns the count the plan path gives: entries in the
 // self-conjugate rows (0, and N/2 for even N) count once, others twice.
+// kWin: the rows and at() cover a window of the plane that every cell is
+// expected to fall in; at() returns nullptr for a position outside it, and a
+// cell outside the window's rows or positions sets `outside` (the caller then
+// discards the walk).
 template <int U>
 struct WalkBatch {
     int fl[U];
@@ -184,9 +188,10 @@ struct WalkBatch {
     }
 };
 
-template <int U, class At>
+template <int U, bool kWin = false, class At>
 __device__ __forceinline__ int direct_walk(const SetDesc& sd, int64_t img, int ch, int n, int r0,
-                                           int nr, int tid, int nt, At at) {
+                                           int nr, int tid, int nt, At at,
+                                           bool* outside = nullptr) {
     const GatherArgs& g = sd.g;
     const double nu = g.freq_hz[ch];
     const CellUnit cu(nu, sd);
@@ -211,18 +216,31 @@ __device__ __forceinline__ int direct_walk(const SetDesc& sd, int64_t img, int c
             const int r2 = r == 0 ? 0 : n - r;
             const bool in1 = (unsigned)(r - r0) < (unsigned)nr;
             const bool in2 = (unsigned)(r2 - r0) < (unsigned)nr;
-            if (!in1 && !in2) continue;
+            if (!in1 && !in2) {
+                if constexpr (kWin) *outside = true;
+                continue;
+            }
             float2 v = select_pol2(cur.x0[k], cur.x1[k], g.pol);
             if (g.corrections) v = cmul(v, unit_phasor(((cur.cf[k] * nu) / kVelLight)));
             if (!vis_ok(v)) continue;
             if (in1) {
                 float2* d = at(r - r0, c);
+                if constexpr (kWin)
+                    if (!d) {
+                        *outside = true;
+                        continue;
+                    }
                 atomicAdd(&d->x, v.x);
                 atomicAdd(&d->y, v.y);
                 n_ok += (r == 0 || 2 * r == n) ? 1 : 2;
             }
             if (in2) {
                 float2* d = at(r2 - r0, c == 0 ? 0 : n - c);
+                if constexpr (kWin)
+                    if (!d) {
+                        *outside = true;
+                        continue;
+                    }
                 atomicAdd(&d->x, v.x);
                 atomicAdd(&d->y, -v.y);
                 n_ok += (r2 == 0 || 2 * r2 == n) ? 1 : 2;

@@ -1532,6 +1532,10 @@ k_grid_rows_direct_fft(SetDesc first, const SetDesc* __restrict__ sets,
     }
 }
 
+bool direct_half_plane_fits(int n) {
+    return n % 2 == 0 && n <= kMaxPixels && lds_fits(sizeof(float2) * (size_t)(n / 2 + 1) * n);
+}
+
 bool direct_row_pass_fits(int n) {
     if (batch_reg_supported(n)) return batch_direct_supported(n);
     return n >= 2 && n <= kMaxPixels && lds_fits(direct_fft_lds_bytes(n));

@@ -251,15 +251,20 @@ hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const
                                 int n_sets, int n, int tile, int64_t gi0, int n_img, float* inter,
                                 const float* tw, hipStream_t stream, bool uniform = false);
 int batch_uniform_row_tile(int n);
-// Direct row pass of dense single-timestep sets (no plan; every row of the
-// half plane written, so the sets' descriptors carry all_rows): one
-// workgroup per image with its whole half plane in LDS, for the register
-// sides batch_direct_supported (128) and the generic sides that fit (up to
-// 186); direct_row_pass_fits tells which.  The count sums and phase
-// coefficients come from launch_direct_prep over the group's sets (max_work:
-// the largest n_img or n_bl among them), launched before.
+// Direct row pass of dense single-timestep sets (no plan): one workgroup per
+// image with its whole half plane in LDS, for the register sides
+// batch_direct_supported (128, and 256 with a window of the plane, below)
+// and the generic sides that fit (up to 186); direct_row_pass_fits tells
+// which.  Every row of the half plane is written, so the sets' descriptors
+// carry all_rows, except at the sides direct_writes_row_starts names (256),
+// whose kernel stores the rows of the window that holds the image's cells
+// and writes the image's row starts (rows above the window empty) for the
+// second pass.  The count sums and phase coefficients come from
+// launch_direct_prep over the group's sets (max_work: the largest n_img or
+// n_bl among them), launched before.
 bool batch_direct_supported(int n);
 bool direct_row_pass_fits(int n);
+bool direct_writes_row_starts(int n);
 hipError_t launch_grid_rows_direct(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                    int n_sets, int n, int64_t gi0, int n_img, float* inter,
                                    const float* tw, hipStream_t stream);
@@ -269,7 +274,9 @@ hipError_t launch_grid_rows_direct_fft(const SetDesc& first, const SetDesc* sets
                                        hipStream_t stream);
 hipError_t launch_direct_prep(const SetDesc* sets, int n_sets, int max_work, hipStream_t stream);
 // The DFT transform's half plane by the direct row pass (rows v, sd.uv_swap
-// 0), for the images img0 .. img0 + n_img - 1 of one set.
+// 0), for the images img0 .. img0 + n_img - 1 of one set, at the even sides
+// whose half plane fits in LDS (direct_half_plane_fits).
+bool direct_half_plane_fits(int n);
 hipError_t launch_grid_half_direct(const SetDesc& sd, int64_t n_img, int64_t img0, float* half,
                                    hipStream_t stream);
 hipError_t launch_cols_reg(const SetDesc& first, const SetDesc* sets, const int* img_set,

@@ -3,7 +3,10 @@ has a plan of its own) at sides whose half plane fits one workgroup's LDS
 (128 on the register kernels, generic sides up to 186, odd ones included)
 are gridded without a plan, one workgroup per image walking all baselines of
 its image and computing their cells itself (k_grid_rows_direct,
-k_grid_rows_direct_fft).
+k_grid_rows_direct_fft).  At 256^2 the workgroup holds the window of the
+half plane that the image's cells fall in (k_grid_rows_direct_win), or,
+when that window does not fit or a channel has several timesteps, blocks of
+whole rows that each walk all baselines.
 
 Against the oracle's reference-precision pipeline (peak-normalised error below
 1e-5) and the oracle's counts (exact): the 256-antenna station at 128^2,
@@ -100,7 +103,7 @@ def _buffers(o):
 
 
 @pytest.mark.parametrize("n,pol", [(128, "I"), (128, "YY"), (180, "XX"), (175, "V"),
-                                   (90, "XX")])
+                                   (90, "XX"), (256, "XX"), (256, "V")])
 def test_direct_sides(ctx, n, pol):
     o = _station(n, seed=500 + n)
     assert o.n_bl > 64 * (n // 2 + 1), "a dense grid"
@@ -176,7 +179,7 @@ def test_direct_dft_half_plane(ctx, n, transform):
     assert oracle.peak_normalised_error(out["1"], out["0"])[0] < IMAGE_TOL
 
 
-@pytest.mark.parametrize("n", [128, 180])
+@pytest.mark.parametrize("n", [128, 180, 256])
 def test_direct_chosen_for_many_images(ctx, n):
     """Without BLINK_DIRECT a dense single-timestep set with images for at
     least half the CUs takes the direct path by itself, and a call of three
@@ -205,7 +208,7 @@ def test_direct_chosen_for_many_images(ctx, n):
     assert np.array_equal(_host(sums), c)
 
 
-@pytest.mark.parametrize("n", [128, 180])
+@pytest.mark.parametrize("n", [128, 180, 256])
 def test_direct_average_without_images(ctx, n):
     """A direct set imaged for its mean only (no image buffer): the mean from
     the row-pass buffers against the oracle's, and the count sums."""
@@ -226,11 +229,11 @@ def test_direct_average_without_images(ctx, n):
         assert np.array_equal(_host(sums), c)
 
 
-@pytest.mark.parametrize("n", [128, 180])
+@pytest.mark.parametrize("n", [128, 180, 256])
 def test_direct_several_chunks(ctx, n):
     """A direct group cut into row-pass chunks (BLINK_INTER_MIB=1: 15 images
-    per chunk at 128^2, 8 at 180^2, of 20), so that the launches past the
-    first chunk index their images from the chunk's offset."""
+    per chunk at 128^2, 8 at 180^2, 3 at 256^2, of 20), so that the launches
+    past the first chunk index their images from the chunk's offset."""
     o = synth.eda2_station(n_ch=20, n_t=1, n_pixels=n, seed=990 + n, height_sigma_m=1.0,
                            freq=synth.mwa_fine_channels(20))
     ref, c = _reference(o, "XX")
@@ -246,7 +249,7 @@ def test_direct_several_chunks(ctx, n):
             launches = ctx.profile()
     finally:
         ctx.profile_enable(False)
-    assert launches["grid_rows_fft"][0] == (2 if n == 128 else 3), launches
+    assert launches["grid_rows_fft"][0] == {128: 2, 180: 3, 256: 7}[n], launches
     assert launches["direct_prep"][0] == 1, launches
     err = oracle.peak_normalised_error(_host(imgs), ref)[0]
     _say(f"  vs FP32 oracle {err:.2e}")
@@ -254,10 +257,13 @@ def test_direct_several_chunks(ctx, n):
     assert np.array_equal(_host(sums), c)
 
 
-def test_direct_forced_sparse_two_timesteps(ctx):
-    """BLINK_DIRECT=1 puts a sparse 128^2 set with two timesteps per channel on
-    the direct path (each image walks the baselines itself)."""
-    o = synth.small(n_ant=24, n_ch=3, n_t=2, n_pixels=128, seed=31, height_sigma_m=1.0)
+@pytest.mark.parametrize("n", [128, 256])
+def test_direct_forced_sparse_two_timesteps(ctx, n):
+    """BLINK_DIRECT=1 puts a sparse set with two timesteps per channel on
+    the direct path (each image walks the baselines itself; at 256^2 in
+    blocks of whole rows, since the channel's images share their row
+    starts)."""
+    o = synth.small(n_ant=24, n_ch=3, n_t=2, n_pixels=n, seed=31, height_sigma_m=1.0)
     ref, c = _reference(o, "XX")
     imgs, sums = _buffers(o)
     st, keep = _set(o, "XX", imgs, sums)
@@ -276,9 +282,9 @@ def test_direct_forced_sparse_two_timesteps(ctx):
     ctx.profile_enable(False)
 
 
-@pytest.mark.parametrize("n", [128, 180])
+@pytest.mark.parametrize("n", [128, 180, 256])
 def test_direct_group_average_cut_and_plan_path(ctx, n):
-    """Two sets in one group (the image -> set table) at 128^2 and at 180^2,
+    """Two sets in one group (the image -> set table) at 128^2, 180^2 and 256^2,
     the second with a uv-distance cut (the cell test's min_uv branch) and an
     average; then the same call on the plan path: identical count sums,
     images within 1e-6."""
@@ -319,3 +325,70 @@ def test_direct_group_average_cut_and_plan_path(ctx, n):
     ctx.profile_enable(False)
     for x, y in zip(out["1"], out["0"]):
         assert oracle.peak_normalised_error(x, y)[0] < 1e-6
+
+
+def test_direct_window_and_blocks(ctx):
+    """256^2 images whose cells need more than the window LDS holds (channels
+    at 240 and 300 MHz: |cell| up to 102 and 128, the latter wrapping at N/2)
+    take blocks of whole rows, beside a set whose channels fit the window, in
+    one call (the image -> set table): images within 1e-5 of the oracle,
+    counts exact, and within 1e-6 of the plan path."""
+    import torch
+    a = _station(256, seed=611)
+    b = synth.eda2_station(n_ch=2, n_t=1, n_pixels=256, seed=612,
+                           freq=np.array([240e6, 300e6]), height_sigma_m=1.0,
+                           bl_flag_fraction=0.1)
+    ra, ca = _reference(a, "XX")
+    rb, cb = _reference(b, "XX")
+    ia, sa = _buffers(a)
+    ib, sb = _buffers(b)
+    sta, ka = _set(a, "XX", ia, sa)
+    stb, kb = _set(b, "XX", ib, sb)
+    out = {}
+    for mode in ("1", "0"):
+        with _env(BLINK_DIRECT=mode):
+            for rep in range(2):
+                ctx.profile_enable(rep == 1)
+                ctx.profile_reset()
+                for t in (ia, ib):
+                    t.fill_(float("nan"))
+                _say(f"window and blocks at 256^2, BLINK_DIRECT={mode}, rep {rep}")
+                ctx.image_batch([sta, stb])
+                ea = oracle.peak_normalised_error(_host(ia), ra)[0]
+                eb = oracle.peak_normalised_error(_host(ib), rb)[0]
+                _say(f"  window set {ea:.2e}, blocks set {eb:.2e}")
+                assert ea < IMAGE_TOL and eb < IMAGE_TOL
+                assert np.array_equal(_host(sa), ca) and np.array_equal(_host(sb), cb)
+                if rep == 1:
+                    assert ("direct_prep" in ctx.profile()) == (mode == "1")
+        out[mode] = (_host(ia).copy(), _host(ib).copy())
+    ctx.profile_enable(False)
+    for x, y in zip(out["1"], out["0"]):
+        assert oracle.peak_normalised_error(x, y)[0] < 1e-6
+
+
+def test_direct_dft_256_keeps_the_plan(ctx):
+    """The DFT transforms' direct half plane needs the whole half plane in
+    LDS, which 256^2 exceeds: forced direct, such a set stays on the plan
+    path, and its images match the oracle."""
+    from imager_amd import capi
+    o = _station(256, seed=620)
+    ref, c = _reference(o, "XX")
+    d = {k: _dev(getattr(o, k)) for k in ("vis", "u", "v", "w", "flags", "freq", "cable")}
+    imgs, sums = _buffers(o)
+    st = capi.make_set(d["vis"], o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
+                       d["freq"], d["cable"], capi.CORR_GEOMETRIC | capi.CORR_CABLE, o.delta_u,
+                       o.delta_v, o.min_uv, 256, "XX", imgs, sums,
+                       transform=capi.TRANSFORM_DFT_FP16X3)
+    ctx.profile_enable(True)
+    ctx.profile_reset()
+    try:
+        with _env(BLINK_DIRECT=1):
+            _say("DFT fp16x3 at 256^2, forced direct")
+            ctx.image_batch([st])
+            launches = ctx.profile()
+    finally:
+        ctx.profile_enable(False)
+    assert "direct_prep" not in launches and "plan" in launches, launches
+    assert oracle.peak_normalised_error(_host(imgs), ref)[0] < IMAGE_TOL
+    assert np.array_equal(_host(sums), c)
