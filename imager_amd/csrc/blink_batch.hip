@@ -338,44 +338,20 @@ __device__ __forceinline__ void win_transform_rows(float2* tile, int r0, int nr,
     }
 }
 
-// The windowed direct row pass's fallback: blocks of rb whole rows, each
-// walking every baseline; returns the thread's count.  Kept out of line, so
-// that the window's path keeps its registers.
-template <int N, int NT>
-__device__ __attribute__((noinline)) int win_blocks(const SetDesc& sd, int img, int ch, int li,
-                                                    float2* tile, int cap,
-                                                    float2* __restrict__ inter,
-                                                    const float2* twl) {
-    constexpr int H = N / 2;
-    const int tid = (int)threadIdx.x;
-    const int rb = (cap / N) & ~1;  // rows per block (the launch checks rb >= 2)
-    int n_ok = 0;
-    for (int r0 = 0; r0 <= H; r0 += rb) {
-        const int nr = min(rb, H + 1 - r0);
-        __syncthreads();  // the previous block's transforms are done with the tile
-        for (int i = tid; i < ((nr + 1) & ~1) * N; i += NT) tile[i] = make_float2(0.f, 0.f);
-        __syncthreads();
-        n_ok += direct_walk<2>(sd, img, ch, N, r0, nr, tid, NT,
-                               [&](int r, int c) { return &tile[r * N + c]; });
-        __syncthreads();
-        win_transform_rows<N, NT, true>(tile, r0, nr, N, 0, li, inter, twl, sd.cached_inter != 0);
-    }
-    return n_ok;
-}
-
 // Direct row pass at sides whose half plane does not fit in LDS (N = 256).
 // A baseline's cell is round((x / wavelength) / cell size) per coordinate,
 // and |cell| grows with |x|, so the cells of an image lie within E_u, E_v of
 // the origin, the cells of the set's largest unflagged |u| and |v| at the
-// image's wavelength.  When plan rows 0 .. E_r and positions -E_c .. E_c (a
-// window, E < N/2) fit in LDS, the workgroup holds only those, walks every
-// baseline once, transforms the window's rows and stores them; the image's
-// row starts (row p holds entries when start[p + 1] != start[p]) tell the
-// second pass that rows above E_r are empty.  Otherwise (a window too large,
-// or several timesteps per channel, whose images share their row starts) it
-// takes blocks of whole rows, each walking every baseline, and stores every
-// row.  A cell outside the window (not expected: the bound is exact) sends
-// the image to the blocks.  `cap`: tile elements in LDS.
+// image's wavelength (partial maxima from the direct set-up launch).  When
+// plan rows 0 .. E_r and positions -E_c .. E_c (a window, E < N/2) fit in
+// LDS, the workgroup holds only those, walks every baseline once, transforms
+// the window's rows and stores them; the image's row starts (row p holds
+// entries when start[p + 1] != start[p]) tell the second pass that rows above
+// E_r are empty.  Otherwise (a window too large, or several timesteps per
+// channel, whose images share their row starts) the workgroup only sets
+// start[N] = -1, which the second pass never reads, and leaves the image to
+// k_grid_rows_direct_blocks, launched next.  A cell outside the window (not
+// expected: the bound is exact) does the same.  `cap`: tile elements in LDS.
 template <int N, int NT, bool kMulti>
 __global__ void __launch_bounds__(NT)
 k_grid_rows_direct_win(SetDesc first, const SetDesc* __restrict__ sets,
@@ -395,17 +371,11 @@ k_grid_rows_direct_win(SetDesc first, const SetDesc* __restrict__ sets,
     const int img = (int)(gi - sd.img_base);
     const int ch = img % sd.g.n_ch;
     const int tid = threadIdx.x;
-    for (int i = tid; i < N; i += NT) twl[i] = tw[i];
+    int* rsg = sd.row_start + (int64_t)ch * (N + 1);
 
     // the window: largest |u|, |v| of the unflagged baselines, then their cells
     float mu = 0.f, mv = 0.f;
-#pragma unroll 4
-    for (int b = tid; b < sd.g.n_bl; b += NT) {
-        const bool f = sd.flags[b] != 0;
-        const float au = fabsf(sd.u_m[b]), av = fabsf(sd.v_m[b]);
-        mu = f ? mu : fmaxf(mu, au);
-        mv = f ? mv : fmaxf(mv, av);
-    }
+    if (tid < kUvMaxParts) mu = sd.uv_max[2 * tid], mv = sd.uv_max[2 * tid + 1];
     block_max2(mu, mv, redm);
     double er, ec;
     {
@@ -416,49 +386,112 @@ k_grid_rows_direct_win(SetDesc first, const SetDesc* __restrict__ sets,
         er = sd.uv_swap ? eu : ev;
         ec = sd.uv_swap ? ev : eu;
     }
-    bool win = sd.n_img == sd.g.n_ch && er < H && ec < H &&
-               (((int)er + 2) & ~1) * (int64_t)max(2 * (int)ec + 1, kMinStride) <= cap;
-    int n_ok = 0, n_rows = H + 1;
-    if (win) {
-        const int e_c = (int)ec, n_win = (int)er + 1, s = max(2 * e_c + 1, kMinStride);
-        const int c_shift = N - 2 * e_c - 1;  // positions N - e_c .. N - 1 -> e_c + 1 ..
-        for (int i = tid; i < ((n_win + 1) & ~1) * s; i += NT) tile[i] = make_float2(0.f, 0.f);
-        __syncthreads();
-        bool outside = false;
-        n_ok = direct_walk<2, true>(
-            sd, img, ch, N, 0, n_win, tid, NT,
-            [&](int r, int c) -> float2* {
-                const int j = c <= e_c ? c : (c >= N - e_c ? c - c_shift : -1);
-                return j >= 0 ? &tile[r * s + j] : nullptr;
-            },
-            &outside);
-        win = !__syncthreads_or(outside ? 1 : 0);
-        if (win) {
-            win_transform_rows<N, NT, false>(tile, 0, n_win, s, e_c, li, inter, twl,
-                                             sd.cached_inter != 0);
-            n_rows = n_win;
-        }
+    const bool fits = sd.n_img == sd.g.n_ch && er < H && ec < H &&
+                      (((int)er + 2) & ~1) * (int64_t)max(2 * (int)ec + 1, kMinStride) <= cap;
+    if (!fits) {  // uniform
+        if (tid == 0) rsg[N] = -1;
+        return;
     }
-    if (!win) n_ok = win_blocks<N, NT>(sd, img, ch, li, tile, cap, inter, twl);
+    const int e_c = (int)ec, n_win = (int)er + 1, s = max(2 * e_c + 1, kMinStride);
+    const int c_shift = N - 2 * e_c - 1;  // positions N - e_c .. N - 1 -> e_c + 1 ..
+    for (int i = tid; i < N; i += NT) twl[i] = tw[i];
+    for (int i = tid; i < ((n_win + 1) & ~1) * s; i += NT) tile[i] = make_float2(0.f, 0.f);
+    __syncthreads();
+    bool outside = false;
+    const int n_ok = direct_walk<2, true>(
+        sd, img, ch, N, 0, n_win, tid, NT,
+        [&](int r, int c) -> float2* {
+            const int j = c <= e_c ? c : (c >= N - e_c ? c - c_shift : -1);
+            return j >= 0 ? &tile[r * s + j] : nullptr;
+        },
+        &outside);
+    if (__syncthreads_or(outside ? 1 : 0)) {
+        if (tid == 0) rsg[N] = -1;
+        return;
+    }
+    win_transform_rows<N, NT, false>(tile, 0, n_win, s, e_c, li, inter, twl,
+                                     sd.cached_inter != 0);
     const int tot = block_sum_int(n_ok, red);
     if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
-    // row starts of the image's channel: rows 0 .. n_rows - 1 hold entries
-    int* rsg = sd.row_start + (int64_t)ch * (N + 1);
-    for (int i = tid; i <= N; i += NT) rsg[i] = min(i, n_rows);
+    // row starts of the image's channel: rows 0 .. n_win - 1 hold entries
+    for (int i = tid; i <= N; i += NT) rsg[i] = min(i, n_win);
+}
+
+// The windowed direct row pass's other images (start[N] = -1): blocks of rb
+// whole rows, each walking every baseline, then every row stored and the
+// row starts of rows 0 .. N/2 set (start[N] stays -1, so that the other
+// images of a channel with several timesteps see it too).  Workgroups take
+// the chunk's images in turn and return at once for the windowed ones.
+// (NT = 512: the walk and two-row transforms in one loop fit its registers.)
+template <int N, int NT, bool kMulti>
+__global__ void __launch_bounds__(NT)
+k_grid_rows_direct_blocks(SetDesc first, const SetDesc* __restrict__ sets,
+                          const int* __restrict__ img_set, int64_t gi0, int n_img,
+                          float2* __restrict__ inter, const float2* __restrict__ tw, int cap) {
+    constexpr int H = N / 2;
+    extern __shared__ float4 rows_blk_lds4[];
+    float2* tile = (float2*)rows_blk_lds4;  // [cap]
+    float2* twl = tile + cap;               // twiddle table copy [N]
+    __shared__ int red[NT / 64];
+    const int tid = threadIdx.x;
+    const int rb = (cap / N) & ~1;  // rows per block (the launch checks rb >= 2)
+    bool have_tw = false;
+    for (int li = blockIdx.x; li < n_img; li += gridDim.x) {
+        const int64_t gi = gi0 + li;
+        const SetDesc sd = set_of<kMulti>(first, sets, img_set, gi);
+        const int img = (int)(gi - sd.img_base);
+        const int ch = img % sd.g.n_ch;
+        int* rsg = sd.row_start + (int64_t)ch * (N + 1);
+        if (rsg[N] >= 0) continue;  // windowed (uniform)
+        if (!have_tw) {
+            for (int i = tid; i < N; i += NT) twl[i] = tw[i];
+            have_tw = true;
+        }
+        int n_ok = 0;
+        for (int r0 = 0; r0 <= H; r0 += rb) {
+            const int nr = min(rb, H + 1 - r0);
+            __syncthreads();  // the previous block's transforms are done with the tile
+            for (int i = tid; i < ((nr + 1) & ~1) * N; i += NT) tile[i] = make_float2(0.f, 0.f);
+            __syncthreads();
+            n_ok += direct_walk<2>(sd, img, ch, N, r0, nr, tid, NT,
+                                   [&](int r, int c) { return &tile[r * N + c]; });
+            __syncthreads();
+            win_transform_rows<N, NT, true>(tile, r0, nr, N, 0, li, inter, twl,
+                                            sd.cached_inter != 0);
+        }
+        const int tot = block_sum_int(n_ok, red);
+        if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
+        for (int i = tid; i < N; i += NT) rsg[i] = min(i, H + 1);
+    }
 }
 
 // Set-up of the direct row pass, per set of the group: the count sums start
 // at zero, and the phase coefficients of the baselines are written once (the
-// plan's count pass does both for the planned sets).
+// plan's count pass does both for the planned sets); for the windowed direct
+// row pass, each block's largest |u| and |v| over the unflagged baselines it
+// visits (slot b of uv_max; slots past the grid hold zeros).
 __global__ void __launch_bounds__(256)
 k_direct_prep(const SetDesc* __restrict__ sets) {
+    __shared__ float redm[2 * (256 / 64)];
     const SetDesc& sd = sets[blockIdx.y];
     const int n = max(sd.n_img, sd.g.n_bl);
+    float mu = 0.f, mv = 0.f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < sd.n_img) sd.isums[i] = 0u;
         if (i < sd.g.n_bl && sd.g.corrections)
             sd.coef_out[i] = phase_coefficient(sd.g.corrections, sd.w_m, sd.cable_m, (unsigned)i);
+        if (sd.uv_max && i < sd.g.n_bl && !sd.flags[i]) {
+            mu = fmaxf(mu, fabsf(sd.u_m[i]));
+            mv = fmaxf(mv, fabsf(sd.v_m[i]));
+        }
     }
+    if (!sd.uv_max) return;  // uniform per block
+    block_max2(mu, mv, redm);
+    if (threadIdx.x == 0)
+        for (int j = blockIdx.x; j < kUvMaxParts; j += gridDim.x) {
+            sd.uv_max[2 * j] = j == (int)blockIdx.x ? mu : 0.f;
+            sd.uv_max[2 * j + 1] = j == (int)blockIdx.x ? mv : 0.f;
+        }
 }
 
 // Second pass.  The gridded plane is Hermitian by construction (every
@@ -985,7 +1018,7 @@ static hipError_t launch_cols_reg_t(const SetDesc& first, const SetDesc* sets,
 // Register sides of the direct row pass: (side, threads), and the side of
 // the windowed direct row pass (k_grid_rows_direct_win) with its threads.
 #define BLINK_DIRECT_VARIANTS(X) X(128, 512)
-constexpr int kDirectWinSide = 256, kDirectWinThreads = 1024;
+constexpr int kDirectWinSide = 256, kDirectWinThreads = 1024, kDirectBlockThreads = 512;
 
 bool batch_reg_supported(int n) {
     switch (n) {
@@ -1106,6 +1139,15 @@ static hipError_t launch_rows_direct_win_t(const SetDesc& first, const SetDesc* 
                          : k_grid_rows_direct_win<N, NT, true>;
     hipLaunchKernelGGL(k, dim3((unsigned)n_img), dim3(NT), lds, stream, first, sets, img_set, gi0,
                        (float2*)inter, (const float2*)tw, (int)cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the images the window does not hold: a workgroup per CU takes them in turn
+    auto kb = n_sets == 1 ? k_grid_rows_direct_blocks<N, kDirectBlockThreads, false>
+                          : k_grid_rows_direct_blocks<N, kDirectBlockThreads, true>;
+    const int blocks = (int)std::min<int64_t>(n_img, std::max(1, device_cu_count()));
+    hipLaunchKernelGGL(kb, dim3((unsigned)blocks), dim3(kDirectBlockThreads), lds, stream, first,
+                       sets, img_set,
+                       gi0, n_img, (float2*)inter, (const float2*)tw, (int)cap);
     return hipGetLastError();
 }
 
@@ -1128,7 +1170,8 @@ hipError_t launch_grid_rows_direct(const SetDesc& first, const SetDesc* sets, co
 hipError_t launch_direct_prep(const SetDesc* sets, int n_sets, int max_work,
                               hipStream_t stream) {
     if (n_sets == 0 || max_work <= 0) return hipSuccess;
-    const unsigned blocks = (unsigned)std::min(64, (max_work + 255) / 256);
+    // (at most kUvMaxParts blocks: one partial maximum each)
+    const unsigned blocks = (unsigned)std::min(kUvMaxParts, (max_work + 255) / 256);
     hipLaunchKernelGGL(k_direct_prep, dim3(blocks, (unsigned)n_sets), dim3(256), 0, stream, sets);
     return hipGetLastError();
 }
@@ -1157,6 +1200,8 @@ hipError_t set_batch_kernel_attributes() {
 #undef X
     set((const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, false>);
     set((const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, true>);
+    set((const void*)k_grid_rows_direct_blocks<kDirectWinSide, kDirectBlockThreads, false>);
+    set((const void*)k_grid_rows_direct_blocks<kDirectWinSide, kDirectBlockThreads, true>);
 #define X(NN, T)                                  \
     set((const void*)k_cols_reg<NN, T, false>);   \
     set((const void*)k_cols_reg<NN, T, true>);

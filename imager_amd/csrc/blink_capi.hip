@@ -1265,6 +1265,11 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
                              ? 1
                              : 0;
             d.cached_inter = g.cached_inter ? 1 : 0;
+            // the windowed direct row pass's |u|, |v| maxima: in the row
+            // lists, which no direct set uses (n_ch (N + 3) >= 2 kUvMaxParts)
+            d.uv_max = g.direct && !is_dft(g.transform) && direct_writes_row_starts(g.n)
+                           ? (float*)bp.sc[i].row_list
+                           : nullptr;
             base += d.n_img;
             desc_of[i] = (int)desc.size();
             desc.push_back(d);
@@ -1400,7 +1405,12 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
     const SetDesc& first = bp.host_desc(g);
     const int* is = t.img_set(g);
     const int ns = (int)g.sets.size();
-    if (g.direct) HIP_TRY(launch_direct_prep(sd, ns, g.direct_work, s));
+    // direct groups: the one row pass runs once, and the column blocks are
+    // timed alone (the row pass's own variation would hide their difference)
+    if (g.direct) {
+        HIP_TRY(launch_direct_prep(sd, ns, g.direct_work, s));
+        HIP_TRY(launch_grid_rows_direct(first, sd, is, ns, n, 0, cnt, bp.inter, tw, s));
+    }
     // Every (row tile, column block) pair is timed as the pipeline runs it
     // (row pass then column pass), in interleaved rounds; each pair keeps its
     // fastest round, so that clock ramp-up and neighbour noise do not favour
@@ -1414,11 +1424,9 @@ static int autotune_group(blink_ctx* ctx, const BatchPlan& bp, const BatchGroup&
             for (int c = 0; c < nc; c++) {
                 float ms;
                 int rc = time_launches(ctx, s, 2, [&] {
-                    hipError_t e =
-                        g.direct ? launch_grid_rows_direct(first, sd, is, ns, n, 0, cnt, bp.inter,
-                                                           tw, s)
-                                 : launch_grid_rows_reg(first, sd, is, ns, n, rows[r], 0, cnt,
-                                                        bp.inter, tw, s);
+                    hipError_t e = g.direct ? hipSuccess
+                                            : launch_grid_rows_reg(first, sd, is, ns, n, rows[r],
+                                                                   0, cnt, bp.inter, tw, s);
                     if (e != hipSuccess) return e;
                     return launch_cols_reg(first, sd, is, ns, n, cols[c], 0, cnt, bp.inter, tw,
                                            s);

@@ -92,7 +92,14 @@ struct SetDesc {
     // stores instead of streaming ones, so that a chunk small enough for the
     // Infinity Cache can be read back from it by the second pass
     int cached_inter;
+    // windowed direct row pass (k_grid_rows_direct_win): [kUvMaxParts][2]
+    // partial maxima of |u| and |v| over the unflagged baselines, written by
+    // the direct set-up launch (in the set's otherwise unused row lists);
+    // nullptr for other sets
+    float* uv_max;
 };
+
+constexpr int kUvMaxParts = 64;
 
 // A packed plan cell that is not gridded (flagged, NaN or out of range).
 constexpr uint32_t kNoCell = 0xffffffffu;
