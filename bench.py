@@ -122,6 +122,10 @@ def compacted(n_bl: int, n_pix: int) -> bool:
             and os.environ.get("BLINK_COMPACT", "1") != "0")
 
 
+# the grid side whose direct row pass holds a window of the half plane
+DIRECT_WINDOW_SIDE = 256
+
+
 def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
                    kernel_size: int = 0, direct: bool = False) -> float | None:
     """HBM bytes a fused kernel must move per launch over n_img images of
@@ -133,7 +137,10 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
                      transformed non-empty plan rows (8 N B each); on the
                      direct path (no plan): u, v, flags once (12 B per
                      baseline) + the records once + every plan row 0 .. N/2
+                     (at 256^2, the window's rows: those up to the last that
+                     holds cells, which the second pass then reads)
       direct_prep    w and the phase coefficient per baseline, the count sums
+                     (at 256^2 also u, v, flags for the |u|, |v| maxima)
       fft_cols       the non-empty plan rows (8 N B each) + the image (8 N^2 B)
       vis_compact    the records (32 B) + the compacted value (8 B) per baseline
       plan           per channel: u, v, flags (12 B) read, the packed cell (4 B)
@@ -143,9 +150,10 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
       dft_cols       the stage-1 product + the image"""
     n, n_bl = o.n_pixels, o.n_bl
     entries, nonempty = stats
-    if kernel_size or direct:
+    window = direct and n == DIRECT_WINDOW_SIDE
+    if kernel_size or (direct and not window):
         # convolutional gridding and the direct row pass write and read every
-        # plan row 0 .. N/2
+        # plan row 0 .. N/2 (the windowed one, the rows of its window)
         nonempty = np.full_like(nonempty, n // 2 + 1)
     ch = np.arange(n_img) % o.n_ch
     e_sum = float(entries[ch].sum())
@@ -155,7 +163,7 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
     if kernel == "grid_rows_fft" and direct:
         return 12.0 * n_bl + 32.0 * n_bl * n_img + 8.0 * n * r_sum
     if kernel == "direct_prep":
-        return 12.0 * n_bl + 4.0 * n_img
+        return (24.0 if window else 12.0) * n_bl + 4.0 * n_img
     if kernel == "grid_rows_fft":
         vis_b = 8.0 if compacted(n_bl, n) else 32.0
         return 4.0 * e_sum + vis_b * n_bl * n_img + 8.0 * n * r_sum

@@ -350,9 +350,10 @@ BLINK_API int blink_profile_reset(blink_ctx* ctx);
  * environment; the last tuned pair is reported), negative for the defaults
  * not yet tuned, 0 when the side uses the generic kernels.  Groups on the
  * direct row pass (no plan: sets with one timestep per channel and more than
- * 64 baselines per grid row, at 128^2 and at generic sides up to 186) have
- * no row tile; at 128^2 their column block is tuned apart from these pairs
- * and not reported here, and a pinned pair pins only the column block. */
+ * 64 baselines per grid row, at 128^2, 256^2 and generic sides up to 186)
+ * have no row tile; at 128^2 and 256^2 their column block is tuned apart
+ * from these pairs and not reported here, and a pinned pair pins only the
+ * column block. */
 BLINK_API int blink_batch_tiles(blink_ctx* ctx, int n_pixels, int* row_tile, int* col_tile);
 /* Pins the tile sizes of the batched kernels for grid side n_pixels (row_tile
  * lines per first-pass tile, col_tile output lines per second-pass block, as
