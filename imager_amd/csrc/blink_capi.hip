@@ -32,6 +32,7 @@ struct DeviceBuffer {
 };
 
 struct GraphEntry {
+    std::string call;  // the call's key without its buffer addresses (graph_key)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void* blob = nullptr;  // device copy of the call's descriptor tables
@@ -52,6 +53,11 @@ void release_graph(GraphEntry& ge) {
 // Captured graphs kept per context; the least recently used is dropped
 // beyond this.
 constexpr size_t kMaxGraphs = 16;
+// Graphs kept per call (the same set fields) with different buffer addresses,
+// so that a caller alternating between two output buffers replays both
+// without updating either; a further address set re-points the least
+// recently used of them.
+constexpr int kBufferVariants = 2;
 
 }  // namespace
 
@@ -1726,7 +1732,8 @@ static int autotune_batch(blink_ctx* ctx, const BatchPlan& bp, const BatchTables
 // struct padding never enters it) except the visibility, image, count-sum and
 // average addresses (their presence is part of the key), plus the environment
 // switches that shape the call.  Calls that differ only in those addresses
-// replay one instantiated graph.
+// share up to kBufferVariants instantiated graphs (the cache's full key adds
+// the addresses).
 static std::string graph_key(const blink_image_set* sets, int n_sets, const BatchPlan& bp) {
     std::string k;
     auto put = [&](const auto& v) { k.append((const char*)&v, sizeof v); };
@@ -1870,11 +1877,31 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
         return enqueue_batch(ctx, sets, bp, t, s);
     }
 
-    const std::string key = graph_key(sets, n_sets, bp);
+    // cache key: the call's fields, then its buffer addresses
+    const std::string call = graph_key(sets, n_sets, bp);
+    const std::vector<const void*> ptrs = io_pointers(sets, n_sets);
+    std::string key = call;
+    key.append((const char*)ptrs.data(), sizeof(const void*) * ptrs.size());
     auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+        // the same call with other buffers: a new graph while fewer than
+        // kBufferVariants exist, else the least recently used one re-pointed
+        auto lru = ctx->graphs.end();
+        int variants = 0;
+        for (auto g = ctx->graphs.begin(); g != ctx->graphs.end(); ++g)
+            if (g->second.call == call) {
+                variants++;
+                if (lru == ctx->graphs.end() || g->second.last_use < lru->second.last_use) lru = g;
+            }
+        if (variants >= kBufferVariants) {
+            GraphEntry moved = lru->second;
+            ctx->graphs.erase(lru);
+            it = ctx->graphs.emplace(key, moved).first;
+        }
+    }
     if (it != ctx->graphs.end()) {
         GraphEntry& ge = it->second;
-        if (ge.ptrs != io_pointers(sets, n_sets)) {
+        if (ge.ptrs != ptrs) {
             // same call with other visibility / image / count-sum buffers:
             // refresh the descriptor blob in stream order, capture the call
             // again and update the instantiated graph's kernel arguments
@@ -1909,10 +1936,11 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
             }
             (void)hipGraphDestroy(ge.graph);
             ge.graph = graph;
-            ge.ptrs = io_pointers(sets, n_sets);
+            ge.ptrs = ptrs;
         }
     } else {
         GraphEntry ge;
+        ge.call = call;
         HIP_TRY(hipEventCreateWithFlags(&ge.done, hipEventDisableTiming));
         hipError_t e = hipMalloc(&ge.blob, bp.blob.size());
         if (e == hipSuccess)
@@ -1931,7 +1959,7 @@ BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int
             release_graph(ge);
             return ctx->fail(BLINK_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         }
-        ge.ptrs = io_pointers(sets, n_sets);
+        ge.ptrs = ptrs;
         if (ctx->graphs.size() >= kMaxGraphs) {
             auto lru = ctx->graphs.begin();
             for (auto g = ctx->graphs.begin(); g != ctx->graphs.end(); ++g)

@@ -282,10 +282,13 @@ typedef struct blink_image_set {
  * every set, with the phase corrections, gridding, weighting, FFT, shift and
  * normalisation fused into a few kernels per set, the whole call captured once as a hipGraph and
  * replayed when it is made again with the same set fields (compared field
- * by field).  A call that differs only in the vis, images, count_sums or
- * average addresses refreshes the captured descriptors and updates the instantiated
- * graph (hipGraphExecUpdate) instead of instantiating a new one; it first
- * waits for that graph's previous replay.  Results equal
+ * by field).  Calls that differ only in the vis, images, count_sums or
+ * average addresses keep a graph per address set, up to two (a caller that
+ * alternates between two output buffers replays both without waiting);
+ * beyond that a call refreshes the captured descriptors of the least recently
+ * used of them and updates that instantiated graph (hipGraphExecUpdate)
+ * instead of instantiating a new one, after waiting for its previous replay.
+ * Results equal
  * apply_*_corrections + gridding + image of the reference (CPacerImager::run,
  * src/pacer_imager.cpp:440-526, with average_images). */
 BLINK_API int blink_image_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets,

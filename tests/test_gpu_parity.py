@@ -517,9 +517,10 @@ def test_image_batch_graph_replays_are_identical(ctx):
 @pytest.mark.parametrize("n,dft", [(1024, False), (180, False), (128, True)])
 def test_image_batch_new_buffers_update_the_graph(ctx, n, dft):
     """A call repeated with other vis / images / count_sums buffers (the
-    fields that do not key the graph cache) updates the captured graph: every
+    fields that do not key the graph cache): two buffer sets keep a graph
+    each, a third re-points (updates) the least recently used one.  Every
     call's outputs land in that call's buffers and match the oracle, and
-    going back to the first buffers gives the first images again."""
+    going back to earlier buffers gives their images again."""
     import torch
     from imager_amd import capi
     o = synth.small(n_ant=48, n_ch=2, n_t=2, n_pixels=n, seed=300 + n, height_sigma_m=1.0)
@@ -530,14 +531,15 @@ def test_image_batch_new_buffers_update_the_graph(ctx, n, dft):
     vis2 = _dev(o2.vis)
     tr = capi.TRANSFORM_DFT_BF16X3 if dft else capi.TRANSFORM_FFT
     bufs = [(d["vis"], torch.empty((o.n_t, o.n_ch, n, n), dtype=torch.complex64, device="cuda"),
-             torch.empty(o.n_images, dtype=torch.float64, device="cuda")) for _ in range(2)]
+             torch.empty(o.n_images, dtype=torch.float64, device="cuda")) for _ in range(3)]
     bufs[1] = (vis2, bufs[1][1], bufs[1][2])
     refs = []
     for obs in (o, o2):
         r, _ = oracle.run_f32(obs.vis, obs.n_ant, obs.u, obs.v, obs.w, obs.flags, obs.freq,
                               obs.cable, True, True, obs.delta_u, obs.delta_v, obs.min_uv, n)
         refs.append(r)
-    for k in (0, 1, 0, 1):
+    refs.append(refs[0])   # buffer set 2: the first visibilities, other outputs
+    for k in (0, 1, 0, 1, 2, 1, 0, 2):
         vis, imgs, sums = bufs[k]
         imgs.fill_(float("nan"))
         st = capi.make_set(vis, o.n_ant, o.n_ch, o.n_t, d["u"], d["v"], d["w"], d["flags"],
