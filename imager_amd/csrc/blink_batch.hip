@@ -230,8 +230,10 @@ k_grid_rows_reg(SetDesc first, const SetDesc* __restrict__ sets, const int* __re
 // baseline of the image in order (direct_walk: flags, u, v and the records
 // coalesced, each record line read once), computes the cells as the plan's
 // count pass does and accumulates each visibility and its conjugate; then it
-// transforms and stores every row (rows without visibilities as zeros: the
-// second pass reads every row).  Built where the half plane fits in LDS with
+// transforms and stores rows 0 .. the highest row holding a cell, and writes
+// the image's row starts so that the second pass reads the rows above as
+// empty (the cells lie within |cell| <= round(N / (osf wavelength)), about
+// 0.3 N at osf 2).  Built where the half plane fits in LDS with
 // room for a second workgroup per CU (N = 128); tiles of part of the rows
 // would each walk all baselines, and measured slower than the plan path
 // (768 channels x 256^2, 32896 baselines: two tiles per image 0.70 against
@@ -257,15 +259,18 @@ k_grid_rows_direct(SetDesc first, const SetDesc* __restrict__ sets,
     for (int i = tid; i < N; i += NT) twl[i] = tw[i];
     for (int i = tid; i < (H + 1) * TS; i += NT) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    const int n_ok = direct_walk<2>(sd, img, ch, N, 0, H + 1, tid, NT,
-                                    [&](int r, int c) { return &tile[r * TS + pad_index(c)]; });
+    int top = -1;
+    const int n_ok = direct_walk<2>(
+        sd, img, ch, N, 0, H + 1, tid, NT,
+        [&](int r, int c) { return &tile[r * TS + pad_index(c)]; }, nullptr, &top);
+    const int n_rows = block_max_int(top, red) + 1;
     const int tot = block_sum_int(n_ok, red);
     if (tid == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();
 
-    // every row, one team of L lanes per row
+    // rows 0 .. n_rows - 1, one team of L lanes per row
     const int team = tid / L, lane = tid % L;
-    for (int r = team; r <= H; r += TEAMS) {
+    for (int r = team; r < n_rows; r += TEAMS) {
         float2* row = tile + r * TS;
         float2 v[F::V];
         F::load(v, lane, [&](int i) { return row[pad_index(i)]; });
@@ -276,6 +281,8 @@ k_grid_rows_direct(SetDesc first, const SetDesc* __restrict__ sets,
         else
             F::store(v, lane, [&](int i, float2 x) { store_stream(&inter[inter_index(N, li, r, i)], x); });
     }
+    int* rsg = sd.row_start + (int64_t)ch * (N + 1);
+    for (int i = tid; i <= N; i += NT) rsg[i] = min(i, n_rows);
 }
 
 // Largest of a and b over the workgroup (every thread gets both); NaN
@@ -1123,7 +1130,7 @@ bool batch_direct_supported(int n) {
     return n == kDirectWinSide;
 }
 
-bool direct_writes_row_starts(int n) { return n == kDirectWinSide; }
+bool direct_uses_window(int n) { return n == kDirectWinSide; }
 
 template <int N, int NT>
 static hipError_t launch_rows_direct_win_t(const SetDesc& first, const SetDesc* sets,

@@ -1265,15 +1265,12 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.uniform = st.weighting == BLINK_WEIGHTING_UNIFORM ? 1 : 0;
             d.kb_size = st.kernel_size;
             d.kb_table = bp.sc[i].kb_table;
-            // (the windowed direct row pass writes the row starts instead)
-            d.all_rows = st.kernel_size > 0 ||
-                                 (g.direct && (is_dft(g.transform) || !direct_writes_row_starts(g.n)))
-                             ? 1
-                             : 0;
+            // (the FFT groups' direct row passes write the row starts instead)
+            d.all_rows = st.kernel_size > 0 || (g.direct && is_dft(g.transform)) ? 1 : 0;
             d.cached_inter = g.cached_inter ? 1 : 0;
             // the windowed direct row pass's |u|, |v| maxima: in the row
             // lists, which no direct set uses (n_ch (N + 3) >= 2 kUvMaxParts)
-            d.uv_max = g.direct && !is_dft(g.transform) && direct_writes_row_starts(g.n)
+            d.uv_max = g.direct && !is_dft(g.transform) && direct_uses_window(g.n)
                            ? (float*)bp.sc[i].row_list
                            : nullptr;
             base += d.n_img;

@@ -158,7 +158,9 @@ __device__ __forceinline__ bool plan_cell(float u_m, float v_m, const CellUnit& 
 // kWin: the rows and at() cover a window of the plane that every cell is
 // expected to fall in; at() returns nullptr for a position outside it, and a
 // cell outside the window's rows or positions sets `outside` (the caller then
-// discards the walk).
+// discards the walk).  `top`, when given, is raised to the highest row (from
+// r0) that holds a cell of the thread's gridded baselines (whatever their
+// values: the same rows for every image of a channel).
 template <int U>
 struct WalkBatch {
     int fl[U];
@@ -191,7 +193,7 @@ struct WalkBatch {
 template <int U, bool kWin = false, class At>
 __device__ __forceinline__ int direct_walk(const SetDesc& sd, int64_t img, int ch, int n, int r0,
                                            int nr, int tid, int nt, At at,
-                                           bool* outside = nullptr) {
+                                           bool* outside = nullptr, int* top = nullptr) {
     const GatherArgs& g = sd.g;
     const double nu = g.freq_hz[ch];
     const CellUnit cu(nu, sd);
@@ -220,6 +222,9 @@ __device__ __forceinline__ int direct_walk(const SetDesc& sd, int64_t img, int c
                 if constexpr (kWin) *outside = true;
                 continue;
             }
+            // (before the value test: the rows depend on the cells only, the
+            // same for every image of a channel)
+            if (top) *top = max(*top, max(in1 ? r - r0 : -1, in2 ? r2 - r0 : -1));
             float2 v = select_pol2(cur.x0[k], cur.x1[k], g.pol);
             if (g.corrections) v = cmul(v, unit_phasor(((cur.cf[k] * nu) / kVelLight)));
             if (!vis_ok(v)) continue;
@@ -285,6 +290,18 @@ __device__ __forceinline__ int gather_range(const GatherArgs& g, int64_t img, in
 }
 
 // Block-wide sum of one int per thread; the result is valid in thread 0.
+// Largest x over the workgroup, returned to every thread (scratch: one int
+// per wavefront).
+__device__ __forceinline__ int block_max_int(int x, int* scratch) {
+    for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) scratch[wid] = x;
+    __syncthreads();
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) x = max(x, scratch[w]);
+    __syncthreads();  // scratch may be reused
+    return x;
+}
+
 __device__ __forceinline__ int block_sum_int(int x, int* scratch) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;

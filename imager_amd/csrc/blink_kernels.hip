@@ -1485,8 +1485,10 @@ hipError_t launch_grid_rows_fft(const SetDesc& first, const SetDesc* sets, const
 // timestep per channel, as k_grid_rows_direct does it for the register
 // sides: one workgroup per image with the whole half plane (plan rows
 // 0 .. N/2 of an even or odd side) in LDS, every baseline walked in order
-// (direct_walk), then the rows transformed by teams of one wave and stored
-// row-major as the generic second pass reads them.  Sides whose half plane
+// (direct_walk), then rows 0 .. the highest holding a cell transformed by
+// teams of one wave and stored row-major as the generic second pass reads
+// them, and the image's row starts written (the rows above empty).  Sides
+// whose half plane
 // and team scratch fit in LDS (up to 186).  The count sums start at zero
 // (the direct set-up launch).
 static size_t direct_fft_lds_bytes(int n) {
@@ -1518,18 +1520,25 @@ k_grid_rows_direct_fft(SetDesc first, const SetDesc* __restrict__ sets,
     for (int i = threadIdx.x; i < (h + 1) * n; i += kDirectThreads) tile[i] = make_float2(0.f, 0.f);
     for (int i = threadIdx.x; i < n; i += kDirectThreads) twl[i] = tw[i];
     __syncthreads();
-    const int n_ok = direct_walk<2>(sd, img, ch, n, 0, h + 1, (int)threadIdx.x, kDirectThreads,
-                                    [&](int r, int c) { return &tile[r * n + c]; });
+    int top = -1;
+    const int n_ok = direct_walk<2>(
+        sd, img, ch, n, 0, h + 1, (int)threadIdx.x, kDirectThreads,
+        [&](int r, int c) { return &tile[r * n + c]; }, nullptr, &top);
+    const int n_rows = block_max_int(top, red) + 1;
     const int tot = block_sum_int(n_ok, red);
     if (threadIdx.x == 0 && tot) atomicAdd(&sd.isums[img], (uint32_t)tot);
     __syncthreads();
-    for (int r = t_id; r <= h; r += teams) {
+    // rows 0 .. n_rows - 1 (the highest holding a cell); the row starts tell
+    // the second pass that the others are empty
+    for (int r = t_id; r < n_rows; r += teams) {
         float2* row = tile + (size_t)r * n;
         const float2* res = fft_row<1, false>(row, tmp, plan, twl, lane, kDirectTeam);
         float2* dst = inter + ((int64_t)li * (h + 1) + r) * n;
         for (int i = lane; i < n; i += kDirectTeam) store_stream(&dst[i], res[i]);
         team_sync<false>();
     }
+    int* rsg = sd.row_start + (int64_t)ch * (n + 1);
+    for (int i = threadIdx.x; i <= n; i += kDirectThreads) rsg[i] = min(i, n_rows);
 }
 
 bool direct_half_plane_fits(int n) {

@@ -260,18 +260,17 @@ hipError_t launch_grid_rows_reg(const SetDesc& first, const SetDesc* sets, const
 int batch_uniform_row_tile(int n);
 // Direct row pass of dense single-timestep sets (no plan): one workgroup per
 // image with its whole half plane in LDS, for the register sides
-// batch_direct_supported (128, and 256 with a window of the plane, below)
-// and the generic sides that fit (up to 186); direct_row_pass_fits tells
-// which.  Every row of the half plane is written, so the sets' descriptors
-// carry all_rows, except at the sides direct_writes_row_starts names (256),
-// whose kernel stores the rows of the window that holds the image's cells
-// and writes the image's row starts (rows above the window empty) for the
-// second pass.  The count sums and phase coefficients come from
-// launch_direct_prep over the group's sets (max_work: the largest n_img or
-// n_bl among them), launched before.
+// batch_direct_supported (128, and 256 with a window of the plane: the
+// sides direct_uses_window names) and the generic sides that fit (up to
+// 186); direct_row_pass_fits tells which.  The kernels store the rows up to
+// the highest that holds a cell and write the image's row starts (the rows
+// above empty) for the second pass.  The count sums and phase coefficients
+// (and, for the window, the |u|, |v| maxima) come from launch_direct_prep
+// over the group's sets (max_work: the largest n_img or n_bl among them),
+// launched before.
 bool batch_direct_supported(int n);
 bool direct_row_pass_fits(int n);
-bool direct_writes_row_starts(int n);
+bool direct_uses_window(int n);
 hipError_t launch_grid_rows_direct(const SetDesc& first, const SetDesc* sets, const int* img_set,
                                    int n_sets, int n, int64_t gi0, int n_img, float* inter,
                                    const float* tw, hipStream_t stream);
