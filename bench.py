@@ -136,9 +136,8 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
                      baseline compacted, else the 32 B records) + the
                      transformed non-empty plan rows (8 N B each); on the
                      direct path (no plan): u, v, flags once (12 B per
-                     baseline) + the records once + every plan row 0 .. N/2
-                     (at 256^2, the window's rows: those up to the last that
-                     holds cells, which the second pass then reads)
+                     baseline) + the records once + the plan rows up to the
+                     last that holds cells (the rows the second pass reads)
       direct_prep    w and the phase coefficient per baseline, the count sums
                      (at 256^2 also u, v, flags for the |u|, |v| maxima)
       fft_cols       the non-empty plan rows (8 N B each) + the image (8 N^2 B)
@@ -151,9 +150,8 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
     n, n_bl = o.n_pixels, o.n_bl
     entries, nonempty = stats
     window = direct and n == DIRECT_WINDOW_SIDE
-    if kernel_size or (direct and not window):
-        # convolutional gridding and the direct row pass write and read every
-        # plan row 0 .. N/2 (the windowed one, the rows of its window)
+    if kernel_size:
+        # convolutional gridding writes and reads every plan row 0 .. N/2
         nonempty = np.full_like(nonempty, n // 2 + 1)
     ch = np.arange(n_img) % o.n_ch
     e_sum = float(entries[ch].sum())
