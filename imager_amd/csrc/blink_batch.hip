@@ -298,9 +298,10 @@ __device__ __forceinline__ void block_max2(float& a, float& b, float* scratch) {
     for (int w = 0; w < nw; w++) a = fmaxf(a, scratch[w]), b = fmaxf(b, scratch[nw + w]);
 }
 
-// Lanes per row of the windowed direct row pass's transforms: 4 values per
-// lane, so that a team keeps two rows in registers.
-constexpr int kWinLanes = 64;
+// Lanes per row of the windowed direct row pass's transforms: 8 values per
+// lane (radix 8, 8, 4: two exchanges), and a team keeps two rows in
+// registers (64 lanes, four values each, measured 2 % slower at 768 x 256^2).
+constexpr int kWinLanes = 32;
 
 // Transforms local rows 0 .. nr - 1 of an LDS tile of stride s and stores
 // them as plan rows r0 .. of image li of the row-pass buffer.  kFull: whole
