@@ -153,6 +153,42 @@ def test_cells_min_uv_cut_and_wrap():
     assert oracle.cell(9.0, 0.0, f, 1.0, 1.0, -np.inf, N)[0] == 1
 
 
+def test_cell_extent_is_the_cell_of_the_largest_coordinate():
+    """|cell| = |round((x / wavelength) / cell size)| never decreases with |x|,
+    so every gridded cell of a channel lies within the cells of the largest
+    unflagged |u| and |v| (the bound by which the direct row pass at 256^2
+    sizes its window of the half plane).  Random layouts of 8-63 antennas,
+    2-40 % of the baselines flagged, 50-300 MHz and oversampling 2.5-4 on a
+    side large enough that no cell wraps."""
+    rng = np.random.default_rng(12)
+    n = 8192
+
+    def unwrap(a):
+        return np.where(a >= n // 2, a - n, a)
+
+    for trial in range(16):
+        n_ant = int(rng.integers(8, 64))
+        scale = rng.uniform(5.0, 1500.0)
+        x, y, z = (rng.normal(0.0, s, n_ant) for s in (scale, scale, 1.0))
+        u, v, w = oracle.zenith_uvw(x, y, z)
+        flags = (oracle.baseline_flags(n_ant).astype(bool)
+                 | (rng.random(len(u)) < rng.uniform(0.02, 0.4))).astype(np.int32)
+        u_max, v_max = oracle.uv_max(u, v, flags)
+        du, dv = oracle.delta_uv(float(np.float32(rng.uniform(2.5, 4.0))), u_max, v_max, n)
+        freq = rng.uniform(50e6, 300e6, 3)
+        cl = oracle.cells(u, v, flags, freq, du, dv, -np.inf, n)
+        good = flags == 0
+        mu = float(np.abs(np.asarray(u, np.float32)[good]).max())
+        mv = float(np.abs(np.asarray(v, np.float32)[good]).max())
+        for ch, f in enumerate(freq):
+            ok = cl[ch, :, 0] >= 0
+            eu = abs(int(unwrap(np.array(oracle.cell(mu, 0.0, f, du, dv, -np.inf, n)[0]))))
+            ev = abs(int(unwrap(np.array(oracle.cell(0.0, mv, f, du, dv, -np.inf, n)[1]))))
+            assert eu < n // 2 and ev < n // 2
+            assert np.abs(unwrap(cl[ch, ok, 0])).max() == eu, (trial, ch)
+            assert np.abs(unwrap(cl[ch, ok, 1])).max() == ev, (trial, ch)
+
+
 # ----------------------------------------------------------------- gridding
 
 @pytest.fixture(scope="module")
