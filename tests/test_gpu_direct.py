@@ -103,7 +103,7 @@ def _buffers(o):
 
 
 @pytest.mark.parametrize("n,pol", [(128, "I"), (128, "YY"), (180, "XX"), (175, "V"),
-                                   (90, "XX"), (256, "XX"), (256, "V")])
+                                   (90, "XX"), (256, "XX"), (256, "V"), (256, "I")])
 def test_direct_sides(ctx, n, pol):
     o = _station(n, seed=500 + n)
     assert o.n_bl > 64 * (n // 2 + 1), "a dense grid"
@@ -392,3 +392,31 @@ def test_direct_dft_256_keeps_the_plan(ctx):
     assert "direct_prep" not in launches and "plan" in launches, launches
     assert oracle.peak_normalised_error(_host(imgs), ref)[0] < IMAGE_TOL
     assert np.array_equal(_host(sums), c)
+
+
+@pytest.mark.parametrize("n", [128, 180, 256])
+def test_direct_all_baselines_flagged(ctx, n):
+    """A direct set whose baselines are all flagged grids nothing: count sums
+    0 and every pixel NaN (0 * (1/0)), as the oracle's images; a second set
+    of the call, with its baselines, images normally (no rows of the empty
+    one's leak into it)."""
+    import dataclasses
+    a = _station(n, seed=640 + n)
+    b = dataclasses.replace(a, flags=np.ones_like(a.flags))
+    ra, ca = _reference(a, "XX")
+    rb, cb = _reference(b, "XX")
+    assert not cb.any() and np.isnan(rb).all()
+    ia, sa = _buffers(a)
+    ib, sb = _buffers(b)
+    sta, ka = _set(a, "XX", ia, sa)
+    stb, kb = _set(b, "XX", ib, sb)
+    for rep in range(2):
+        for t in (ia, ib):
+            t.fill_(0.0)
+        _say(f"all flagged beside a normal set, N={n}, rep {rep}")
+        with _env(BLINK_DIRECT=1):
+            ctx.image_batch([stb, sta])
+        assert np.array_equal(_host(sb), cb)
+        assert np.isnan(_host(ib).real).all()
+        assert oracle.peak_normalised_error(_host(ia), ra)[0] < IMAGE_TOL
+        assert np.array_equal(_host(sa), ca)
