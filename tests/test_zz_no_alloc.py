@@ -73,6 +73,32 @@ def test_noalloc_matches_image_batch_on_two_calls(ctx):
             assert oracle.peak_normalised_error(avg, ref_avg)[0] < IMAGE_TOL
 
 
+@pytest.mark.parametrize("n", [128, 180, 256])
+def test_noalloc_direct_row_pass(ctx, n):
+    """The 256-antenna station's dense single-timestep sets on the direct row
+    pass (BLINK_DIRECT=1: three images), whose set-up and row starts live in
+    the fixed region (at 256^2 with the window's |u|, |v| maxima): the
+    no-allocation call against blink_image_batch and the oracle, twice."""
+    import os
+    o = synth.eda2_station(n_ch=3, n_t=1, n_pixels=n, seed=1010 + n,
+                           freq=synth.mwa_fine_channels(3))
+    old = os.environ.get("BLINK_DIRECT")
+    os.environ["BLINK_DIRECT"] = "1"
+    try:
+        for rep in range(2):
+            (ref_img, ref_sums, _), (img, sums, _) = _both(ctx, o)
+            assert np.array_equal(sums, ref_sums)
+            assert oracle.peak_normalised_error(img, ref_img)[0] < IMAGE_TOL
+    finally:
+        if old is None:
+            os.environ.pop("BLINK_DIRECT", None)
+        else:
+            os.environ["BLINK_DIRECT"] = old
+    r32, _ = oracle.run_f32(o.vis, o.n_ant, o.u, o.v, o.w, o.flags, o.freq, o.cable, True, True,
+                            o.delta_u, o.delta_v, o.min_uv, n)
+    assert oracle.peak_normalised_error(img, r32)[0] < IMAGE_TOL
+
+
 def test_noalloc_refusals(ctx):
     import torch
     from imager_amd import capi
