@@ -150,9 +150,12 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
     n, n_bl = o.n_pixels, o.n_bl
     entries, nonempty = stats
     window = direct and n == DIRECT_WINDOW_SIDE
-    if kernel_size:
+    if kernel_size and not direct:
         # convolutional gridding writes and reads every plan row 0 .. N/2
         nonempty = np.full_like(nonempty, n // 2 + 1)
+    elif kernel_size:
+        # on the direct row pass, the rows up to the last cell's plus K/2
+        nonempty = np.minimum(nonempty + kernel_size // 2, n // 2 + 1)
     ch = np.arange(n_img) % o.n_ch
     e_sum = float(entries[ch].sum())
     r_sum = float(nonempty[ch].sum())
@@ -165,6 +168,8 @@ def physical_bytes(kernel: str, o, stats, n_img: int, transform: str = "fft",
     if kernel == "grid_rows_fft":
         vis_b = 8.0 if compacted(n_bl, n) else 32.0
         return 4.0 * e_sum + vis_b * n_bl * n_img + 8.0 * n * r_sum
+    if kernel == "grid_rows_kb" and direct:   # u, v, flags and records once, the rows
+        return 12.0 * n_bl + 32.0 * n_bl * n_img + 8.0 * n * r_sum
     if kernel == "grid_rows_kb":   # entries and records once (a lower bound), every row
         return 4.0 * e_sum + 32.0 * n_bl * n_img + 8.0 * n * r_sum
     if kernel == "fft_cols":

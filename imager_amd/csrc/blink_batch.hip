@@ -359,8 +359,10 @@ __device__ __forceinline__ void win_transform_rows(float2* tile, int r0, int nr,
 // channel, whose images share their row starts) the workgroup only sets
 // start[N] = -1, which the second pass never reads, and leaves the image to
 // k_grid_rows_direct_blocks, launched next.  A cell outside the window (not
-// expected: the bound is exact) does the same.  `cap`: tile elements in LDS.
-template <int N, int NT, bool kMulti>
+// expected: the bound is exact) does the same.  kKb: convolutional gridding
+// (kb_walk), the window widened by K/2 on every side for the footprints.
+// `cap`: tile elements in LDS.
+template <int N, int NT, bool kMulti, bool kKb>
 __global__ void __launch_bounds__(NT)
 k_grid_rows_direct_win(SetDesc first, const SetDesc* __restrict__ sets,
                        const int* __restrict__ img_set, int64_t gi0, float2* __restrict__ inter,
@@ -391,8 +393,9 @@ k_grid_rows_direct_win(SetDesc first, const SetDesc* __restrict__ sets,
         const double wl = wavelength_of(sd.g.freq_hz[ch]);
         const double eu = fabs(round(((double)mu / wl) / sd.du));
         const double ev = fabs(round(((double)mv / wl) / sd.dv));
-        er = sd.uv_swap ? eu : ev;
-        ec = sd.uv_swap ? ev : eu;
+        const double margin = kKb ? (double)(sd.kb_size / 2) : 0.0;
+        er = (sd.uv_swap ? eu : ev) + margin;
+        ec = (sd.uv_swap ? ev : eu) + margin;
     }
     const bool fits = sd.n_img == sd.g.n_ch && er < H && ec < H &&
                       (((int)er + 2) & ~1) * (int64_t)max(2 * (int)ec + 1, kMinStride) <= cap;
@@ -406,13 +409,15 @@ k_grid_rows_direct_win(SetDesc first, const SetDesc* __restrict__ sets,
     for (int i = tid; i < ((n_win + 1) & ~1) * s; i += NT) tile[i] = make_float2(0.f, 0.f);
     __syncthreads();
     bool outside = false;
-    const int n_ok = direct_walk<2, true>(
-        sd, img, ch, N, 0, n_win, tid, NT,
-        [&](int r, int c) -> float2* {
-            const int j = c <= e_c ? c : (c >= N - e_c ? c - c_shift : -1);
-            return j >= 0 ? &tile[r * s + j] : nullptr;
-        },
-        &outside);
+    auto at = [&](int r, int c) -> float2* {
+        const int j = c <= e_c ? c : (c >= N - e_c ? c - c_shift : -1);
+        return j >= 0 ? &tile[r * s + j] : nullptr;
+    };
+    int n_ok;
+    if constexpr (kKb)
+        n_ok = kb_walk<true>(sd, img, ch, N, 0, n_win, tid, NT, at, &outside);
+    else
+        n_ok = direct_walk<2, true>(sd, img, ch, N, 0, n_win, tid, NT, at, &outside);
     if (__syncthreads_or(outside ? 1 : 0)) {
         if (tid == 0) rsg[N] = -1;
         return;
@@ -431,7 +436,8 @@ k_grid_rows_direct_win(SetDesc first, const SetDesc* __restrict__ sets,
 // images of a channel with several timesteps see it too).  Workgroups take
 // the chunk's images in turn and return at once for the windowed ones.
 // (NT = 512: the walk and two-row transforms in one loop fit its registers.)
-template <int N, int NT, bool kMulti>
+// kKb: convolutional gridding (kb_walk).
+template <int N, int NT, bool kMulti, bool kKb>
 __global__ void __launch_bounds__(NT)
 k_grid_rows_direct_blocks(SetDesc first, const SetDesc* __restrict__ sets,
                           const int* __restrict__ img_set, int64_t gi0, int n_img,
@@ -461,8 +467,11 @@ k_grid_rows_direct_blocks(SetDesc first, const SetDesc* __restrict__ sets,
             __syncthreads();  // the previous block's transforms are done with the tile
             for (int i = tid; i < ((nr + 1) & ~1) * N; i += NT) tile[i] = make_float2(0.f, 0.f);
             __syncthreads();
-            n_ok += direct_walk<2>(sd, img, ch, N, r0, nr, tid, NT,
-                                   [&](int r, int c) { return &tile[r * N + c]; });
+            auto at = [&](int r, int c) { return &tile[r * N + c]; };
+            if constexpr (kKb)
+                n_ok += kb_walk<false>(sd, img, ch, N, r0, nr, tid, NT, at, nullptr);
+            else
+                n_ok += direct_walk<2>(sd, img, ch, N, r0, nr, tid, NT, at);
             __syncthreads();
             win_transform_rows<N, NT, true>(tile, r0, nr, N, 0, li, inter, twl,
                                             sd.cached_inter != 0);
@@ -1133,7 +1142,7 @@ bool batch_direct_supported(int n) {
 
 bool direct_uses_window(int n) { return n == kDirectWinSide; }
 
-template <int N, int NT>
+template <int N, int NT, bool kKb>
 static hipError_t launch_rows_direct_win_t(const SetDesc& first, const SetDesc* sets,
                                            const int* img_set, int n_sets, int64_t gi0,
                                            int n_img, float* inter, const float* tw,
@@ -1143,15 +1152,15 @@ static hipError_t launch_rows_direct_win_t(const SetDesc& first, const SetDesc* 
     const int64_t cap = g_lds_limit / (int64_t)sizeof(float2) - N;
     if (((cap / N) & ~1) < 2) return hipErrorInvalidConfiguration;
     const size_t lds = sizeof(float2) * (size_t)(cap + N);
-    auto k = n_sets == 1 ? k_grid_rows_direct_win<N, NT, false>
-                         : k_grid_rows_direct_win<N, NT, true>;
+    auto k = n_sets == 1 ? k_grid_rows_direct_win<N, NT, false, kKb>
+                         : k_grid_rows_direct_win<N, NT, true, kKb>;
     hipLaunchKernelGGL(k, dim3((unsigned)n_img), dim3(NT), lds, stream, first, sets, img_set, gi0,
                        (float2*)inter, (const float2*)tw, (int)cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // the images the window does not hold: a workgroup per CU takes them in turn
-    auto kb = n_sets == 1 ? k_grid_rows_direct_blocks<N, kDirectBlockThreads, false>
-                          : k_grid_rows_direct_blocks<N, kDirectBlockThreads, true>;
+    auto kb = n_sets == 1 ? k_grid_rows_direct_blocks<N, kDirectBlockThreads, false, kKb>
+                          : k_grid_rows_direct_blocks<N, kDirectBlockThreads, true, kKb>;
     const int blocks = (int)std::min<int64_t>(n_img, std::max(1, device_cu_count()));
     hipLaunchKernelGGL(kb, dim3((unsigned)blocks), dim3(kDirectBlockThreads), lds, stream, first,
                        sets, img_set,
@@ -1164,8 +1173,12 @@ hipError_t launch_grid_rows_direct(const SetDesc& first, const SetDesc* sets, co
                                    const float* tw, hipStream_t stream) {
     if (n_img == 0) return hipSuccess;
     if (n == kDirectWinSide)
-        return launch_rows_direct_win_t<kDirectWinSide, kDirectWinThreads>(
-            first, sets, img_set, n_sets, gi0, n_img, inter, tw, stream);
+        return first.kb_size > 0
+                   ? launch_rows_direct_win_t<kDirectWinSide, kDirectWinThreads, true>(
+                         first, sets, img_set, n_sets, gi0, n_img, inter, tw, stream)
+                   : launch_rows_direct_win_t<kDirectWinSide, kDirectWinThreads, false>(
+                         first, sets, img_set, n_sets, gi0, n_img, inter, tw, stream);
+    if (first.kb_size > 0) return hipErrorNotSupported;  // convolutional: the window side only
 #define X(NN, T) \
     if (n == NN) \
         return launch_rows_direct_t<NN, T>(first, sets, img_set, n_sets, gi0, n_img, inter, tw, \
@@ -1206,10 +1219,16 @@ hipError_t set_batch_kernel_attributes() {
     set((const void*)k_grid_rows_direct<NN, T, true>);
     BLINK_DIRECT_VARIANTS(X)
 #undef X
-    set((const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, false>);
-    set((const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, true>);
-    set((const void*)k_grid_rows_direct_blocks<kDirectWinSide, kDirectBlockThreads, false>);
-    set((const void*)k_grid_rows_direct_blocks<kDirectWinSide, kDirectBlockThreads, true>);
+    for (const void* f :
+         {(const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, false, false>,
+          (const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, true, false>,
+          (const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, false, true>,
+          (const void*)k_grid_rows_direct_win<kDirectWinSide, kDirectWinThreads, true, true>,
+          (const void*)k_grid_rows_direct_blocks<kDirectWinSide, kDirectBlockThreads, false, false>,
+          (const void*)k_grid_rows_direct_blocks<kDirectWinSide, kDirectBlockThreads, true, false>,
+          (const void*)k_grid_rows_direct_blocks<kDirectWinSide, kDirectBlockThreads, false, true>,
+          (const void*)k_grid_rows_direct_blocks<kDirectWinSide, kDirectBlockThreads, true, true>})
+        set(f);
 #define X(NN, T)                                  \
     set((const void*)k_cols_reg<NN, T, false>);   \
     set((const void*)k_cols_reg<NN, T, true>);

@@ -998,7 +998,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
         const bool beside_main = bp.groups.size() > 1 && &g - bp.groups.data() != bp.main;
         if (bp.direct_mode < 0 && 2 * g.n_img < cus && !beside_main) continue;
         // (the DFT transforms' half plane too: the same walk, no row FFT)
-        if (g.uniform || g.kb_size || bp.direct_mode == 0) continue;
+        if (g.uniform || bp.direct_mode == 0) continue;
+        // convolutional gridding: the windowed direct row pass (256^2) only
+        if (g.kb_size && !(g.kb_blocked && direct_uses_window(g.n))) continue;
         if (is_dft(g.transform) ? !direct_half_plane_fits(g.n) : !direct_row_pass_fits(g.n))
             continue;
         bool ok = true;
@@ -1266,7 +1268,9 @@ static int build_batch(blink_ctx* ctx, const blink_image_set* sets, int n_sets, 
             d.kb_size = st.kernel_size;
             d.kb_table = bp.sc[i].kb_table;
             // (the FFT groups' direct row passes write the row starts instead)
-            d.all_rows = st.kernel_size > 0 || (g.direct && is_dft(g.transform)) ? 1 : 0;
+            d.all_rows = (st.kernel_size > 0 && !g.direct) || (g.direct && is_dft(g.transform))
+                             ? 1
+                             : 0;
             d.cached_inter = g.cached_inter ? 1 : 0;
             // the windowed direct row pass's |u|, |v| maxima: in the row
             // lists, which no direct set uses (n_ch (N + 3) >= 2 kUvMaxParts)
@@ -1597,7 +1601,10 @@ static int enqueue_chunks(blink_ctx* ctx, const blink_image_set* sets, const Bat
         // the buffer's previous chunk (k - 2) must have been read
         if (two && k >= 2)
             HIP_TRY(hipStreamWaitEvent(s, ctx->pass_ev[2 * ((k - 2) % kPassEvents) + 1], 0));
-        if (g.direct && g.reg)
+        if (g.direct && g.kb_size)
+            LAUNCH("grid_rows_kb", s,
+                   launch_grid_rows_direct(first, sd, is, ns, n, gi0, cnt, buf, tw, s));
+        else if (g.direct && g.reg)
             LAUNCH("grid_rows_fft", s,
                    launch_grid_rows_direct(first, sd, is, ns, n, gi0, cnt, buf, tw, s));
         else if (g.direct)

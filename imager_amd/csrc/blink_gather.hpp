@@ -256,6 +256,86 @@ __device__ __forceinline__ int direct_walk(const SetDesc& sd, int64_t img, int c
     return n_ok;
 }
 
+// Column (or row) of footprint coordinate p for sign s, as calc_xy_grid2
+// computes it (src/pacer_imager.cpp:716-750): index s*p + N/2, then the
+// half swap; coordinates that land outside the grid are dropped.
+__device__ __forceinline__ bool kb_cell(int p, int s, int n, int& x) {
+    const int c = n / 2;
+    const int idx = s * p + c;
+    x = idx < c ? c + idx : idx - c;
+    return x >= 0 && x < n;
+}
+
+// The direct row passes' walk for convolutional gridding (transposed plan:
+// rows are u): every gridded baseline of the image adds its K x K footprint
+// around its nearest cell with value V and, with conj(V), around the mirrored
+// cell (as k_grid_rows_kb adds them per plan entry: the kernel rows from the
+// 1023-times oversampled fractional position, weights in FP64), keeping the
+// cells of plan rows r0 .. r0 + nr - 1 (of 0 .. N/2) through at(row - r0,
+// position).  kWin: rows 0 .. nr - 1 are a window that every cell is expected
+// to fall in; a cell of rows nr .. N/2, or a position at() has no slot for
+// (nullptr), sets *outside.  Returns the nearest-cell counts as direct_walk
+// does (two per gridded visibility, split by row).
+template <bool kWin, class At>
+__device__ __forceinline__ int kb_walk(const SetDesc& sd, int64_t img, int ch, int n, int r0,
+                                       int nr, int tid, int nt, At at, bool* outside) {
+#pragma clang fp contract(off)
+    const GatherArgs& g = sd.g;
+    const double nu = g.freq_hz[ch];
+    const double wl = wavelength_of(nu);
+    const float2* rec = (const float2*)g.vis + 4 * (img * g.n_bl);
+    const int p0 = g.pol == 1 ? 3 : (g.pol == 3 ? 1 : 0);
+    const int p1 = g.pol == 3 ? 2 : 3;
+    const int K = sd.kb_size, mid = K / 2, h = n / 2;
+    int n_ok = 0;
+    for (int bl = tid; bl < g.n_bl; bl += nt) {
+        if (sd.flags[bl]) continue;
+        float2 v = select_pol2(rec[4 * (size_t)bl + p0],
+                               g.pol >= 2 ? rec[4 * (size_t)bl + p1] : make_float2(0.f, 0.f),
+                               g.pol);
+        if (g.corrections) v = cmul(v, unit_phasor(((g.phase_coef[bl] * nu) / kVelLight)));
+        if (!vis_ok(v)) continue;
+        int up, vp;
+        if (!cell_of(sd.u_m[bl], sd.v_m[bl], wl, sd.du, sd.dv, sd.min_uv, n, up, vp)) continue;
+        // the nearest cell's plan row and its mirror's: the counts
+        const int q1 = wrap_index(up, n), q2 = q1 == 0 ? 0 : n - q1;
+        if ((unsigned)(q1 - r0) < (unsigned)nr && q1 <= h) n_ok += (q1 == 0 || 2 * q1 == n) ? 1 : 2;
+        if ((unsigned)(q2 - r0) < (unsigned)nr && q2 <= h) n_ok += (q2 == 0 || 2 * q2 == n) ? 1 : 2;
+        const double ue = (double)sd.u_m[bl] / wl / sd.du;
+        const double ve = (double)sd.v_m[bl] / wl / sd.dv;
+        int kx = (int)round((ue - (double)up) * 1023.0);
+        int ky = (int)round((ve - (double)vp) * 1023.0);
+        kx = (kx + 1534) % 1023;
+        ky = (ky + 1534) % 1023;
+        const double* xk = sd.kb_table + (int64_t)kx * K;
+        const double* yk = sd.kb_table + (int64_t)ky * K;
+        for (int sgn = 1; sgn >= -1; sgn -= 2) {
+            for (int c = 0; c < K; c++) {
+                int x;
+                if (!kb_cell(up - mid + c, sgn, n, x) || x > h) continue;
+                if ((unsigned)(x - r0) >= (unsigned)nr) {
+                    if constexpr (kWin) *outside = true;
+                    continue;
+                }
+                const double wx = xk[c];
+                for (int j = 0; j < K; j++) {
+                    int y;
+                    if (!kb_cell(vp - mid + j, sgn, n, y)) continue;
+                    float2* d = at(x - r0, y);
+                    if (!d) {
+                        *outside = true;
+                        continue;
+                    }
+                    const double w = yk[j] * wx;
+                    atomicAdd(&d->x, (float)((double)v.x * w));
+                    atomicAdd(&d->y, (float)((double)sgn * (double)v.y * w));
+                }
+            }
+        }
+    }
+    return n_ok;
+}
+
 // Phase coefficient of baseline bl (metres): w - (L[a2] - L[a1]), each term
 // when its correction is on; exp(+2 pi i coef nu / c) is the product of the
 // geometric (src/gpu/corrections_gpu.cpp:61-63) and cable (:27-28) rotations.

@@ -279,16 +279,6 @@ hipError_t launch_grid_atomic(const float* vis, int n_ant, int n_ch, int n_t, co
 
 // --------------------------------------- (b) convolutional (Kaiser-Bessel) gridding
 
-// Column (or row) of footprint coordinate p for sign s, as calc_xy_grid2
-// computes it (src/pacer_imager.cpp:716-750): index s*p + N/2, then the
-// half swap; coordinates that land outside the grid are dropped.
-__device__ __forceinline__ bool kb_cell(int p, int s, int n, int& x) {
-    const int c = n / 2;
-    const int idx = s * p + c;
-    x = idx < c ? c + idx : idx - c;
-    return x >= 0 && x < n;
-}
-
 // One thread per (t, ch, bl).  The visibility and its conjugate are spread
 // over K x K cells with weights kt[ky][j] * kt[kx][i] (double), the kernel
 // row chosen from the 1023-times oversampled fractional cell position as

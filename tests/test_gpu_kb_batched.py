@@ -115,3 +115,85 @@ def test_kb_batch_refusals(ctx):
         ctx.image_batch([mk(kernel_size=7, weighting="U")])
     with pytest.raises(capi.BlinkUnsupported):
         ctx.image_batch([mk(kernel_size=7, transform=capi.TRANSFORM_DFT_BF16X3)])
+
+
+class _env:
+    """Sets environment variables for the calls inside the block."""
+
+    def __init__(self, **kv):
+        self.kv = kv
+        self.old = {}
+
+    def __enter__(self):
+        import os
+        for k, v in self.kv.items():
+            self.old[k] = os.environ.get(k)
+            os.environ[k] = str(v)
+
+    def __exit__(self, *exc):
+        import os
+        for k, v in self.old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("K,mode,freq", [(7, 1, (151.25e6, 159.375e6, 171.0e6)),
+                                         (5, 2, (169.6e6, 185.0e6, 200.3e6)),
+                                         (7, 1, (240e6, 300e6, 160e6))])
+def test_kb_direct_window_256(ctx, K, mode, freq):
+    """The 256-antenna station at 256^2 (dense, one timestep per channel) on
+    the windowed direct row pass (BLINK_DIRECT=1: three images), the window
+    widened by K/2 for the footprints; the 240 and 300 MHz channels need more
+    than the LDS holds and take blocks of whole rows.  Images against the
+    oracle's convolutional gridding within 1e-5, exact count sums, and the
+    plan path's images (BLINK_DIRECT=0) within the same tolerance."""
+    import torch
+    o = synth.eda2_station(n_ch=3, n_t=1, n_pixels=256, seed=700 + K, freq=np.array(freq),
+                           height_sigma_m=1.0, bl_flag_fraction=0.1)
+    ref, c = _reference(o, K, mode == 1)
+    images = torch.full((1, 3, 256, 256), float("nan"), dtype=torch.complex64, device="cuda")
+    sums = torch.empty(3, dtype=torch.float64, device="cuda")
+    st, keep = _set(o, images, sums, K, mode)
+    out = {}
+    for direct in ("1", "0"):
+        with _env(BLINK_DIRECT=direct):
+            for rep in range(2):
+                ctx.profile_enable(rep == 1)
+                ctx.profile_reset()
+                images.fill_(float("nan"))
+                print(f"KB direct={direct} 256^2 K={K} mode={mode} rep {rep}", flush=True)
+                ctx.image_batch([st])
+                got = _host(images)
+                err = oracle.peak_normalised_error(got, ref)[0]
+                print(f"  vs FP32 oracle {err:.2e}", flush=True)
+                assert err < IMAGE_TOL
+                assert np.array_equal(_host(sums), c.reshape(3, -1).sum(1).astype(np.float64))
+                if rep == 1:
+                    assert ("direct_prep" in ctx.profile()) == (direct == "1")
+        out[direct] = _host(images).copy()
+    ctx.profile_enable(False)
+    assert oracle.peak_normalised_error(out["1"], out["0"])[0] < IMAGE_TOL
+
+
+def test_kb_direct_chosen_for_many_images(ctx):
+    """Without BLINK_DIRECT, 160 channels of the station at 256^2 with K = 7
+    take the direct row pass by themselves."""
+    import torch
+    o = synth.eda2_station(n_ch=160, n_t=1, n_pixels=256, seed=720,
+                           freq=synth.mwa_fine_channels(160), height_sigma_m=1.0)
+    ref, c = _reference(o, 7, True)
+    images = torch.empty((1, 160, 256, 256), dtype=torch.complex64, device="cuda")
+    sums = torch.empty(160, dtype=torch.float64, device="cuda")
+    st, keep = _set(o, images, sums, 7, 1)
+    ctx.profile_enable(True)
+    ctx.profile_reset()
+    try:
+        print("KB auto choice, 160 images of 256^2", flush=True)
+        ctx.image_batch([st])
+        assert "direct_prep" in ctx.profile()
+    finally:
+        ctx.profile_enable(False)
+    assert oracle.peak_normalised_error(_host(images), ref)[0] < IMAGE_TOL
+    assert np.array_equal(_host(sums), c.reshape(160, -1).sum(1).astype(np.float64))
